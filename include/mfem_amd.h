@@ -1,0 +1,201 @@
+/*
+ * mfem_amd.h -- C ABI of the MI355X (gfx950) per-timestep soft-body solve.
+ *
+ * Drop-in boundary for warp-mfem's RefinementSolver.step
+ * (src/mfem/refinement/solver.py:429-579 in the reference).  The reference is
+ * pure Python/Warp, so no C FFI exists there; these are the entry points a
+ * ctypes binding of that path binds (see INTEGRATION.md for the stub).  The
+ * Python mirror of the reference's solver API lives in
+ * warp-mfem_amd/mfem_amd/solver.py and calls exactly these functions.
+ *
+ * Conventions
+ *   - Every pointer inside mfem_solver is a device pointer the caller owns
+ *     (the Python host allocates them as torch tensors on the GPU).  The
+ *     library allocates nothing.
+ *   - Every entry point enqueues work on `stream` (a hipStream_t, passed as
+ *     void*) and returns without synchronising.
+ *   - Return value: 0 on success, <0 on error (MFEM_E*); the message is
+ *     available from mfem_last_error() on the calling thread.
+ *   - Device-side anomalies (grid-barrier timeout, CG not converged, line
+ *     search rejected) do not fail the call: they are recorded in
+ *     status[] / stats[] (read them after synchronising), as the reference
+ *     ignores CG non-convergence (solver.py:607) and line-search failure
+ *     (line_search.py:143-161).
+ */
+#ifndef MFEM_AMD_H
+#define MFEM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFEM_OK 0
+#define MFEM_EINVAL -1   /* bad argument / inconsistent sizes */
+#define MFEM_EHIP -2     /* a HIP runtime call failed */
+#define MFEM_ECAPACITY -3 /* problem larger than the persistent kernels support */
+
+#define MFEM_SHAPE_PLANE 0
+#define MFEM_SHAPE_CAPSULE 1
+
+#define MFEM_STATS_PER_ITER 8 /* cg_iters, cg_relres, ls_trials, ls_accepted, alpha, f0, f1, restarts */
+#define MFEM_MAX_STATS_ITERS 64
+#define MFEM_RED_MAX_BLOCKS 2048
+
+/* status[] words */
+#define MFEM_STATUS_ERRBITS 0 /* bit 0: grid barrier timed out */
+#define MFEM_STATUS_EPOCH 1   /* grid-barrier epoch counter (internal) */
+
+/*
+ * Solver description: sizes, parameters and device buffers.
+ * Field meanings follow RefinementSolver's constructor kwargs
+ * (src/mfem/refinement/solver.py:56-128, 241-245) and its buffers
+ * (solver.py:170-304, additional_state.py:17-110, contact.py:778-862).
+ */
+typedef struct mfem_solver {
+  /* sizes */
+  int32_t n_verts;
+  int32_t n_tets;
+  int32_t n_tris;
+  int32_t n_shapes;
+  int32_t n_chunks; /* ceil(n_verts / 64): SELL-64 row chunks of the system matrix */
+  int32_t n_slots;  /* total block slots over all chunks (each slot = 64 lanes) */
+  /* integer parameters */
+  int32_t energy;     /* 0 = ARAP, 1 = Neo-Hookean (solver.py:108-122) */
+  int32_t iterations; /* Newton iterations per step */
+  int32_t cg_maxiter;
+  int32_t precond;     /* block-Jacobi on/off (solver.py:128) */
+  int32_t line_search; /* (solver.py:185) */
+  int32_t ls_max;      /* line_search_max_iterations */
+  int32_t tri_contact; /* triangle-vs-capsule barrier (contact.py:417-775) */
+  int32_t friction_on; /* any shape with mu > 0 (contact.py:937-947) */
+  /* float parameters */
+  float cg_tol;
+  float ls_alpha0;
+  float ls_tau;
+  float ls_c;
+  float ls_thr;
+  float sing_thr; /* preconditioner_singular_threshold */
+  float attach_k; /* attachment_stiffness */
+  float d0, d1, kappa, eps_v;
+  float qb[3];      /* barrier(d0), barrier'(d0), barrier''(d0) */
+  float gravity[3]; /* model.gravity */
+  float pad0;
+  /* model / topology (device) */
+  const int32_t* tets;      /* [n_tets][4] */
+  const float* dminv;       /* [n_tets][9] tet_poses, row-major */
+  const float* materials;   /* [n_tets][3] mu, lambda, damping */
+  const float* vol;         /* [n_tets] signed rest volume 1/(6 det DmInv) */
+  const float* mass;        /* [n_verts] particle_mass */
+  const float* inv_mass;    /* [n_verts] particle_inv_mass */
+  const float* pin_pos;     /* [n_verts][3] attachment target (only where inv_mass == 0) */
+  const int32_t* tris;      /* [n_tris][3] surface triangles */
+  const int32_t* shape_type; /* [n_shapes] */
+  const float* shape_xf;     /* [n_shapes][7] world pose: p(3), q(4) xyzw */
+  const float* shape_scale;  /* [n_shapes][3] capsule radius, half height */
+  const float* shape_mu;     /* [n_shapes] friction coefficient */
+  /* system-matrix pattern: SELL-64 (64 rows per chunk, slot-major) */
+  const int32_t* chunk_ptr;   /* [n_chunks + 1] first slot of each chunk */
+  const int32_t* sell_col;    /* [n_slots][64] column vertex of each lane (padding: 0) */
+  float* sell_val;            /* [n_slots][9][64] 3x3 block values, lane-minor */
+  const int32_t* diag_lane;   /* [n_verts] lane-slot index (slot*64 + lane) of the diagonal block */
+  const int32_t* contrib_ptr; /* [n_slots*64 + 1] per lane-slot range into contrib_src */
+  const int32_t* contrib_src; /* (pair*n_tets + tet) << 1 | transpose; pair indexes the 10 tet blocks */
+  const int32_t* vinc_ptr;    /* [n_verts + 1] vertex -> incident (tet, corner) range */
+  const int32_t* vinc_src;    /* tet*4 + corner */
+  /* state (device) */
+  float* x;       /* [n_verts][3] particle_q being solved (state_out) */
+  float* v;       /* [n_verts][3] particle_qd (state_out) */
+  float* s;       /* [n_tets][6] tet_stretch */
+  float* lam;     /* [n_tets][6] tet_lambda */
+  float* x_tilde; /* [n_verts][3] predictor */
+  float* x_start; /* [n_verts][3] start-of-step positions (friction anchor, velocity) */
+  float* lag_n;   /* [n_verts][3] lagged contact normal */
+  float* lag_fn;  /* [n_verts] lagged normal force */
+  int32_t* lag_sid; /* [n_verts] lagged contact shape */
+  /* per-tet work */
+  float* G;     /* [n_tets][72] constraint Jacobian, 6x12 row-major */
+  float* Hl;    /* [n_tets][21] H_lambda, packed upper triangle */
+  float* gl;    /* [n_tets][6] g_lambda */
+  float* Hsinv; /* [n_tets][21] (H_s)^-1, packed upper triangle */
+  float* gs;    /* [n_tets][6] dPsi/ds */
+  float* psi;   /* [n_tets] elastic energy (solver._elastic_energy) */
+  float* Kb;    /* [10][n_tets][9] unique 3x3 blocks of G^T H_lambda G */
+  float* fe;    /* [n_tets][12] G^T H_lambda g_lambda */
+  /* per-vertex work */
+  float* gx;    /* [n_verts][3] kinetic + attachment gradient */
+  float* gB;    /* [n_verts][3] barrier gradient */
+  float* gD;    /* [n_verts][3] friction gradient */
+  float* Hv;    /* [n_verts][9] barrier + friction Hessian block */
+  float* dist;  /* [n_verts] contact distance */
+  float* eB;    /* [n_verts] barrier energy */
+  float* eD;    /* [n_verts] friction energy */
+  float* g;     /* [n_verts][3] Newton right-hand side */
+  double* dx;   /* [n_verts][3] PCG solution */
+  float* ds;    /* [n_tets][6] stretch update */
+  /* persistent-kernel scratch */
+  float* pz;    /* [n_chunks*64][3] published PCG direction */
+  double* px;   /* [n_chunks*64][3] published PCG iterate (residual check) */
+  double* red;  /* [2][MFEM_RED_MAX_BLOCKS][4] grid reduction partials */
+  uint32_t* flags; /* [MFEM_RED_MAX_BLOCKS] grid barrier arrival epochs */
+  int32_t* status; /* [8] see MFEM_STATUS_* */
+  double* stats;   /* [MFEM_MAX_STATS_ITERS][MFEM_STATS_PER_ITER] per Newton iteration */
+} mfem_solver;
+
+/* ABI self-description (checked against the Python mirror by the tests) */
+int64_t mfem_solver_sizeof(void);
+int64_t mfem_solver_offsetof(const char* field);
+int mfem_version(void);
+const char* mfem_last_error(void);
+
+/* AdditionalState.from_model stretch init: s = sym6(S(F(x)))
+   (src/mfem/refinement/kernels.py:55-75, additional_state.py:88-97) */
+int mfem_init_stretch(const mfem_solver* S, void* stream);
+
+/* Start of step: predictor x~ = x + v dt + g dt^2/2, friction anchor and
+   lagged normal force (kernels.py:77-91; solver.py:499-520;
+   contact.py:937-967). */
+int mfem_begin_step(const mfem_solver* S, float dt, void* stream);
+
+/* _update_system (solver.py:633-639): kinetic + attachment gradient
+   (kernels.py:172-185, solver.py:946-947), vertex contact + friction
+   (contact.py:969-1048), elastic energy / gradient / PSD Hessian
+   (elastic_energy.py), constraints c, G (kernels.py:131-169) and the Schur
+   blocks H_lambda, g_lambda (kernels.py:189-249). */
+int mfem_update_system(const mfem_solver* S, float dt, void* stream);
+
+/* _assemble_system (solver.py:641-799): H into the SELL pattern, g. */
+int mfem_assemble(const mfem_solver* S, float dt, void* stream);
+
+/* _global_solve (solver.py:581-609): block-Jacobi PCG on H dx = g from the
+   guess x~ - x (solver.py:530-531), FP64 reductions, one persistent launch.
+   Writes stats[iter][0..1] and stats[iter][7]. */
+int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream);
+
+/* _local_solve (solver.py:611-631, kernels.py:252-283); lam <- new lambda. */
+int mfem_local_solve(const mfem_solver* S, void* stream);
+
+/* _line_search (solver.py:871-928, line_search.py:76-163), FP64 merit sums,
+   one persistent launch; or x += dx, s += ds when line_search == 0
+   (solver.py:555-557).  Writes stats[iter][2..6]. */
+int mfem_line_search(const mfem_solver* S, float dt, int newton_iter, void* stream);
+
+/* Velocity v = (x - x_start) / dt (kernels.py:351-363, solver.py:565-577). */
+int mfem_end_step(const mfem_solver* S, float dt, void* stream);
+
+/* RefinementSolver.step without refinement (solver.py:429-579): all of the
+   above in order, no host synchronisation. */
+int mfem_step(const mfem_solver* S, float dt, void* stream);
+
+/* y = H x with the assembled SELL matrix (warp.sparse.bsr_mv). */
+int mfem_spmv(const mfem_solver* S, const float* x, float* y, void* stream);
+
+/* Line-search merit f(x + alpha dx, s + alpha ds) with lambda fixed
+   (solver.py:803-869) into out[0] (device double). */
+int mfem_merit(const mfem_solver* S, float dt, float alpha, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFEM_AMD_H */

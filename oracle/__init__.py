@@ -1,0 +1,300 @@
+"""CPU oracle for the per-timestep soft-body solve (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+load this package, and only as the checker / CPU baseline: the product path
+(``mfem_amd``) never imports it.
+
+``Oracle`` wraps ``liboracle64.so`` (FP64) or ``liboracle32.so`` (FP32), both
+built from ``mfem_oracle.c`` by ``oracle/Makefile``.  Each stage function
+mirrors one phase of ``RefinementSolver.step``
+(src/mfem/refinement/solver.py:429-579 of the reference).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+_LIBS: dict[int, ctypes.CDLL] = {}
+
+
+def build() -> None:
+    """Compile both oracle variants (gcc, seconds)."""
+    subprocess.run(["make", "-s", "-B", "-C", _DIR], check=True)
+
+
+def _lib(bits: int) -> ctypes.CDLL:
+    if bits in _LIBS:
+        return _LIBS[bits]
+    path = os.path.join(_DIR, f"liboracle{bits}.so")
+    if not os.path.exists(path):
+        build()
+    lib = ctypes.CDLL(path)
+    real = ctypes.c_double if bits == 64 else ctypes.c_float
+    P = ctypes.c_void_p
+    lib.orc_create.restype = P
+    lib.orc_create.argtypes = [ctypes.c_int] * 5
+    lib.orc_set_param.argtypes = [P, ctypes.c_char_p, ctypes.c_double]
+    lib.orc_set_param.restype = ctypes.c_int
+    lib.orc_get_param.argtypes = [P, ctypes.c_char_p]
+    lib.orc_get_param.restype = ctypes.c_double
+    lib.orc_buf.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]
+    lib.orc_buf.restype = P
+    lib.orc_finalize.argtypes = [P]
+    lib.orc_finalize.restype = ctypes.c_int
+    for name in ("orc_init_stretch", "orc_local_solve", "orc_destroy"):
+        getattr(lib, name).argtypes = [P]
+        getattr(lib, name).restype = None
+    for name in ("orc_begin_step", "orc_update_system", "orc_assemble", "orc_step"):
+        getattr(lib, name).argtypes = [P, real]
+        getattr(lib, name).restype = None
+    lib.orc_global_solve.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+    lib.orc_global_solve.restype = ctypes.c_int
+    lib.orc_line_search.argtypes = [P, real, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+    lib.orc_line_search.restype = ctypes.c_int
+    lib.orc_merit.argtypes = [P, P, P, real]
+    lib.orc_merit.restype = ctypes.c_double
+    lib.orc_spmv.argtypes = [P, P, P]
+    lib.orc_spmv.restype = None
+    lib.orc_real_size.restype = ctypes.c_int
+    _LIBS[bits] = lib
+    return lib
+
+
+# RefinementSolver keyword (src/mfem/refinement/solver.py:56-128, 241-245) -> oracle parameter
+_PARAM_NAMES = {
+    "cg_max_iterations": "cg_maxiter", "cg_tolerance": "cg_tol", "preconditioner": "precond",
+    "line_search_max_iterations": "ls_max", "line_search_alpha0": "ls_alpha0", "line_search_tau": "ls_tau",
+    "line_search_c": "ls_c", "line_search_threshold": "ls_thr", "attachment_stiffness": "attach_k",
+    "contact_d0": "d0", "contact_d1": "d1", "contact_stiffness": "kappa", "friction_eps_v": "eps_v",
+    "preconditioner_singular_threshold": "sing_thr",
+}
+
+
+def _real(bits: int):
+    return np.float64 if bits == 64 else np.float32
+
+
+class Kernels:
+    """Per-element oracle functions (no solver context)."""
+
+    def __init__(self, bits: int = 64):
+        self.bits = bits
+        self.lib = _lib(bits)
+        self.dt = _real(bits)
+
+    def _arr(self, a, n=None):
+        a = np.ascontiguousarray(np.asarray(a, dtype=self.dt).reshape(-1))
+        if n is not None:
+            assert a.size == n
+        return a
+
+    def _p(self, a):
+        return a.ctypes.data_as(ctypes.c_void_p)
+
+    def svd3(self, F):
+        F = self._arr(F, 9)
+        U, s, V = np.zeros(9, self.dt), np.zeros(3, self.dt), np.zeros(9, self.dt)
+        self.lib.orc_svd3(self._p(F), self._p(U), self._p(s), self._p(V))
+        return U.reshape(3, 3), s, V.reshape(3, 3)
+
+    def tet_constraints(self, x12, dminv, s6):
+        x12, D, s6 = self._arr(x12, 12), self._arr(dminv, 9), self._arr(s6, 6)
+        c, G, S6 = np.zeros(6, self.dt), np.zeros(72, self.dt), np.zeros(6, self.dt)
+        self.lib.orc_tet_constraints(self._p(x12), self._p(D), self._p(s6), self._p(c), self._p(G), self._p(S6))
+        return c, G.reshape(6, 12), S6
+
+    def stretch_gradient(self, F):
+        F = self._arr(F, 9)
+        U, s, V = self.svd3(F)
+        U, s, V = self._arr(U), self._arr(s), self._arr(V)
+        out = np.zeros(54, self.dt)
+        self.lib.orc_stretch_gradient(self._p(F), self._p(U), self._p(s), self._p(V), self._p(out))
+        return out.reshape(6, 9)
+
+    def psd_fix(self, A):
+        A = np.asarray(A)
+        n = A.shape[0]
+        a = self._arr(A, n * n)
+        fixed, inv = np.zeros(n * n, self.dt), np.zeros(n * n, self.dt)
+        self.lib.orc_psd_fix(self._p(a), ctypes.c_int(n), self._p(fixed), self._p(inv))
+        return fixed.reshape(n, n), inv.reshape(n, n)
+
+    def gmw81(self, A):
+        A = np.asarray(A)
+        n = A.shape[0]
+        a = self._arr(A, n * n)
+        L = np.zeros(n * n, self.dt)
+        self.lib.orc_gmw81(self._p(a), ctypes.c_int(n), self._p(L))
+        return L.reshape(n, n)
+
+    def arap(self, s6, mu, vol):
+        s6 = self._arr(s6, 6)
+        E = np.zeros(1, self.dt)
+        g, H, Hi = np.zeros(6, self.dt), np.zeros(36, self.dt), np.zeros(36, self.dt)
+        real = ctypes.c_double if self.bits == 64 else ctypes.c_float
+        self.lib.orc_arap(self._p(s6), real(mu), real(vol), self._p(E), self._p(g), self._p(H), self._p(Hi))
+        return float(E[0]), g, H.reshape(6, 6), Hi.reshape(6, 6)
+
+    def neohookean(self, s6, mu, lam, vol):
+        s6 = self._arr(s6, 6)
+        E = np.zeros(1, self.dt)
+        g, H, Hi, Hr = (np.zeros(n, self.dt) for n in (6, 36, 36, 36))
+        real = ctypes.c_double if self.bits == 64 else ctypes.c_float
+        self.lib.orc_neohookean(self._p(s6), real(mu), real(lam), real(vol), self._p(E), self._p(g),
+                                self._p(H), self._p(Hi), self._p(Hr))
+        return float(E[0]), g, H.reshape(6, 6), Hi.reshape(6, 6), Hr.reshape(6, 6)
+
+    def barrier(self, d, d0, d1):
+        real = ctypes.c_double if self.bits == 64 else ctypes.c_float
+        q = np.zeros(3, self.dt)
+        self.lib.orc_barrier(real(d0), real(0.0), real(d1), self._p(q), self._p(q))
+        out = np.zeros(3, self.dt)
+        self.lib.orc_barrier(real(d), real(d0), real(d1), self._p(q), self._p(out))
+        return out
+
+    def closest_point_triangle_bary(self, x, a, b, c):
+        x, a, b, c = (self._arr(v, 3) for v in (x, a, b, c))
+        w = np.zeros(3, self.dt)
+        self.lib.orc_closest_point_triangle_bary(self._p(x), self._p(a), self._p(b), self._p(c), self._p(w))
+        return w
+
+    def closest_point_triangle_segment(self, a, b, c, p0, p1):
+        a, b, c, p0, p1 = (self._arr(v, 3) for v in (a, b, c, p0, p1))
+        bary, q = np.zeros(3, self.dt), np.zeros(3, self.dt)
+        hit = ctypes.c_int(0)
+        self.lib.orc_closest_point_triangle_segment.restype = ctypes.c_double if self.bits == 64 else ctypes.c_float
+        d = self.lib.orc_closest_point_triangle_segment(self._p(a), self._p(b), self._p(c), self._p(p0),
+                                                         self._p(p1), self._p(bary), self._p(q), ctypes.byref(hit))
+        return bary, q, float(d), hit.value
+
+
+class Oracle:
+    """Full-solver oracle context.  Construct from a ``mfem_amd`` Scene-like
+    object (numpy fields), then call ``step(dt)`` or the stage methods."""
+
+    INT_FIELDS = {"tets", "pin_idx", "tris", "shape_type", "rowptr", "col", "diag", "lag_sid"}
+
+    def __init__(self, scene, bits: int = 64, params: dict | None = None):
+        self.bits = bits
+        self.lib = _lib(bits)
+        self.dt = _real(bits)
+        nv, nt = scene.particle_q.shape[0], scene.tet_indices.shape[0]
+        ntris = 0 if scene.tri_indices is None else scene.tri_indices.shape[0]
+        nshapes = len(scene.shapes)
+        pins = np.nonzero(scene.particle_inv_mass == 0)[0].astype(np.int32)
+        self.ctx = self.lib.orc_create(nv, nt, ntris, nshapes, len(pins))
+        self.nv, self.nt = nv, nt
+        self.buf("tets")[:] = scene.tet_indices.reshape(-1)
+        self.buf("dminv")[:] = scene.tet_poses.reshape(-1)
+        self.buf("materials")[:] = scene.tet_materials.reshape(-1)
+        self.buf("mass")[:] = scene.particle_mass
+        self.buf("inv_mass")[:] = scene.particle_inv_mass
+        if len(pins):
+            self.buf("pin_idx")[:] = pins
+            self.buf("pin_pos")[:] = scene.particle_q[pins].reshape(-1)
+        if ntris:
+            self.buf("tris")[:] = scene.tri_indices.reshape(-1)
+        for k, sh in enumerate(scene.shapes):
+            self.buf("shape_type")[k] = sh.type
+            self.buf("shape_xf")[7 * k:7 * k + 7] = np.concatenate([sh.pos, sh.quat])
+            self.buf("shape_scale")[3 * k:3 * k + 3] = sh.scale
+        self.buf("x")[:] = scene.particle_q.reshape(-1)
+        self.buf("v")[:] = scene.particle_qd.reshape(-1)
+        p = dict(scene.params)
+        if params:
+            p.update(params)
+        mu = np.asarray(p.pop("friction_mu", 0.0), dtype=np.float64)
+        mu = np.full(nshapes, float(mu)) if mu.ndim == 0 else mu
+        if nshapes:
+            self.buf("shape_mu")[:] = mu
+        op = dict(gx=float(scene.gravity[0]), gy=float(scene.gravity[1]), gz=float(scene.gravity[2]))
+        for k, v in p.items():
+            if k == "energy":
+                v = {"arap": 0, "neohookean": 1}.get(v, v)
+            op[_PARAM_NAMES.get(k, k)] = v
+        for k, v in op.items():
+            if self.lib.orc_set_param(self.ctx, k.encode(), float(v)) != 0:
+                raise KeyError(k)
+        if self.lib.orc_finalize(self.ctx) != 0:
+            raise ValueError("bad topology")
+        self.lib.orc_init_stretch(self.ctx)
+
+    def __del__(self):
+        try:
+            self.lib.orc_destroy(self.ctx)
+        except Exception:
+            pass
+
+    def buf(self, name: str) -> np.ndarray:
+        n = ctypes.c_int64(0)
+        k = ctypes.c_int(0)
+        ptr = self.lib.orc_buf(self.ctx, name.encode(), ctypes.byref(n), ctypes.byref(k))
+        if not ptr:
+            raise KeyError(name)
+        if n.value == 0:
+            return np.zeros(0, np.int32 if k.value else self.dt)
+        ctype = ctypes.c_int32 if k.value else (ctypes.c_double if self.bits == 64 else ctypes.c_float)
+        return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), shape=(n.value,))
+
+    def param(self, name: str) -> float:
+        return self.lib.orc_get_param(self.ctx, name.encode())
+
+    def set_param(self, name: str, value: float) -> None:
+        if self.lib.orc_set_param(self.ctx, name.encode(), float(value)) != 0:
+            raise KeyError(name)
+
+    # stages -------------------------------------------------------------
+    def begin_step(self, dt):
+        self.lib.orc_begin_step(self.ctx, dt)
+
+    def update_system(self, dt):
+        self.lib.orc_update_system(self.ctx, dt)
+
+    def assemble(self, dt):
+        self.lib.orc_assemble(self.ctx, dt)
+
+    def global_solve(self):
+        rr = ctypes.c_double(0)
+        it = self.lib.orc_global_solve(self.ctx, ctypes.byref(rr))
+        return it, rr.value
+
+    def local_solve(self):
+        self.lib.orc_local_solve(self.ctx)
+
+    def line_search(self, dt):
+        acc = ctypes.c_int(0)
+        alpha = ctypes.c_double(0)
+        it = self.lib.orc_line_search(self.ctx, dt, ctypes.byref(acc), ctypes.byref(alpha))
+        return it, bool(acc.value), alpha.value
+
+    def merit(self, x, s, dt):
+        x = np.ascontiguousarray(x, dtype=self.dt).reshape(-1)
+        s = np.ascontiguousarray(s, dtype=self.dt).reshape(-1)
+        return self.lib.orc_merit(self.ctx, x.ctypes.data_as(ctypes.c_void_p), s.ctypes.data_as(ctypes.c_void_p), dt)
+
+    def step(self, dt):
+        self.lib.orc_step(self.ctx, dt)
+
+    def stats(self):
+        n = int(self.param("iterations"))
+        return self.buf("stats")[: 4 * n].reshape(n, 4).copy()
+
+    def csr(self):
+        """Assembled H as a scipy.sparse.bsr_matrix (3x3 blocks)."""
+        import scipy.sparse as sp
+        rowptr = self.buf("rowptr").copy()
+        col = self.buf("col").copy()
+        val = self.buf("val").reshape(-1, 3, 3).copy()
+        return sp.bsr_matrix((val, col, rowptr), shape=(3 * self.nv, 3 * self.nv))
+
+    @property
+    def x(self):
+        return self.buf("x").reshape(-1, 3)
+
+    @property
+    def v(self):
+        return self.buf("v").reshape(-1, 3)

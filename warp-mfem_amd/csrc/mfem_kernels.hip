@@ -1,0 +1,1205 @@
+// mfem_kernels.hip -- gfx950 kernels + C ABI for the per-timestep soft-body
+// solve (warp-mfem RefinementSolver.step, src/mfem/refinement/solver.py:429-579).
+//
+// Kernel map (one launch each unless noted; all use 256-thread workgroups):
+//   k_begin_step   per vertex   predictor, friction anchor, lagged normal force
+//   k_vertex       per vertex   kinetic/attachment gradient, barrier, friction
+//   k_element      per tet      F, SVD, c, G, energy, PSD Hessian, Schur blocks,
+//                               12x12 stiffness (10 unique 3x3 blocks) and force
+//   k_assemble     per lane-slot of the SELL-64 matrix: deterministic gather of
+//                               the tet blocks that land on that block
+//   k_rhs          per vertex   gather of tet forces, diagonal mass/contact terms
+//   k_pcg          persistent, cooperative: whole block-Jacobi PCG loop with
+//                               FP64 reductions and register-resident row state
+//   k_local        per tet      lambda, ds
+//   k_linesearch   persistent, cooperative: whole backtracking loop, FP64 merit
+//   k_end_step     per vertex   velocity
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+
+#include "../../include/mfem_amd.h"
+#include "mfem_math.h"
+
+using namespace mfem;
+
+#define MFEM_VERSION 1
+static constexpr int kBlock = 256;
+static constexpr int kWaves = kBlock / 64;
+// Grid-barrier wait limit in wall_clock64 ticks (100 MHz on CDNA3/4): a wait
+// this long means a workgroup never arrived; the kernel then records the error
+// and exits instead of spinning forever.
+static constexpr uint64_t kBarrierTimeout = 500000000ull;
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+  do {                                                                                                  \
+    hipError_t _e = (expr);                                                                             \
+    if (_e != hipSuccess) return fail(MFEM_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));   \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load3(const float* p, int i, float* o) {
+  o[0] = p[3 * i];
+  o[1] = p[3 * i + 1];
+  o[2] = p[3 * i + 2];
+}
+// packed upper triangle index of (i, j), i <= j, for a 6x6 symmetric matrix
+__device__ __forceinline__ constexpr int sym21(int i, int j) {
+  return i <= j ? (i * 6 - i * (i - 1) / 2 + (j - i)) : (j * 6 - j * (j - 1) / 2 + (i - j));
+}
+// the 10 unique vertex pairs of a tet: 4 diagonal then the 6 upper pairs
+__device__ constexpr int PA[10] = {0, 1, 2, 3, 0, 0, 0, 1, 1, 2};
+__device__ constexpr int PB[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+
+// minimum signed distance over the rigid shapes, world gradient, shape id
+// (src/mfem/refinement/contact.py:62-164, src/mfem/ipc/distance.py:4-57)
+__device__ float shape_distance(const mfem_solver& S, const float* x, float* grad, int* sid) {
+  float dm = 1.0e8f;
+  grad[0] = grad[1] = grad[2] = 0.f;
+  *sid = -1;
+  for (int k = 0; k < S.n_shapes; ++k) {
+    const float* xf = S.shape_xf + 7 * k;
+    float R[9];
+    quat_to_mat(xf + 3, R);
+    float dw[3] = {x[0] - xf[0], x[1] - xf[1], x[2] - xf[2]};
+    float xl[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) xl[i] = R[i] * dw[0] + R[3 + i] * dw[1] + R[6 + i] * dw[2];
+    float d, n[3] = {0.f, 0.f, 1.f};
+    if (S.shape_type[k] == MFEM_SHAPE_PLANE) {
+      d = xl[2];
+    } else if (S.shape_type[k] == MFEM_SHAPE_CAPSULE) {
+      float rad = S.shape_scale[3 * k], hh = S.shape_scale[3 * k + 1];
+      float az = fminf(fmaxf(xl[2], -hh), hh);
+      float df[3] = {xl[0], xl[1], xl[2] - az};
+      float rho = sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+      if (rho > 1e-8f) {
+        n[0] = df[0] / rho; n[1] = df[1] / rho; n[2] = df[2] / rho;
+      }
+      d = rho - rad;
+    } else {
+      continue;
+    }
+    if (d < dm) {
+      dm = d;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) grad[i] = R[i * 3] * n[0] + R[i * 3 + 1] * n[1] + R[i * 3 + 2] * n[2];
+      *sid = k;
+    }
+  }
+  return dm;
+}
+
+// lagged IPC friction at position x (contact.py:304-387)
+__device__ float friction_eval(const mfem_solver& S, int i, const float* x, float dt, float* grad, float* hess) {
+  float lamn = S.lag_fn[i];
+  int sid = S.lag_sid[i];
+  if (grad) grad[0] = grad[1] = grad[2] = 0.f;
+  if (hess)
+#pragma unroll
+    for (int q = 0; q < 9; ++q) hess[q] = 0.f;
+  if (!(lamn > 0.f && sid >= 0)) return 0.f;
+  float mu = S.shape_mu[sid];
+  if (!(mu > 0.f)) return 0.f;
+  float n[3], xs[3];
+  load3(S.lag_n, i, n);
+  load3(S.x_start, i, xs);
+  float u[3] = {x[0] - xs[0], x[1] - xs[1], x[2] - xs[2]};
+  float un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+  float ut[3] = {u[0] - un * n[0], u[1] - un * n[1], u[2] - un * n[2]};
+  float y = sqrtf(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+  float ev = S.eps_v * dt, sc = mu * lamn;
+  float f1y = friction_f1_over_y(y, ev);
+  if (grad)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) grad[k] = sc * f1y * ut[k];
+  if (hess)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) hess[a * 3 + b] = sc * f1y * ((a == b ? 1.f : 0.f) - n[a] * n[b]);
+  return sc * friction_f0(y, ev);
+}
+
+// ---------------------------------------------------------------------------
+// per-vertex kernels
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_begin_step(mfem_solver S, float dt) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.n_verts) return;
+  float x[3], v[3];
+  load3(S.x, i, x);
+  load3(S.v, i, v);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    S.x_start[3 * i + k] = x[k];
+    S.x_tilde[3 * i + k] = x[k] + v[k] * dt + S.gravity[k] * dt * dt / 2.0f;
+  }
+  if (!S.friction_on) return;
+  float grad[3];
+  int sid;
+  float d = shape_distance(S, x, grad, &sid);
+  float lamn = 0.f, n[3] = {0.f, 0.f, 0.f};
+  int s = -1;
+  if (d < S.d1) {
+    float e, de, d2e;
+    barrier(d, S.d0, S.d1, S.qb, e, de, d2e);
+    lamn = S.kappa * fmaxf(-de, 0.f);
+    n[0] = grad[0]; n[1] = grad[1]; n[2] = grad[2];
+    s = sid;
+  }
+  S.lag_n[3 * i] = n[0];
+  S.lag_n[3 * i + 1] = n[1];
+  S.lag_n[3 * i + 2] = n[2];
+  S.lag_fn[i] = lamn;
+  S.lag_sid[i] = s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_vertex(mfem_solver S, float dt) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.n_verts) return;
+  const float idt2 = 1.f / (dt * dt);
+  float x[3], xt[3];
+  load3(S.x, i, x);
+  load3(S.x_tilde, i, xt);
+  float m = S.mass[i];
+  float gx[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) gx[k] = m * (x[k] - xt[k]) * idt2;
+  if (S.inv_mass[i] == 0.f) {
+    float p[3];
+    load3(S.pin_pos, i, p);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gx[k] += S.attach_k * (x[k] - p[k]) * idt2;
+  }
+  float grad[3] = {0.f, 0.f, 0.f}, eb = 0.f, gb[3] = {0.f, 0.f, 0.f}, H[9];
+  int sid;
+  float d = S.n_shapes > 0 ? shape_distance(S, x, grad, &sid) : 1.0e8f;
+  float h2 = 0.f;
+  if (d < S.d1) {
+    float e, de, d2e;
+    barrier(d, S.d0, S.d1, S.qb, e, de, d2e);
+    eb = S.kappa * e;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gb[k] = S.kappa * de * grad[k];
+    h2 = S.kappa * fmaxf(d2e, 0.f);
+  }
+  float gd[3], hd[9];
+  float ed = friction_eval(S, i, x, dt, gd, hd);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) H[a * 3 + b] = h2 * grad[a] * grad[b] + hd[a * 3 + b];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    S.gx[3 * i + k] = gx[k];
+    S.gB[3 * i + k] = gb[k];
+    S.gD[3 * i + k] = gd[k];
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) S.Hv[9 * i + q] = H[q];
+  S.dist[i] = d;
+  S.eB[i] = eb;
+  S.eD[i] = ed;
+}
+
+// ---------------------------------------------------------------------------
+// per-tet element kernel
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void gather_tet(const mfem_solver& S, int e, float (&xv)[4][3], int4& tv) {
+  tv = reinterpret_cast<const int4*>(S.tets)[e];
+  load3(S.x, tv.x, xv[0]);
+  load3(S.x, tv.y, xv[1]);
+  load3(S.x, tv.z, xv[2]);
+  load3(S.x, tv.w, xv[3]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_init_stretch(mfem_solver S) {
+  int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= S.n_tets) return;
+  float xv[4][3];
+  int4 tv;
+  gather_tet(S, e, xv, tv);
+  float D[9], F[9], U[9], sg[3], V[9], S6[6];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) D[q] = S.dminv[9 * e + q];
+  deformation_gradient(xv[0], xv[1], xv[2], xv[3], D, F);
+  svd3(F, U, sg, V);
+  stretch_sym6(sg, V, S6);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) S.s[6 * e + k] = S6[k];
+}
+
+__global__ void __launch_bounds__(kBlock) k_element(mfem_solver S) {
+  int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= S.n_tets) return;
+  float xv[4][3];
+  int4 tv;
+  gather_tet(S, e, xv, tv);
+  float D[9], s[6];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) D[q] = S.dminv[9 * e + q];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) s[q] = S.s[6 * e + q];
+  const float mu = S.materials[3 * e], la = S.materials[3 * e + 1], vol = S.vol[e];
+
+  // kinematics: c = SYM (sym6(S) - s), G = -SYM dS/dF dF/dx
+  float F[9], U[9], sg[3], V[9], S6[6], dS[54], G[72];
+  deformation_gradient(xv[0], xv[1], xv[2], xv[3], D, F);
+  svd3(F, U, sg, V);
+  stretch_sym6(sg, V, S6);
+  float c[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) c[k] = SYMD[k] * (S6[k] - s[k]);
+  stretch_gradient(F, U, sg, V, dS);
+  constraint_jacobian(dS, D, G);
+
+  // elastic energy, gradient, (PSD) Hessian and inverse in stretch space
+  float gs[6], H[36], Hi[36], psi;
+  if (S.energy == 0) {
+    psi = arap_energy(s, mu, vol);
+    arap_grad_hess(s, mu, vol, gs, H, Hi);
+  } else {
+    psi = nh_energy(s, mu, la, vol);
+    nh_grad_hess(s, mu, la, vol, gs, H, Hi);
+  }
+  // Schur blocks: H_l = INV_SYM H INV_SYM, g_l = c + SYM H^-1 g_s
+  float Hl[36], gl[6], hg[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = 0; b < 6; ++b) Hl[a * 6 + b] = H[a * 6 + b] / (SYMD[a] * SYMD[b]);
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    float acc = 0.f;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) acc += Hi[a * 6 + b] * gs[b];
+    gl[a] = c[a] + SYMD[a] * acc;
+  }
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    float acc = 0.f;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) acc += Hl[a * 6 + b] * gl[b];
+    hg[a] = acc;
+  }
+  // stores needed by the local solve and the line search
+#pragma unroll
+  for (int q = 0; q < 72; ++q) S.G[72 * e + q] = G[q];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = i; j < 6; ++j) {
+      S.Hl[21 * e + sym21(i, j)] = Hl[i * 6 + j];
+      S.Hsinv[21 * e + sym21(i, j)] = Hi[i * 6 + j];
+    }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    S.gl[6 * e + k] = gl[k];
+    S.gs[6 * e + k] = gs[k];
+  }
+  S.psi[e] = psi;
+  // HG = H_l G  (6x12)
+  float HG[72];
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      float acc = 0.f;
+#pragma unroll
+      for (int b = 0; b < 6; ++b) acc += Hl[a * 6 + b] * G[b * 12 + k];
+      HG[a * 12 + k] = acc;
+    }
+  // force f = G^T H_l g_l and the 10 unique blocks of K = G^T H_l G
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    float acc = 0.f;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) acc += G[a * 12 + k] * hg[a];
+    S.fe[12 * e + k] = acc;
+  }
+  const size_t T = (size_t)S.n_tets;
+#pragma unroll
+  for (int p = 0; p < 10; ++p) {
+    float* dst = S.Kb + ((size_t)p * T + e) * 9;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        float acc = 0.f;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) acc += G[a * 12 + 3 * PA[p] + r] * HG[a * 12 + 3 * PB[p] + cc];
+        dst[3 * r + cc] = acc;
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// assembly
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ size_t val_index(int lane_slot, int q) {
+  return ((size_t)(lane_slot >> 6) * 9 + q) * 64 + (lane_slot & 63);
+}
+
+__global__ void __launch_bounds__(kBlock) k_assemble(mfem_solver S) {
+  int l = blockIdx.x * kBlock + threadIdx.x;
+  if (l >= S.n_slots * 64) return;
+  float acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) acc[q] = 0.f;
+  const int k0 = S.contrib_ptr[l], k1 = S.contrib_ptr[l + 1];
+  for (int k = k0; k < k1; ++k) {
+    int src = S.contrib_src[k];
+    const float* B = S.Kb + (size_t)(src >> 1) * 9;
+    float b[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) b[q] = B[q];
+    if (src & 1) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[3 * r + c] += b[3 * c + r];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) acc[q] += b[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) S.sell_val[val_index(l, q)] = acc[q];
+}
+
+// g = (H_l G)^T g_l - g_x - grad B - grad D; diagonal += (M + Q)/dt^2 + H_B + H_D
+__global__ void __launch_bounds__(kBlock) k_rhs(mfem_solver S, float dt) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.n_verts) return;
+  float g[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) g[k] = -S.gx[3 * i + k];
+  const int k0 = S.vinc_ptr[i], k1 = S.vinc_ptr[i + 1];
+  for (int k = k0; k < k1; ++k) {
+    int src = S.vinc_src[k];
+    const float* f = S.fe + (size_t)(src >> 2) * 12 + 3 * (src & 3);
+    g[0] += f[0];
+    g[1] += f[1];
+    g[2] += f[2];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) S.g[3 * i + k] = g[k] - S.gB[3 * i + k] - S.gD[3 * i + k];
+  const float idt2 = 1.f / (dt * dt);
+  float im = S.inv_mass[i];
+  float md = (im == 0.f ? 1.f + S.attach_k : 1.f / im) * idt2;
+  int dl = S.diag_lane[i];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    float add = S.Hv[9 * i + q] + ((q % 4 == 0) ? md : 0.f);
+    S.sell_val[val_index(dl, q)] += add;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// grid-wide reduction + barrier for the persistent kernels
+//
+// Each workgroup writes its partial sums and then its arrival epoch
+// (release); every workgroup waits until all epochs match (acquire) and sums
+// the partials in the same fixed order, so every workgroup obtains the
+// bit-identical total and takes the same branch.  Partials ping-pong between
+// two slots by epoch parity.  Waits are bounded (kBarrierTimeout): on timeout
+// the error bit is set and every workgroup leaves the kernel.
+// ---------------------------------------------------------------------------
+struct GridSync {
+  double* red;
+  uint32_t* flags;
+  int32_t* status;
+  uint32_t epoch;
+  bool abort;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int NV>
+__device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
+  static_assert(NV >= 1 && NV <= 4, "at most 4 values per reduction");
+  __shared__ double sm[kWaves][4];
+  __shared__ double tot[kWaves][4];
+  __shared__ int abort_flag;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) abort_flag = 0;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double w = wave_sum(v[i]);
+    if (lane == 0) sm[wave][i] = w;
+  }
+  __syncthreads();
+  gs.epoch += 1u;
+  const uint32_t ep = gs.epoch;
+  const int slot = ep & 1u;
+  const int nb = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double acc = sm[0][i];
+#pragma unroll
+      for (int w = 1; w < kWaves; ++w) acc += sm[w][i];
+      gs.red[((size_t)slot * MFEM_RED_MAX_BLOCKS + blockIdx.x) * 4 + i] = acc;
+    }
+    __threadfence();
+    __hip_atomic_store(&gs.flags[blockIdx.x], ep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  bool to = false;
+  for (int b = threadIdx.x; b < nb && !to; b += kBlock) {
+    uint64_t t0 = wall_clock64();
+    while (__hip_atomic_load(&gs.flags[b], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != ep) {
+      if (wall_clock64() - t0 > kBarrierTimeout ||
+          __hip_atomic_load(&gs.status[MFEM_STATUS_ERRBITS], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        to = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  if (to) {
+    atomicOr(&gs.status[MFEM_STATUS_ERRBITS], 1);
+    abort_flag = 1;
+  }
+  __syncthreads();
+  if (abort_flag) {
+    gs.abort = true;
+    return false;
+  }
+  __threadfence();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) acc += gs.red[((size_t)slot * MFEM_RED_MAX_BLOCKS + b) * 4 + i];
+    acc = wave_sum(acc);
+    if (lane == 0) tot[wave][i] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double acc = tot[0][i];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) acc += tot[w][i];
+    v[i] = acc;
+  }
+  __syncthreads();
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// persistent block-Jacobi PCG (warp.optim.linear.cg semantics: stop when the
+// residual norm <= tol * |g|, at most cg_maxiter iterations)
+//
+// Each wave owns up to CPW chunks of 64 rows and keeps every per-row vector
+// (x in FP64; r, p, q, z and the 3x3 inverse diagonal in FP32) in registers
+// for the whole solve.  Only z (FP32) is published to HBM for the neighbours'
+// SpMV.  Two grid reductions per iteration:
+//   1. y = H z; q = y + beta q (= H p); p = z + beta p; reduce p.q
+//   2. x += alpha p; r -= alpha q; z = M r; publish z; reduce r.z, r.r
+// When the recurrence residual passes, the true residual g - H x is
+// recomputed with FP64 x and FP64 accumulation; if it fails the iteration
+// restarts from it (residual replacement).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void spmv_row_f32(const mfem_solver& S, int c, int lane, const float* vec, float* out) {
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  const int s0 = S.chunk_ptr[c], s1 = S.chunk_ptr[c + 1];
+  for (int s = s0; s < s1; ++s) {
+    const int j = S.sell_col[s * 64 + lane];
+    const float* vb = S.sell_val + (size_t)s * 576 + lane;
+    float v0 = vec[3 * j], v1 = vec[3 * j + 1], v2 = vec[3 * j + 2];
+    a0 += vb[0] * v0 + vb[64] * v1 + vb[128] * v2;
+    a1 += vb[192] * v0 + vb[256] * v1 + vb[320] * v2;
+    a2 += vb[384] * v0 + vb[448] * v1 + vb[512] * v2;
+  }
+  out[0] = a0;
+  out[1] = a1;
+  out[2] = a2;
+}
+
+__device__ __forceinline__ void spmv_row_f64(const mfem_solver& S, int c, int lane, const double* vec, double* out) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  const int s0 = S.chunk_ptr[c], s1 = S.chunk_ptr[c + 1];
+  for (int s = s0; s < s1; ++s) {
+    const int j = S.sell_col[s * 64 + lane];
+    const float* vb = S.sell_val + (size_t)s * 576 + lane;
+    double v0 = vec[3 * j], v1 = vec[3 * j + 1], v2 = vec[3 * j + 2];
+    a0 += (double)vb[0] * v0 + (double)vb[64] * v1 + (double)vb[128] * v2;
+    a1 += (double)vb[192] * v0 + (double)vb[256] * v1 + (double)vb[320] * v2;
+    a2 += (double)vb[384] * v0 + (double)vb[448] * v1 + (double)vb[512] * v2;
+  }
+  out[0] = a0;
+  out[1] = a1;
+  out[2] = a2;
+}
+
+template <int CPW>
+__global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) {
+  GridSync gs{S.red, S.flags, S.status, (uint32_t)S.status[MFEM_STATUS_EPOCH], false};
+  const int lane = threadIdx.x & 63;
+  const int gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int nw = gridDim.x * kWaves;
+  int chunk[CPW];
+  bool live[CPW], own[CPW];
+  double x[CPW][3];
+  float r[CPW][3], p[CPW][3], q[CPW][3], z[CPW][3], M[CPW][9], b[CPW][3];
+
+  double bb = 0.0;
+#pragma unroll
+  for (int k = 0; k < CPW; ++k) {
+    chunk[k] = gw + k * nw;
+    own[k] = chunk[k] < S.n_chunks;
+    const int row = chunk[k] * 64 + lane;
+    live[k] = own[k] && row < S.n_verts;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      x[k][a] = 0.0;
+      b[k][a] = 0.f;
+      p[k][a] = q[k][a] = r[k][a] = z[k][a] = 0.f;
+    }
+#pragma unroll
+    for (int a = 0; a < 9; ++a) M[k][a] = (a % 4 == 0) ? 1.f : 0.f;
+    if (live[k]) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        b[k][a] = S.g[3 * row + a];
+        x[k][a] = (double)S.x_tilde[3 * row + a] - (double)S.x[3 * row + a];
+      }
+      if (S.precond) {
+        float Dg[9];
+        const int dl = S.diag_lane[row];
+#pragma unroll
+        for (int a = 0; a < 9; ++a) Dg[a] = S.sell_val[val_index(dl, a)];
+        if (fabsf(det3(Dg)) >= S.sing_thr) inv3(Dg, M[k]);
+      }
+      bb += (double)b[k][0] * b[k][0] + (double)b[k][1] * b[k][1] + (double)b[k][2] * b[k][2];
+    }
+    if (own[k]) {
+      const int row2 = chunk[k] * 64 + lane;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) S.px[3 * row2 + a] = x[k][a];
+    }
+  }
+  double red1[1] = {bb};
+  if (!grid_reduce(gs, red1)) return;
+  const double bnorm2 = red1[0];
+  const double tol2 = (double)S.cg_tol * (double)S.cg_tol * bnorm2;
+  int it = 0, restarts = 0;
+  double rt2 = 0.0;
+
+  if (bnorm2 > 0.0) {
+    // r0 = g - H x0 (FP64), z0 = M r0
+    double rz = 0.0, rr = 0.0;
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+      if (!own[k]) continue;
+      double y[3];
+      spmv_row_f64(S, chunk[k], lane, S.px, y);
+      const int row = chunk[k] * 64 + lane;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) r[k][a] = live[k] ? (float)((double)b[k][a] - y[a]) : 0.f;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) z[k][a] = M[k][3 * a] * r[k][0] + M[k][3 * a + 1] * r[k][1] + M[k][3 * a + 2] * r[k][2];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        S.pz[3 * row + a] = z[k][a];
+        rz += (double)r[k][a] * z[k][a];
+        rr += (double)r[k][a] * r[k][a];
+      }
+    }
+    double red2[2] = {rz, rr};
+    if (!grid_reduce(gs, red2)) return;
+    rz = red2[0];
+    rr = red2[1];
+    double beta = 0.0;
+    for (;;) {
+      if (rr <= tol2 || it >= S.cg_maxiter) {
+        // true residual with FP64 iterate
+#pragma unroll
+        for (int k = 0; k < CPW; ++k) {
+          if (!own[k]) continue;
+          const int row = chunk[k] * 64 + lane;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) S.px[3 * row + a] = x[k][a];
+        }
+        double red0[1] = {0.0};
+        if (!grid_reduce(gs, red0)) return;
+        double acc = 0.0;
+        float rt[CPW][3];
+#pragma unroll
+        for (int k = 0; k < CPW; ++k) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) rt[k][a] = 0.f;
+          if (!live[k]) continue;
+          double y[3];
+          spmv_row_f64(S, chunk[k], lane, S.px, y);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            double d = (double)b[k][a] - y[a];
+            rt[k][a] = (float)d;
+            acc += d * d;
+          }
+        }
+        double redt[1] = {acc};
+        if (!grid_reduce(gs, redt)) return;
+        rt2 = redt[0];
+        if (rt2 <= tol2 || it >= S.cg_maxiter || restarts >= 16) break;
+        ++restarts;
+        rz = 0.0;
+        rr = 0.0;
+#pragma unroll
+        for (int k = 0; k < CPW; ++k) {
+          if (!own[k]) continue;
+          const int row = chunk[k] * 64 + lane;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) r[k][a] = rt[k][a];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) z[k][a] = M[k][3 * a] * r[k][0] + M[k][3 * a + 1] * r[k][1] + M[k][3 * a + 2] * r[k][2];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            S.pz[3 * row + a] = z[k][a];
+            rz += (double)r[k][a] * z[k][a];
+            rr += (double)r[k][a] * r[k][a];
+          }
+        }
+        double red3[2] = {rz, rr};
+        if (!grid_reduce(gs, red3)) return;
+        rz = red3[0];
+        rr = red3[1];
+        beta = 0.0;
+        continue;
+      }
+      // phase 1: q = H p via y = H z
+      double pq = 0.0;
+      const float fb = (float)beta;
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) {
+        if (!own[k]) continue;
+        float y[3];
+        spmv_row_f32(S, chunk[k], lane, S.pz, y);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          q[k][a] = y[a] + fb * q[k][a];
+          p[k][a] = z[k][a] + fb * p[k][a];
+          pq += (double)p[k][a] * q[k][a];
+        }
+      }
+      double redp[1] = {pq};
+      if (!grid_reduce(gs, redp)) return;
+      pq = redp[0];
+      if (!(pq > 0.0)) {  // breakdown (non-SPD or exact solution): fall to the residual check
+        it = S.cg_maxiter;
+        continue;
+      }
+      const double alpha = rz / pq;
+      const float fa = (float)alpha;
+      double rz2 = 0.0;
+      rr = 0.0;
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) {
+        if (!own[k]) continue;
+        const int row = chunk[k] * 64 + lane;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          x[k][a] += alpha * (double)p[k][a];
+          r[k][a] -= fa * q[k][a];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) z[k][a] = M[k][3 * a] * r[k][0] + M[k][3 * a + 1] * r[k][1] + M[k][3 * a + 2] * r[k][2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          S.pz[3 * row + a] = z[k][a];
+          rz2 += (double)r[k][a] * z[k][a];
+          rr += (double)r[k][a] * r[k][a];
+        }
+      }
+      double redz[2] = {rz2, rr};
+      if (!grid_reduce(gs, redz)) return;
+      rz2 = redz[0];
+      rr = redz[1];
+      beta = rz2 / rz;
+      rz = rz2;
+      ++it;
+    }
+  }
+  // write the solution (g == 0 gives dx = 0)
+#pragma unroll
+  for (int k = 0; k < CPW; ++k) {
+    if (!live[k]) continue;
+    const int row = chunk[k] * 64 + lane;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) S.dx[3 * row + a] = bnorm2 > 0.0 ? x[k][a] : 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (newton_iter >= 0 && newton_iter < MFEM_MAX_STATS_ITERS) {
+      double* st = S.stats + newton_iter * MFEM_STATS_PER_ITER;
+      st[0] = (double)it;
+      st[1] = bnorm2 > 0.0 ? sqrt(rt2 / bnorm2) : 0.0;
+      st[7] = (double)restarts;
+    }
+    S.status[MFEM_STATUS_EPOCH] = (int32_t)gs.epoch;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// local solve (kernels.py:252-283)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_local(mfem_solver S) {
+  int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= S.n_tets) return;
+  int4 tv = reinterpret_cast<const int4*>(S.tets)[e];
+  const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+  float d[12];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[3 * a + k] = (float)S.dx[3 * vid[a] + k];
+  float t[6];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    float acc = S.gl[6 * e + r];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc -= S.G[72 * e + r * 12 + k] * d[k];
+    t[r] = acc;
+  }
+  float lam[6], w[6];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    float acc = 0.f;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc += S.Hl[21 * e + sym21(r, c)] * t[c];
+    lam[r] = -acc;
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) w[c] = SYMD[c] * lam[c] + S.gs[6 * e + c];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    float acc = 0.f;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc += S.Hsinv[21 * e + sym21(r, c)] * w[c];
+    S.ds[6 * e + r] = -acc;
+    S.lam[6 * e + r] = lam[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// line search (line_search.py:76-163; solver.py:803-928)
+// ---------------------------------------------------------------------------
+// kinetic / attachment / barrier / friction terms of vertex i at x + a dx
+__device__ double vertex_merit(const mfem_solver& S, int i, float a, float dt) {
+  float x[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) x[k] = S.x[3 * i + k] + a * (float)S.dx[3 * i + k];
+  const double idt2 = 1.0 / ((double)dt * (double)dt);
+  double d2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double d = (double)(x[k] - S.x_tilde[3 * i + k]);
+    d2 += d * d;
+  }
+  double f = 0.5 * (double)S.mass[i] * d2 * idt2;
+  if (S.inv_mass[i] == 0.f) {
+    double p2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double d = (double)(x[k] - S.pin_pos[3 * i + k]);
+      p2 += d * d;
+    }
+    f += 0.5 * (double)S.attach_k * p2 * idt2;
+  }
+  if (S.n_shapes > 0) {
+    float grad[3];
+    int sid;
+    float d = shape_distance(S, x, grad, &sid);
+    if (d < S.d1) {
+      float e, de, d2e;
+      barrier(d, S.d0, S.d1, S.qb, e, de, d2e);
+      f += (double)(S.kappa * e);
+    }
+    f += (double)friction_eval(S, i, x, dt, nullptr, nullptr);
+  }
+  return f;
+}
+
+// Psi(s + a ds) - lambda . c(x + a dx, s + a ds)
+__device__ double tet_merit(const mfem_solver& S, int e, float a) {
+  int4 tv = reinterpret_cast<const int4*>(S.tets)[e];
+  const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+  float xv[4][3];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) xv[v][k] = S.x[3 * vid[v] + k] + a * (float)S.dx[3 * vid[v] + k];
+  float s[6], D[9];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) s[k] = S.s[6 * e + k] + a * S.ds[6 * e + k];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) D[k] = S.dminv[9 * e + k];
+  const float mu = S.materials[3 * e], la = S.materials[3 * e + 1], vol = S.vol[e];
+  float psi = S.energy == 0 ? arap_energy(s, mu, vol) : nh_energy(s, mu, la, vol);
+  float F[9], U[9], sg[3], V[9], S6[6];
+  deformation_gradient(xv[0], xv[1], xv[2], xv[3], D, F);
+  svd3(F, U, sg, V);
+  stretch_sym6(sg, V, S6);
+  double lc = 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) lc += (double)(SYMD[k] * (S6[k] - s[k])) * (double)S.lam[6 * e + k];
+  return (double)psi - lc;
+}
+
+// (g_x + grad B + grad D - G^T lambda) . dx for vertex i
+__device__ double vertex_slope(const mfem_solver& S, int i) {
+  float gr[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) gr[k] = S.gx[3 * i + k] + S.gB[3 * i + k] + S.gD[3 * i + k];
+  const int k0 = S.vinc_ptr[i], k1 = S.vinc_ptr[i + 1];
+  for (int kk = k0; kk < k1; ++kk) {
+    int src = S.vinc_src[kk];
+    int e = src >> 2, a = src & 3;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      float acc = 0.f;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) acc += S.G[72 * e + q * 12 + 3 * a + r] * S.lam[6 * e + q];
+      gr[r] -= acc;
+    }
+  }
+  return (double)gr[0] * S.dx[3 * i] + (double)gr[1] * S.dx[3 * i + 1] + (double)gr[2] * S.dx[3 * i + 2];
+}
+
+__device__ double tet_slope(const mfem_solver& S, int e) {
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    acc += (double)(S.gs[6 * e + k] + SYMD[k] * S.lam[6 * e + k]) * (double)S.ds[6 * e + k];
+  return acc;
+}
+
+__global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, int newton_iter) {
+  GridSync gs{S.red, S.flags, S.status, (uint32_t)S.status[MFEM_STATUS_EPOCH], false};
+  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  const int nth = gridDim.x * kBlock;
+  double f = 0.0, sl = 0.0;
+  for (int i = tid; i < S.n_verts; i += nth) {
+    f += vertex_merit(S, i, 0.f, dt);
+    sl += vertex_slope(S, i);
+  }
+  for (int e = tid; e < S.n_tets; e += nth) {
+    f += tet_merit(S, e, 0.f);
+    sl += tet_slope(S, e);
+  }
+  double r0[2] = {f, sl};
+  if (!grid_reduce(gs, r0)) return;
+  const double f0 = r0[0];
+  const double t = -(double)S.ls_c * r0[1];
+  double alpha = S.ls_alpha0, used = alpha, f1 = f0;
+  int it = 0;
+  bool cond;
+  do {
+    const float a = (float)alpha;
+    f = 0.0;
+    for (int i = tid; i < S.n_verts; i += nth) f += vertex_merit(S, i, a, dt);
+    for (int e = tid; e < S.n_tets; e += nth) f += tet_merit(S, e, a);
+    double r1[1] = {f};
+    if (!grid_reduce(gs, r1)) return;
+    f1 = r1[0];
+    cond = ((f0 - f1) + (double)S.ls_thr < alpha * t) && it < S.ls_max;
+    ++it;
+    used = alpha;
+    alpha *= (double)S.ls_tau;
+  } while (cond);
+  const bool accepted = it < S.ls_max;
+  if (accepted) {
+    const float a = (float)used;
+    for (int i = tid; i < S.n_verts; i += nth)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) S.x[3 * i + k] = S.x[3 * i + k] + a * (float)S.dx[3 * i + k];
+    for (int e = tid; e < S.n_tets; e += nth)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) S.s[6 * e + k] = S.s[6 * e + k] + a * S.ds[6 * e + k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (newton_iter >= 0 && newton_iter < MFEM_MAX_STATS_ITERS) {
+      double* st = S.stats + newton_iter * MFEM_STATS_PER_ITER;
+      st[2] = (double)it;
+      st[3] = accepted ? 1.0 : 0.0;
+      st[4] = used;
+      st[5] = f0;
+      st[6] = f1;
+    }
+    S.status[MFEM_STATUS_EPOCH] = (int32_t)gs.epoch;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_plain_update(mfem_solver S, int newton_iter) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < S.n_verts)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) S.x[3 * i + k] += (float)S.dx[3 * i + k];
+  if (i < S.n_tets)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.s[6 * i + k] += S.ds[6 * i + k];
+  if (i == 0 && newton_iter >= 0 && newton_iter < MFEM_MAX_STATS_ITERS) {
+    double* st = S.stats + newton_iter * MFEM_STATS_PER_ITER;
+    st[2] = 0.0;
+    st[3] = 1.0;
+    st[4] = 1.0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_end_step(mfem_solver S, float dt) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.n_verts) return;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) S.v[3 * i + k] = (S.x[3 * i + k] - S.x_start[3 * i + k]) / dt;
+}
+
+__global__ void __launch_bounds__(kBlock) k_spmv(mfem_solver S, const float* xin, float* yout) {
+  int row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= S.n_verts) return;
+  float y[3];
+  spmv_row_f32(S, row >> 6, row & 63, xin, y);
+  yout[3 * row] = y[0];
+  yout[3 * row + 1] = y[1];
+  yout[3 * row + 2] = y[2];
+}
+
+__global__ void __launch_bounds__(kBlock) k_merit(mfem_solver S, float dt, float a, double* out) {
+  __shared__ double sm[kWaves];
+  const int tid = blockIdx.x * kBlock + threadIdx.x;
+  double f = 0.0;
+  if (tid < S.n_verts) f += vertex_merit(S, tid, a, dt);
+  if (tid < S.n_tets) f += tet_merit(S, tid, a);
+  f = wave_sum(f);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = f;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double acc = 0.0;
+    for (int w = 0; w < kWaves; ++w) acc += sm[w];
+    atomicAdd(out, acc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+
+struct CoopInfo {
+  int max_blocks_pcg[4] = {0, 0, 0, 0};
+  int max_blocks_ls = 0;
+  bool ready = false;
+};
+std::map<int, CoopInfo> g_coop;
+
+int coop_info(CoopInfo** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  CoopInfo& ci = g_coop[dev];
+  if (!ci.ready) {
+    int cus = 0, coop = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    if (!coop) return fail(MFEM_EHIP, "device does not support cooperative launches");
+    const void* fns[4] = {reinterpret_cast<const void*>(&k_pcg<1>), reinterpret_cast<const void*>(&k_pcg<2>),
+                          reinterpret_cast<const void*>(&k_pcg<4>), reinterpret_cast<const void*>(&k_pcg<8>)};
+    for (int k = 0; k < 4; ++k) {
+      int nb = 0;
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fns[k], kBlock, 0));
+      ci.max_blocks_pcg[k] = nb * cus;
+    }
+    int nb = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_linesearch), kBlock, 0));
+    ci.max_blocks_ls = nb * cus;
+    ci.ready = true;
+  }
+  *out = &ci;
+  return MFEM_OK;
+}
+
+int check_solver(const mfem_solver* S) {
+  if (!S) return fail(MFEM_EINVAL, "null solver");
+  if (S->n_verts <= 0 || S->n_tets <= 0) return fail(MFEM_EINVAL, "empty mesh");
+  if (S->n_chunks != (S->n_verts + 63) / 64) return fail(MFEM_EINVAL, "n_chunks != ceil(n_verts/64)");
+  if (S->n_slots < S->n_chunks) return fail(MFEM_EINVAL, "n_slots < n_chunks");
+  if (S->n_shapes < 0 || S->n_tris < 0) return fail(MFEM_EINVAL, "negative count");
+  if (S->iterations > MFEM_MAX_STATS_ITERS) return fail(MFEM_EINVAL, "iterations > MFEM_MAX_STATS_ITERS");
+  if (!S->x || !S->v || !S->s || !S->lam || !S->tets || !S->sell_val || !S->status || !S->stats)
+    return fail(MFEM_EINVAL, "null buffer");
+  return MFEM_OK;
+}
+
+inline unsigned blocks_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+int launched() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(MFEM_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return MFEM_OK;
+}
+
+}  // namespace
+
+#define FIELD_TABLE(X)                                                                                   \
+  X(n_verts) X(n_tets) X(n_tris) X(n_shapes) X(n_chunks) X(n_slots) X(energy) X(iterations) X(cg_maxiter) \
+  X(precond) X(line_search) X(ls_max) X(tri_contact) X(friction_on) X(cg_tol) X(ls_alpha0) X(ls_tau)      \
+  X(ls_c) X(ls_thr) X(sing_thr) X(attach_k) X(d0) X(d1) X(kappa) X(eps_v) X(qb) X(gravity) X(pad0) X(tets) \
+  X(dminv) X(materials) X(vol) X(mass) X(inv_mass) X(pin_pos) X(tris) X(shape_type) X(shape_xf)            \
+  X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
+  X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
+  X(lag_sid) X(G) X(Hl) X(gl) X(Hsinv) X(gs) X(psi) X(Kb) X(fe) X(gx) X(gB) X(gD) X(Hv) X(dist) X(eB)      \
+  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats)
+
+extern "C" {
+
+int64_t mfem_solver_sizeof(void) { return (int64_t)sizeof(mfem_solver); }
+
+int64_t mfem_solver_offsetof(const char* field) {
+#define X(f) \
+  if (!strcmp(field, #f)) return (int64_t)offsetof(mfem_solver, f);
+  FIELD_TABLE(X)
+#undef X
+  return -1;
+}
+
+int mfem_version(void) { return MFEM_VERSION; }
+
+const char* mfem_last_error(void) { return g_last_error.c_str(); }
+
+int mfem_init_stretch(const mfem_solver* S, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  hipLaunchKernelGGL(k_init_stretch, dim3(blocks_for(S->n_tets)), dim3(kBlock), 0, (hipStream_t)stream, *S);
+  return launched();
+}
+
+int mfem_begin_step(const mfem_solver* S, float dt, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  if (!(dt > 0.f)) return fail(MFEM_EINVAL, "dt must be > 0");
+  hipLaunchKernelGGL(k_begin_step, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, (hipStream_t)stream, *S, dt);
+  return launched();
+}
+
+int mfem_update_system(const mfem_solver* S, float dt, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  if (!(dt > 0.f)) return fail(MFEM_EINVAL, "dt must be > 0");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_vertex, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, st, *S, dt);
+  if (int rc = launched()) return rc;
+  hipLaunchKernelGGL(k_element, dim3(blocks_for(S->n_tets)), dim3(kBlock), 0, st, *S);
+  return launched();
+}
+
+int mfem_assemble(const mfem_solver* S, float dt, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_assemble, dim3(blocks_for((long)S->n_slots * 64)), dim3(kBlock), 0, st, *S);
+  if (int rc = launched()) return rc;
+  hipLaunchKernelGGL(k_rhs, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, st, *S, dt);
+  return launched();
+}
+
+int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  CoopInfo* ci = nullptr;
+  if (int rc = coop_info(&ci)) return rc;
+  static const int cpw_opts[4] = {1, 2, 4, 8};
+  int sel = -1;
+  unsigned nblocks = 0;
+  for (int k = 0; k < 4; ++k) {
+    long waves = (S->n_chunks + cpw_opts[k] - 1) / cpw_opts[k];
+    long nb = (waves + kWaves - 1) / kWaves;
+    if (nb <= ci->max_blocks_pcg[k] && nb <= MFEM_RED_MAX_BLOCKS) {
+      sel = k;
+      nblocks = (unsigned)nb;
+      break;
+    }
+  }
+  if (sel < 0) return fail(MFEM_ECAPACITY, "system too large for the persistent PCG kernel");
+  mfem_solver Sc = *S;
+  void* args[] = {(void*)&Sc, (void*)&newton_iter};
+  const void* fn = sel == 0 ? reinterpret_cast<const void*>(&k_pcg<1>)
+                 : sel == 1 ? reinterpret_cast<const void*>(&k_pcg<2>)
+                 : sel == 2 ? reinterpret_cast<const void*>(&k_pcg<4>)
+                            : reinterpret_cast<const void*>(&k_pcg<8>);
+  HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(kBlock), args, 0, (hipStream_t)stream));
+  return launched();
+}
+
+int mfem_local_solve(const mfem_solver* S, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  hipLaunchKernelGGL(k_local, dim3(blocks_for(S->n_tets)), dim3(kBlock), 0, (hipStream_t)stream, *S);
+  return launched();
+}
+
+int mfem_line_search(const mfem_solver* S, float dt, int newton_iter, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (!S->line_search) {
+    long n = S->n_verts > S->n_tets ? S->n_verts : S->n_tets;
+    hipLaunchKernelGGL(k_plain_update, dim3(blocks_for(n)), dim3(kBlock), 0, st, *S, newton_iter);
+    return launched();
+  }
+  CoopInfo* ci = nullptr;
+  if (int rc = coop_info(&ci)) return rc;
+  long n = S->n_verts > S->n_tets ? S->n_verts : S->n_tets;
+  long nb = blocks_for(n);
+  if (nb > ci->max_blocks_ls) nb = ci->max_blocks_ls;
+  if (nb > MFEM_RED_MAX_BLOCKS) nb = MFEM_RED_MAX_BLOCKS;
+  if (nb < 1) return fail(MFEM_EHIP, "line-search kernel cannot be resident");
+  mfem_solver Sc = *S;
+  void* args[] = {(void*)&Sc, (void*)&dt, (void*)&newton_iter};
+  HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_linesearch), dim3((unsigned)nb), dim3(kBlock),
+                                     args, 0, st));
+  return launched();
+}
+
+int mfem_end_step(const mfem_solver* S, float dt, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  hipLaunchKernelGGL(k_end_step, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, (hipStream_t)stream, *S, dt);
+  return launched();
+}
+
+int mfem_step(const mfem_solver* S, float dt, void* stream) {
+  if (int rc = mfem_begin_step(S, dt, stream)) return rc;
+  for (int it = 0; it < S->iterations; ++it) {
+    if (int rc = mfem_update_system(S, dt, stream)) return rc;
+    if (int rc = mfem_assemble(S, dt, stream)) return rc;
+    if (int rc = mfem_global_solve(S, it, stream)) return rc;
+    if (int rc = mfem_local_solve(S, stream)) return rc;
+    if (int rc = mfem_line_search(S, dt, it, stream)) return rc;
+  }
+  return mfem_end_step(S, dt, stream);
+}
+
+int mfem_spmv(const mfem_solver* S, const float* x, float* y, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  if (!x || !y) return fail(MFEM_EINVAL, "null vector");
+  hipLaunchKernelGGL(k_spmv, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, (hipStream_t)stream, *S, x, y);
+  return launched();
+}
+
+int mfem_merit(const mfem_solver* S, float dt, float alpha, double* out, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  if (!out) return fail(MFEM_EINVAL, "null output");
+  hipStream_t st = (hipStream_t)stream;
+  HIP_TRY(hipMemsetAsync(out, 0, sizeof(double), st));
+  long n = S->n_verts > S->n_tets ? S->n_verts : S->n_tets;
+  hipLaunchKernelGGL(k_merit, dim3(blocks_for(n)), dim3(kBlock), 0, st, *S, dt, alpha, out);
+  return launched();
+}
+
+}  // extern "C"

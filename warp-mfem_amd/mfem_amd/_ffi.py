@@ -1,0 +1,102 @@
+"""ctypes binding of libmfem_amd.so (declared in include/mfem_amd.h).
+
+The library is the only compute path: importing this module on a machine
+without the built library raises, and nothing falls back to a CPU
+implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmfem_amd.so")
+
+c_i32, c_f32, c_ptr = ctypes.c_int32, ctypes.c_float, ctypes.c_void_p
+
+# (name, ctype) in declaration order of struct mfem_solver
+FIELDS = [
+    ("n_verts", c_i32), ("n_tets", c_i32), ("n_tris", c_i32), ("n_shapes", c_i32), ("n_chunks", c_i32),
+    ("n_slots", c_i32), ("energy", c_i32), ("iterations", c_i32), ("cg_maxiter", c_i32), ("precond", c_i32),
+    ("line_search", c_i32), ("ls_max", c_i32), ("tri_contact", c_i32), ("friction_on", c_i32),
+    ("cg_tol", c_f32), ("ls_alpha0", c_f32), ("ls_tau", c_f32), ("ls_c", c_f32), ("ls_thr", c_f32),
+    ("sing_thr", c_f32), ("attach_k", c_f32), ("d0", c_f32), ("d1", c_f32), ("kappa", c_f32), ("eps_v", c_f32),
+    ("qb", c_f32 * 3), ("gravity", c_f32 * 3), ("pad0", c_f32),
+] + [(n, c_ptr) for n in (
+    "tets", "dminv", "materials", "vol", "mass", "inv_mass", "pin_pos", "tris", "shape_type", "shape_xf",
+    "shape_scale", "shape_mu", "chunk_ptr", "sell_col", "sell_val", "diag_lane", "contrib_ptr", "contrib_src",
+    "vinc_ptr", "vinc_src", "x", "v", "s", "lam", "x_tilde", "x_start", "lag_n", "lag_fn", "lag_sid",
+    "G", "Hl", "gl", "Hsinv", "gs", "psi", "Kb", "fe", "gx", "gB", "gD", "Hv", "dist", "eB", "eD", "g",
+    "dx", "ds", "pz", "px", "red", "flags", "status", "stats",
+)]
+
+
+class MfemSolverStruct(ctypes.Structure):
+    _fields_ = FIELDS
+
+
+# mirrors of the header's constants
+MFEM_STATS_PER_ITER = 8
+MFEM_MAX_STATS_ITERS = 64
+MFEM_RED_MAX_BLOCKS = 2048
+SHAPE_PLANE, SHAPE_CAPSULE = 0, 1
+
+ENTRY_POINTS = [
+    "mfem_solver_sizeof", "mfem_solver_offsetof", "mfem_version", "mfem_last_error", "mfem_init_stretch",
+    "mfem_begin_step", "mfem_update_system", "mfem_assemble", "mfem_global_solve", "mfem_local_solve",
+    "mfem_line_search", "mfem_end_step", "mfem_step", "mfem_spmv", "mfem_merit",
+]
+
+
+class MfemError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libmfem_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MfemError(f"{path} not found: build it with `python warp-mfem_amd/build.py`")
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER(MfemSolverStruct)
+    vp = ctypes.c_void_p
+    lib.mfem_solver_sizeof.restype = ctypes.c_int64
+    lib.mfem_solver_offsetof.restype = ctypes.c_int64
+    lib.mfem_solver_offsetof.argtypes = [ctypes.c_char_p]
+    lib.mfem_version.restype = ctypes.c_int
+    lib.mfem_last_error.restype = ctypes.c_char_p
+    for name in ("mfem_init_stretch", "mfem_local_solve"):
+        getattr(lib, name).argtypes = [P, vp]
+    for name in ("mfem_begin_step", "mfem_update_system", "mfem_assemble", "mfem_end_step", "mfem_step"):
+        getattr(lib, name).argtypes = [P, ctypes.c_float, vp]
+    lib.mfem_global_solve.argtypes = [P, ctypes.c_int, vp]
+    lib.mfem_line_search.argtypes = [P, ctypes.c_float, ctypes.c_int, vp]
+    lib.mfem_spmv.argtypes = [P, vp, vp, vp]
+    lib.mfem_merit.argtypes = [P, ctypes.c_float, ctypes.c_float, vp, vp]
+    for name in ENTRY_POINTS[4:]:
+        getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check_layout(lib: ctypes.CDLL | None = None) -> None:
+    """Raise if the ctypes mirror disagrees with the compiled struct layout."""
+    lib = lib or load()
+    if lib.mfem_solver_sizeof() != ctypes.sizeof(MfemSolverStruct):
+        raise MfemError("mfem_solver size mismatch")
+    for name, _ in FIELDS:
+        off = lib.mfem_solver_offsetof(name.encode())
+        if off != getattr(MfemSolverStruct, name).offset:
+            raise MfemError(f"mfem_solver.{name} offset mismatch ({off})")
+
+
+def call(fn_name: str, *args) -> None:
+    lib = load()
+    rc = getattr(lib, fn_name)(*args)
+    if rc != 0:
+        raise MfemError(f"{fn_name} failed ({rc}): {lib.mfem_last_error().decode()}")

@@ -1,0 +1,128 @@
+"""Host-side construction of the system-matrix pattern (numpy, once per topology).
+
+The reference assembles H with warp.sparse (bsr_set_from_triplets + bsr_mm +
+bsr_axpy into a padded BSR, src/mfem/refinement/solver.py:641-799).  Here the
+pattern is fixed per topology and laid out for the GPU:
+
+* SELL-64: rows (vertices) are grouped in chunks of 64 = one wavefront; a
+  chunk stores max-row-length "slots", and slot s of chunk c holds, for each of
+  the 64 lanes, one 3x3 block as 9 lane-minor floats
+  (value[(slot*9 + q)*64 + lane]).  Every load in the SpMV is a contiguous
+  256-byte wave access.  Padding blocks are zero with column 0.
+* Gather lists for a deterministic assembly: every lane-slot (block (i, j))
+  owns the ordered list of tet-block sources that sum into it; each vertex owns
+  the ordered list of (tet, corner) forces that sum into its right-hand side.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .mesh import vertex_adjacency_pattern
+
+# the 10 unique vertex pairs of a tet (4 diagonal, then the 6 upper pairs);
+# must match PA / PB in csrc/mfem_kernels.hip
+PAIRS = [(0, 0), (1, 1), (2, 2), (3, 3), (0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+
+
+@dataclass
+class SystemPattern:
+    n_verts: int
+    n_chunks: int
+    n_slots: int
+    rowptr: np.ndarray      # CSR of the block pattern (for reference / tests)
+    cols: np.ndarray
+    chunk_ptr: np.ndarray   # int32 [n_chunks + 1]
+    sell_col: np.ndarray    # int32 [n_slots * 64]
+    diag_lane: np.ndarray   # int32 [n_verts]
+    csr_lane: np.ndarray    # int64 [nnz] lane-slot of each CSR block
+    contrib_ptr: np.ndarray  # int32 [n_slots * 64 + 1]
+    contrib_src: np.ndarray  # int32
+    vinc_ptr: np.ndarray    # int32 [n_verts + 1]
+    vinc_src: np.ndarray    # int32
+
+    @property
+    def padding_ratio(self) -> float:
+        return self.n_slots * 64 / max(1, len(self.cols))
+
+
+def build_system_pattern(n_verts: int, tets: np.ndarray) -> SystemPattern:
+    tets = np.asarray(tets, dtype=np.int64)
+    T = tets.shape[0]
+    if T and (tets.min() < 0 or tets.max() >= n_verts):
+        raise ValueError("tet index out of range")
+    rowptr, cols = vertex_adjacency_pattern(n_verts, tets)
+    nnz = len(cols)
+    n_chunks = (n_verts + 63) // 64
+    lens = np.diff(rowptr.astype(np.int64))
+    lens_p = np.zeros(n_chunks * 64, dtype=np.int64)
+    lens_p[:n_verts] = lens
+    width = lens_p.reshape(n_chunks, 64).max(axis=1)
+    chunk_ptr = np.zeros(n_chunks + 1, dtype=np.int64)
+    chunk_ptr[1:] = np.cumsum(width)
+    n_slots = int(chunk_ptr[-1])
+
+    rows = np.repeat(np.arange(n_verts, dtype=np.int64), lens)
+    k_in_row = np.arange(nnz, dtype=np.int64) - rowptr[rows].astype(np.int64)
+    csr_lane = (chunk_ptr[rows // 64] + k_in_row) * 64 + rows % 64
+    sell_col = np.zeros(n_slots * 64, dtype=np.int32)
+    sell_col[csr_lane] = cols
+    diag_mask = cols == rows
+    diag_lane = np.zeros(n_verts, dtype=np.int64)
+    diag_lane[rows[diag_mask]] = csr_lane[diag_mask]
+
+    # tet block -> lane-slot contributions
+    keys = rows * n_verts + cols.astype(np.int64)  # sorted (row-major, cols ascending)
+    srcs, dests = [], []
+    for p, (a, b) in enumerate(PAIRS):
+        blk = (p * T + np.arange(T, dtype=np.int64))
+        ia, ib = tets[:, a], tets[:, b]
+        k = np.searchsorted(keys, ia * n_verts + ib)
+        dests.append(csr_lane[k])
+        srcs.append(blk * 2)
+        if a != b:  # block (b, a) is the transpose of (a, b)
+            k = np.searchsorted(keys, ib * n_verts + ia)
+            dests.append(csr_lane[k])
+            srcs.append(blk * 2 + 1)
+    dest = np.concatenate(dests)
+    src = np.concatenate(srcs)
+    order = np.lexsort((src, dest))
+    dest, src = dest[order], src[order]
+    contrib_ptr = np.zeros(n_slots * 64 + 1, dtype=np.int64)
+    np.add.at(contrib_ptr, dest + 1, 1)
+    contrib_ptr = np.cumsum(contrib_ptr)
+
+    # vertex -> (tet, corner) incidence
+    vs = tets.reshape(-1)
+    vsrc = np.arange(4 * T, dtype=np.int64)
+    order = np.lexsort((vsrc, vs))
+    vinc_ptr = np.zeros(n_verts + 1, dtype=np.int64)
+    np.add.at(vinc_ptr, vs + 1, 1)
+    vinc_ptr = np.cumsum(vinc_ptr)
+
+    for name, arr in (("contrib_src", src), ("contrib_ptr", contrib_ptr), ("lane", csr_lane)):
+        if arr.size and arr.max() >= 2**31:
+            raise ValueError(f"{name} exceeds int32")
+    return SystemPattern(
+        n_verts=n_verts,
+        n_chunks=n_chunks,
+        n_slots=n_slots,
+        rowptr=rowptr,
+        cols=cols,
+        chunk_ptr=chunk_ptr.astype(np.int32),
+        sell_col=sell_col,
+        diag_lane=diag_lane.astype(np.int32),
+        csr_lane=csr_lane,
+        contrib_ptr=contrib_ptr.astype(np.int32),
+        contrib_src=src.astype(np.int32),
+        vinc_ptr=vinc_ptr.astype(np.int32),
+        vinc_src=vsrc[order].astype(np.int32),
+    )
+
+
+def sell_to_csr_values(pat: SystemPattern, sell_val: np.ndarray) -> np.ndarray:
+    """(nnz, 3, 3) CSR block values from the lane-minor SELL value array."""
+    sv = np.asarray(sell_val).reshape(pat.n_slots, 9, 64)
+    lane = pat.csr_lane
+    return sv[lane // 64, :, lane % 64].reshape(-1, 3, 3)

@@ -1,0 +1,215 @@
+"""Model / State containers and the synthetic benchmark scenes.
+
+``Model`` carries the fields ``RefinementSolver`` reads from a newton Model
+(src/mfem/refinement/solver.py:41-304; src/mfem/refinement/contact.py:778-862):
+particle_q/qd/mass/inv_mass, tet_indices/poses/materials, tri_indices, the
+rigid shapes and gravity.  Arrays are numpy on the host; the solver copies them
+to the device once.  ``State`` holds the per-step device tensors
+(particle_q, particle_qd, body_q).
+
+Scenes:
+
+* ``beam_scene``      cantilever beam 1.0 x 0.2 x 0.2 m, 20x4x4 cubes x 6 tets
+                      (1,920 tets, 525 vertices), ARAP, E=1e5 Pa, nu=0.45,
+                      rho=1000, x=0 face pinned, g=(0,0,-9.81), dt=1/60.
+* ``octopus_scene``   ellipsoid mantle (0.3x0.2x0.2 m) plus 8 tapered arms
+                      0.6 m long, lattice-meshed at h=7.3 mm (~2e5 tets),
+                      Neo-Hookean with a seeded log-normal stiffness field
+                      (mean E=5e4 Pa, sigma_log=0.5, correlation length
+                      0.05 m), dropped onto the plane z=0 with an IPC barrier
+                      (dhat=1e-3) and friction mu=0.3, dt=1/100.
+* ``drop_box_scene``  small Neo-Hookean box over a plane, for parity tests.
+
+``Model.params`` holds RefinementSolver keyword arguments (reference names,
+src/mfem/refinement/solver.py:56-128, 241-245) plus ``iterations``.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import mesh as _mesh
+
+SHAPE_PLANE = 0
+SHAPE_CAPSULE = 1
+
+
+@dataclass
+class Shape:
+    """Rigid collider: a plane (local +Z normal through the origin) or a
+    capsule (local Z axis, scale = (radius, half_height)), posed in the world
+    by ``pos`` and the quaternion ``quat`` = (x, y, z, w)
+    (src/mfem/ipc/distance.py:4-57)."""
+
+    type: int
+    pos: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    quat: np.ndarray = field(default_factory=lambda: np.array([0.0, 0.0, 0.0, 1.0]))
+    scale: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    mu: float = 0.0
+
+
+@dataclass
+class Model:
+    particle_q: np.ndarray
+    particle_qd: np.ndarray
+    particle_mass: np.ndarray
+    particle_inv_mass: np.ndarray
+    tet_indices: np.ndarray
+    tet_poses: np.ndarray
+    tet_materials: np.ndarray
+    tri_indices: np.ndarray | None
+    shapes: list
+    gravity: np.ndarray
+    params: dict = field(default_factory=dict)
+    name: str = ""
+
+    @property
+    def particle_count(self) -> int:
+        return int(self.particle_q.shape[0])
+
+    @property
+    def tet_count(self) -> int:
+        return int(self.tet_indices.shape[0])
+
+
+def build_model(particles, tets, materials, density, *, pinned=None, shapes=(), gravity=(0.0, 0.0, -9.81),
+                velocity=None, params=None, name="") -> Model:
+    """Rest pose, lumped masses and surface triangles for a tet mesh."""
+    particles = np.asarray(particles, dtype=np.float64)
+    tets = np.asarray(tets, dtype=np.int32)
+    dminv, vol = _mesh.rest_pose(particles, tets)
+    mass = _mesh.lumped_mass(particles.shape[0], tets, vol, density)
+    inv_mass = np.where(mass > 0, 1.0 / np.maximum(mass, 1e-300), 0.0)
+    if pinned is not None:
+        inv_mass[np.asarray(pinned)] = 0.0
+    qd = np.zeros_like(particles) if velocity is None else np.broadcast_to(velocity, particles.shape).copy()
+    return Model(
+        particle_q=particles,
+        particle_qd=qd,
+        particle_mass=mass,
+        particle_inv_mass=inv_mass,
+        tet_indices=tets,
+        tet_poses=dminv,
+        tet_materials=np.asarray(materials, dtype=np.float64),
+        tri_indices=_mesh.surface_tris(tets),
+        shapes=list(shapes),
+        gravity=np.asarray(gravity, dtype=np.float64),
+        params=dict(params or {}),
+        name=name,
+    )
+
+
+def _materials(mu, lam, n):
+    m = np.zeros((n, 3))
+    m[:, 0] = mu
+    m[:, 1] = lam
+    return m
+
+
+def beam_scene(cells=(20, 4, 4), energy="arap") -> Model:
+    """Cantilever beam, x=0 face pinned (25 vertices at the default size)."""
+    p, t = _mesh.box_tet_mesh((0.0, 0.0, 0.0), (1.0, 0.2, 0.2), cells)
+    mu, lam = _mesh.lame(1.0e5, 0.45)
+    pinned = np.nonzero(np.abs(p[:, 0]) < 1e-9)[0]
+    return build_model(
+        p, t, _materials(mu, lam, len(t)), 1000.0, pinned=pinned,
+        params=dict(energy=energy, iterations=8, cg_max_iterations=1000, cg_tolerance=1e-6,
+                    attachment_stiffness=10.0, line_search=True, line_search_max_iterations=20),
+        name="beam_%dx%dx%d" % tuple(cells),
+    )
+
+
+def gaussian_field(points: np.ndarray, length: float, seed: int, n_features: int = 256) -> np.ndarray:
+    """Unit-variance Gaussian random field with squared-exponential covariance
+    of correlation length ``length``, by random Fourier features."""
+    rng = np.random.default_rng(seed)
+    w = rng.normal(0.0, 1.0 / length, size=(n_features, 3))
+    b = rng.uniform(0.0, 2.0 * np.pi, size=n_features)
+    return np.sqrt(2.0 / n_features) * np.cos(points @ w.T + b).sum(axis=1)
+
+
+def barrier_slope(d, d1):
+    d = np.asarray(d, dtype=np.float64)
+    return -(2.0 * (d - d1) * np.log(d / d1) + (d - d1) ** 2 / d)
+
+
+def auto_kappa(model: Model, dhat: float, factor: float = 10.0) -> float:
+    """Barrier stiffness that holds ``factor`` x the mean vertex weight at
+    half the activation distance: kappa |b'(dhat/2)| = factor m g."""
+    w = float(np.mean(model.particle_mass)) * float(np.linalg.norm(model.gravity))
+    return factor * w / abs(float(barrier_slope(0.5 * dhat, dhat)))
+
+
+def octopus_inside(points: np.ndarray, arms: int = 8) -> np.ndarray:
+    """Ellipsoid mantle (semi-axes 0.15, 0.10, 0.10 m, centre 0.10 m above the
+    arm plane) united with ``arms`` tapered arms (radius 0.03 -> 0.01 m, length
+    0.6 m) radiating horizontally, their undersides flush with z=0."""
+    p = points
+    c = np.array([0.0, 0.0, 0.13])
+    inside = ((p[:, 0] - c[0]) / 0.15) ** 2 + ((p[:, 1] - c[1]) / 0.10) ** 2 + ((p[:, 2] - c[2]) / 0.10) ** 2 <= 1.0
+    r0, r1, L = 0.03, 0.01, 0.6
+    for k in range(arms):
+        th = 2.0 * np.pi * (k + 0.5) / arms
+        d = np.array([np.cos(th), np.sin(th), 0.0])
+        a = np.array([0.08 * np.cos(th), 0.06 * np.sin(th), 0.0])
+        rel = p - a
+        t = np.clip(rel @ d / L, 0.0, 1.0)
+        r = r0 + (r1 - r0) * t
+        axis = a[None, :] + (t * L)[:, None] * d[None, :]
+        axis[:, 2] = r  # underside on z = 0
+        inside |= np.linalg.norm(p - axis, axis=1) <= r
+    return inside
+
+
+def octopus_scene(h: float = 7.3e-3, seed: int = 7, clearance: float = 2.0e-3, dhat: float = 1.0e-3,
+                  friction_mu: float = 0.3, iterations: int = 12) -> Model:
+    lo = np.array([-0.72, -0.72, -0.001])
+    hi = np.array([0.72, 0.72, 0.24])
+    p, t = _mesh.implicit_tet_mesh(octopus_inside, lo, hi, h)
+    p[:, 2] += clearance - p[:, 2].min()
+    cen = p[t].mean(axis=1)
+    g = gaussian_field(cen, 0.05, seed)
+    E = 5.0e4 * np.exp(0.5 * g - 0.125)
+    mu, lam = _mesh.lame(E, 0.45)
+    ground = Shape(SHAPE_PLANE, mu=friction_mu)
+    m = build_model(p, t, np.stack([mu, lam, np.zeros_like(mu)], axis=1), 1000.0, shapes=[ground],
+                    name="octopus_h%.1fmm" % (h * 1e3))
+    m.params.update(energy="neohookean", iterations=iterations, cg_max_iterations=5000, cg_tolerance=1e-6,
+                    line_search=True, line_search_max_iterations=30, contact_d0=0.1 * dhat, contact_d1=dhat,
+                    contact_stiffness=auto_kappa(m, dhat), friction_mu=[friction_mu], friction_eps_v=1e-3,
+                    attachment_stiffness=10.0)
+    return m
+
+
+def drop_box_scene(cells=(4, 3, 3), size=(0.2, 0.15, 0.15), clearance=5.0e-4, E=2.0e4, energy="neohookean",
+                   friction_mu=0.3, dhat=1.0e-3, velocity=(0.3, 0.0, -0.2)) -> Model:
+    """Small box falling and sliding onto the ground plane (contact + friction)."""
+    p, t = _mesh.box_tet_mesh((0.0, 0.0, clearance), (size[0], size[1], size[2] + clearance), cells)
+    mu, lam = _mesh.lame(E, 0.4)
+    m = build_model(p, t, _materials(mu, lam, len(t)), 1000.0, shapes=[Shape(SHAPE_PLANE, mu=friction_mu)],
+                    velocity=np.asarray(velocity), name="drop_box")
+    m.params.update(energy=energy, iterations=8, cg_max_iterations=2000, cg_tolerance=1e-6,
+                    line_search=True, line_search_max_iterations=20, contact_d0=0.1 * dhat, contact_d1=dhat,
+                    contact_stiffness=auto_kappa(m, dhat), friction_mu=[friction_mu], friction_eps_v=1e-3)
+    return m
+
+
+def capsule_poke_scene(cells=(6, 6, 2), energy="arap") -> Model:
+    """Slab pinned at its base, poked from above by a tilted capsule (triangle
+    and vertex contact against a capsule, src/mfem/refinement/contact.py)."""
+    p, t = _mesh.box_tet_mesh((0.0, 0.0, 0.0), (0.3, 0.3, 0.1), cells)
+    mu, lam = _mesh.lame(5.0e4, 0.4)
+    pinned = np.nonzero(p[:, 2] < 1e-9)[0]
+    # capsule axis along world x (local Z rotated 90 degrees about y), its
+    # surface 2 mm above the slab's top face
+    cap = Shape(SHAPE_CAPSULE, pos=np.array([0.15, 0.15, 0.1 + 0.012]),
+                quat=np.array([0.0, np.sin(np.pi / 4), 0.0, np.cos(np.pi / 4)]),
+                scale=np.array([0.01, 0.08, 0.0]), mu=0.2)
+    m = build_model(p, t, _materials(mu, lam, len(t)), 1000.0, pinned=pinned, shapes=[cap], gravity=(0, 0, -9.81),
+                    name="capsule_poke")
+    m.params.update(energy=energy, iterations=6, cg_max_iterations=2000, cg_tolerance=1e-6,
+                    line_search=True, line_search_max_iterations=20, contact_d0=1e-3, contact_d1=5e-3,
+                    contact_stiffness=50.0, friction_mu=[0.2], friction_eps_v=1e-3, attachment_stiffness=10.0,
+                    tri_contact=True)
+    return m
