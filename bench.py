@@ -83,7 +83,15 @@ def main():
     if solver.status() != 0:
         raise SystemExit("device error during the timed steps")
 
-    st = np.stack([h.cpu().numpy().reshape(64, 8)[: solver.iterations] for h in hist])  # (K, iters, 8)
+    from mfem_amd import _ffi
+
+    st = np.stack([h.cpu().numpy().reshape(_ffi.MFEM_MAX_STATS_ITERS, _ffi.MFEM_STATS_PER_ITER)[: solver.iterations]
+                   for h in hist])  # (K, iters, stats)
+    khz = solver.lib.mfem_wall_clock_khz()
+    ticks = st[:, :, 8:14].sum(axis=(0, 1))
+    n_it = max(1.0, float(st[:, :, 0].sum()))
+    pcg_phase_us = {name: float(ticks[k] / khz * 1e3 / n_it) for k, name in
+                    enumerate(["spmv", "reduce1", "update", "reduce2", "checks", "prologue"])}
     cg_iters = st[:, :, 0]
     relres = st[:, :, 1]
     assert np.all(relres <= 1e-6), "a PCG solve missed rel-res 1e-6"
@@ -118,11 +126,16 @@ def main():
             "newton_iterations": solver.iterations,
             "dt": dt,
             "preconditioner": "block_jacobi",
+            "pcg": "pipelined" if solver._S.pcg_flags & 1 else "standard",
             "learned_preconditioner": False,
             "reductions": "f64",
             "cg_iters_mean": float(cg_iters.mean()),
             "cg_iters_per_step": float(cg_iters.sum(axis=1).mean()),
             "cg_relres_max": float(relres.max()),
+            "cg_iters_last_step": cg_iters[-1].astype(int).tolist(),
+            "cg_restarts_last_step": st[-1, :, 7].astype(int).tolist(),
+            "ls_trials_last_step": st[-1, :, 2].astype(int).tolist(),
+            "pcg_phase_us_per_iter": pcg_phase_us,
             "phase_ms_per_step": {k: v / args.steps for k, v in phase_ms.items()},
         },
         "roofline": {

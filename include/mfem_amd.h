@@ -39,9 +39,17 @@ extern "C" {
 #define MFEM_SHAPE_PLANE 0
 #define MFEM_SHAPE_CAPSULE 1
 
-#define MFEM_STATS_PER_ITER 8 /* cg_iters, cg_relres, ls_trials, ls_accepted, alpha, f0, f1, restarts */
+/* stats[iter][k]: 0 cg_iters, 1 cg_relres (FP64, true residual), 2 ls_trials,
+   3 ls_accepted, 4 alpha, 5 merit f0, 6 merit f1, 7 cg_restarts,
+   8..13 PCG phase device-clock ticks (SpMV, reduce 1, update, reduce 2,
+   residual checks, prologue) */
+#define MFEM_STATS_PER_ITER 16
 #define MFEM_MAX_STATS_ITERS 64
 #define MFEM_RED_MAX_BLOCKS 2048
+
+/* pcg_flags bits */
+#define MFEM_PCG_PIPELINED 1   /* one grid reduction per CG iteration (pipelined PCG) */
+#define MFEM_PCG_STREAM_ROWS 2 /* stream matrix rows from HBM instead of keeping them in VGPRs */
 
 /* status[] words */
 #define MFEM_STATUS_ERRBITS 0 /* bit 0: grid barrier timed out */
@@ -81,7 +89,9 @@ typedef struct mfem_solver {
   float d0, d1, kappa, eps_v;
   float qb[3];      /* barrier(d0), barrier'(d0), barrier''(d0) */
   float gravity[3]; /* model.gravity */
-  float pad0;
+  int32_t max_width; /* widest chunk, in slots (max row length of the pattern) */
+  int32_t pcg_flags; /* MFEM_PCG_* bits */
+  int32_t pad1;
   /* model / topology (device) */
   const int32_t* tets;      /* [n_tets][4] */
   const float* dminv;       /* [n_tets][9] tet_poses, row-major */
@@ -135,7 +145,7 @@ typedef struct mfem_solver {
   double* dx;   /* [n_verts][3] PCG solution */
   float* ds;    /* [n_tets][6] stretch update */
   /* persistent-kernel scratch */
-  float* pz;    /* [n_chunks*64][3] published PCG direction */
+  float* pz;    /* [2][n_chunks*64][3] published PCG vector (double-buffered) */
   double* px;   /* [n_chunks*64][3] published PCG iterate (residual check) */
   double* red;  /* [2][MFEM_RED_MAX_BLOCKS][4] grid reduction partials */
   uint32_t* flags; /* [MFEM_RED_MAX_BLOCKS] grid barrier arrival epochs */
@@ -147,6 +157,7 @@ typedef struct mfem_solver {
 int64_t mfem_solver_sizeof(void);
 int64_t mfem_solver_offsetof(const char* field);
 int mfem_version(void);
+int mfem_wall_clock_khz(void); /* rate of the device clock used in stats[8..13], -1 on error */
 const char* mfem_last_error(void);
 
 /* AdditionalState.from_model stretch init: s = sym6(S(F(x)))

@@ -25,11 +25,11 @@ def _solver_kwargs(model):
     return kw
 
 
-def _make(model, device):
+def _make(model, device, **extra):
     from mfem_amd.solver import RefinementSolver, model_state
 
     solver = RefinementSolver(model, model.params["iterations"], max_tets=model.tet_count, device=device,
-                              **_solver_kwargs(model))
+                              **_solver_kwargs(model), **extra)
     return solver, model_state(model, device)
 
 
@@ -99,11 +99,20 @@ def test_newton_stages_match_oracle(gpu, name):
     nt = m.tet_count
     _close(B["x_tilde"].ravel(), o.buf("x_tilde"), 1e-6, "x_tilde")
     _close(B["G"].ravel(), o.buf("G"), 2e-4, "G")
-    _close(B["gs"].ravel(), o.buf("gs"), 2e-4, "g_s")
+    # g_s = vol mu (SYM s - 1) + ... cancels to FP32 noise near the rest
+    # stretch: scale its tolerance by the size of the terms, vol * mu * |s|
+    gs_scale = np.max(np.abs(1.0 / (6.0 * np.linalg.det(m.tet_poses))) * m.tet_materials[:, 0]) * 3.0
+    err = np.abs(B["gs"].ravel() - o.buf("gs")).max()
+    print(f"g_s: max abs err {err:.3e}, term scale {gs_scale:.3e}")
+    assert err <= 2e-6 * gs_scale
     _close(_unpack21(B["Hl"]).ravel(), o.buf("Hl"), 2e-4, "H_lambda")
     _close(_unpack21(B["Hsinv"]).ravel(), o.buf("Hsinv"), 2e-4, "H_s^-1")
     _close(B["gl"].ravel(), o.buf("gl"), 2e-4, "g_lambda")
-    _close(B["psi"].ravel(), o.buf("psi"), 2e-4, "psi")
+    # psi also cancels to ~0 at the rest stretch: tolerance from the term sizes
+    psi_scale = np.max(np.abs(1.0 / (6.0 * np.linalg.det(m.tet_poses))) * (m.tet_materials[:, 0] + m.tet_materials[:, 1]))
+    err = np.abs(B["psi"].ravel() - o.buf("psi")).max()
+    print(f"psi: max abs err {err:.3e}, term scale {psi_scale:.3e}")
+    assert err <= 2e-6 * psi_scale
     _close(B["gB"].ravel(), o.buf("gB"), 2e-4, "barrier grad")
     _close(B["gD"].ravel(), o.buf("gD"), 2e-4, "friction grad")
     Hg = solver.system_csr().toarray()
@@ -172,6 +181,25 @@ def test_full_steps_match_fp64_oracle(gpu, name, steps):
     disp = np.abs(o.x - m.particle_q).max()
     print("displacement", disp, "abs err", np.abs(x - o.x).max())
     assert np.all(np.isfinite(v))
+
+
+@pytest.mark.parametrize("pipelined", [True, False])
+@pytest.mark.parametrize("resident", [True, False])
+def test_pcg_variants_match_oracle(gpu, pipelined, resident):
+    # every persistent-PCG variant: standard / pipelined, rows in VGPRs / streamed
+    m, dt = _perturbed_scene("drop_box")
+    solver, state = _make(m, gpu, pcg_pipelined=pipelined, pcg_resident_rows=resident)
+    o = Oracle(m, 64)
+    out = state.clone()
+    for k in range(4):
+        solver.step(state, out, None, None, dt)
+        state, out = out, state
+        o.step(dt)
+        st = solver.last_stats()
+        print("variant", pipelined, resident, "cg", st[:, 0].astype(int).tolist(), "restarts", st[:, 7].astype(int).tolist())
+        assert np.all(st[:, 1] <= 1e-6), st
+        assert solver.status() == 0
+    _close(state.particle_q.double().cpu().numpy(), o.x, 1e-4, f"positions pipelined={pipelined} resident={resident}")
 
 
 def test_graph_free_repeatability(gpu):

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -411,12 +412,18 @@ __global__ void __launch_bounds__(kBlock) k_rhs(mfem_solver S, float dt) {
 // ---------------------------------------------------------------------------
 // grid-wide reduction + barrier for the persistent kernels
 //
-// Each workgroup writes its partial sums and then its arrival epoch
-// (release); every workgroup waits until all epochs match (acquire) and sums
-// the partials in the same fixed order, so every workgroup obtains the
-// bit-identical total and takes the same branch.  Partials ping-pong between
-// two slots by epoch parity.  Waits are bounded (kBarrierTimeout): on timeout
-// the error bit is set and every workgroup leaves the kernel.
+// Cross-workgroup data (published vectors, partial sums, arrival epochs) is
+// written and read with agent-scope coherent accesses (sc1: write-through to
+// memory on store, no stale L1/L2 hit on load).  Arrival protocol: every wave
+// waits for its coherent stores to complete (s_waitcnt vmcnt(0)), the
+// workgroup synchronises, thread 0 stores its FP64 partials, waits again and
+// stores its arrival epoch.  Every workgroup polls all epochs, then sums the
+// partials in one fixed order, so every workgroup obtains the bit-identical
+// total and takes the same branch.  This avoids agent-scope release/acquire
+// fences, whose L2 write-back / invalidate cost ~13 us per barrier at ~180
+// workgroups on MI355X (vs ~4 us for this protocol).  Partials ping-pong
+// between two slots by epoch parity.  Waits are bounded (kBarrierTimeout): on
+// timeout the error bit is set and every workgroup leaves the kernel.
 // ---------------------------------------------------------------------------
 struct GridSync {
   double* red;
@@ -425,6 +432,18 @@ struct GridSync {
   uint32_t epoch;
   bool abort;
 };
+
+// s_waitcnt vmcnt(0): all of this wave's memory operations have completed
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+template <typename T>
+__device__ __forceinline__ void pub_store(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T pub_load(const T* p) {
+  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -445,6 +464,7 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
     double w = wave_sum(v[i]);
     if (lane == 0) sm[wave][i] = w;
   }
+  wait_vmem();  // this wave's published data has reached the coherence point
   __syncthreads();
   gs.epoch += 1u;
   const uint32_t ep = gs.epoch;
@@ -456,17 +476,17 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
       double acc = sm[0][i];
 #pragma unroll
       for (int w = 1; w < kWaves; ++w) acc += sm[w][i];
-      gs.red[((size_t)slot * MFEM_RED_MAX_BLOCKS + blockIdx.x) * 4 + i] = acc;
+      pub_store(&gs.red[((size_t)slot * MFEM_RED_MAX_BLOCKS + blockIdx.x) * 4 + i], acc);
     }
-    __threadfence();
-    __hip_atomic_store(&gs.flags[blockIdx.x], ep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    wait_vmem();
+    pub_store(&gs.flags[blockIdx.x], ep);
   }
   bool to = false;
   for (int b = threadIdx.x; b < nb && !to; b += kBlock) {
-    uint64_t t0 = wall_clock64();
-    while (__hip_atomic_load(&gs.flags[b], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != ep) {
-      if (wall_clock64() - t0 > kBarrierTimeout ||
-          __hip_atomic_load(&gs.status[MFEM_STATUS_ERRBITS], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+    if (pub_load(&gs.flags[b]) == ep) continue;
+    const uint64_t t0 = wall_clock64();
+    while (pub_load(&gs.flags[b]) != ep) {
+      if (wall_clock64() - t0 > kBarrierTimeout || pub_load(&gs.status[MFEM_STATUS_ERRBITS]) != 0) {
         to = true;
         break;
       }
@@ -482,11 +502,10 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
     gs.abort = true;
     return false;
   }
-  __threadfence();
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     double acc = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) acc += gs.red[((size_t)slot * MFEM_RED_MAX_BLOCKS + b) * 4 + i];
+    for (int b = threadIdx.x; b < nb; b += kBlock) acc += pub_load(&gs.red[((size_t)slot * MFEM_RED_MAX_BLOCKS + b) * 4 + i]);
     acc = wave_sum(acc);
     if (lane == 0) tot[wave][i] = acc;
   }
@@ -506,58 +525,140 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
 // persistent block-Jacobi PCG (warp.optim.linear.cg semantics: stop when the
 // residual norm <= tol * |g|, at most cg_maxiter iterations)
 //
-// Each wave owns up to CPW chunks of 64 rows and keeps every per-row vector
-// (x in FP64; r, p, q, z and the 3x3 inverse diagonal in FP32) in registers
-// for the whole solve.  Only z (FP32) is published to HBM for the neighbours'
-// SpMV.  Two grid reductions per iteration:
+// Each wave owns CPW chunks of 64 rows and keeps every per-row vector in
+// registers for the whole solve.  With RW > 0 (one chunk per wave, chunk
+// width <= RW) the wave's matrix rows (RW slots x 9 values + columns) are
+// loaded into VGPRs once, so an iteration reads only the published vector.
+// Published vectors ping-pong between two buffers (publish count parity).
+//
+// Standard variant (PIPE = false), two grid reductions per iteration:
 //   1. y = H z; q = y + beta q (= H p); p = z + beta p; reduce p.q
 //   2. x += alpha p; r -= alpha q; z = M r; publish z; reduce r.z, r.r
-// When the recurrence residual passes, the true residual g - H x is
-// recomputed with FP64 x and FP64 accumulation; if it fails the iteration
-// restarts from it (residual replacement).
+// Pipelined variant (PIPE = true; Ghysels & Vanroose 2014, "Hiding global
+// synchronization latency in the preconditioned conjugate gradient
+// algorithm", Alg. 4), one grid reduction per iteration that also publishes
+// m = M w for the single SpMV n = H m.
+// Both: when the recurrence residual passes (at tol/2), the true residual
+// g - H x is recomputed with the FP64 iterate and FP64 accumulation; if it
+// fails tol, the iteration restarts from it (residual replacement, <= 16).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void spmv_row_f32(const mfem_solver& S, int c, int lane, const float* vec, float* out) {
-  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+template <typename T, typename A, int GRP = 8>
+__device__ __forceinline__ void spmv_row_stream(const mfem_solver& S, int c, int lane, const T* vec, A* out,
+                                                bool coherent) {
+  A a0 = 0, a1 = 0, a2 = 0;
   const int s0 = S.chunk_ptr[c], s1 = S.chunk_ptr[c + 1];
-  for (int s = s0; s < s1; ++s) {
-    const int j = S.sell_col[s * 64 + lane];
-    const float* vb = S.sell_val + (size_t)s * 576 + lane;
-    float v0 = vec[3 * j], v1 = vec[3 * j + 1], v2 = vec[3 * j + 2];
-    a0 += vb[0] * v0 + vb[64] * v1 + vb[128] * v2;
-    a1 += vb[192] * v0 + vb[256] * v1 + vb[320] * v2;
-    a2 += vb[384] * v0 + vb[448] * v1 + vb[512] * v2;
+  for (int sb = s0; sb < s1; sb += GRP) {
+    int col[GRP];
+    float vv[GRP][9];
+    T g[GRP][3];
+#pragma unroll
+    for (int k = 0; k < GRP; ++k) {
+      const bool ok = sb + k < s1;
+      col[k] = ok ? S.sell_col[(sb + k) * 64 + lane] : 0;
+      const float* vb = S.sell_val + (size_t)(ok ? sb + k : sb) * 576 + lane;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) vv[k][q] = ok ? vb[64 * q] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < GRP; ++k)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) g[k][a] = coherent ? pub_load(&vec[3 * col[k] + a]) : vec[3 * col[k] + a];
+#pragma unroll
+    for (int k = 0; k < GRP; ++k) {
+      a0 += (A)vv[k][0] * g[k][0] + (A)vv[k][1] * g[k][1] + (A)vv[k][2] * g[k][2];
+      a1 += (A)vv[k][3] * g[k][0] + (A)vv[k][4] * g[k][1] + (A)vv[k][5] * g[k][2];
+      a2 += (A)vv[k][6] * g[k][0] + (A)vv[k][7] * g[k][1] + (A)vv[k][8] * g[k][2];
+    }
   }
   out[0] = a0;
   out[1] = a1;
   out[2] = a2;
 }
 
-__device__ __forceinline__ void spmv_row_f64(const mfem_solver& S, int c, int lane, const double* vec, double* out) {
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  const int s0 = S.chunk_ptr[c], s1 = S.chunk_ptr[c + 1];
-  for (int s = s0; s < s1; ++s) {
-    const int j = S.sell_col[s * 64 + lane];
-    const float* vb = S.sell_val + (size_t)s * 576 + lane;
-    double v0 = vec[3 * j], v1 = vec[3 * j + 1], v2 = vec[3 * j + 2];
-    a0 += (double)vb[0] * v0 + (double)vb[64] * v1 + (double)vb[128] * v2;
-    a1 += (double)vb[192] * v0 + (double)vb[256] * v1 + (double)vb[320] * v2;
-    a2 += (double)vb[384] * v0 + (double)vb[448] * v1 + (double)vb[512] * v2;
+template <int RW>
+__device__ __forceinline__ void spmv_row_resident(const float (&val)[RW][9], const int (&col)[RW], const float* vec,
+                                                  float* out) {
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  constexpr int GRP = 8;  // gathers in flight per group
+#pragma unroll
+  for (int kb = 0; kb < RW; kb += GRP) {
+    float g[GRP][3];
+#pragma unroll
+    for (int k = 0; k < GRP; ++k)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) g[k][a] = pub_load(&vec[3 * col[kb + k] + a]);
+#pragma unroll
+    for (int k = 0; k < GRP; ++k) {
+      const float* v = val[kb + k];
+      a0 += v[0] * g[k][0] + v[1] * g[k][1] + v[2] * g[k][2];
+      a1 += v[3] * g[k][0] + v[4] * g[k][1] + v[5] * g[k][2];
+      a2 += v[6] * g[k][0] + v[7] * g[k][1] + v[8] * g[k][2];
+    }
   }
   out[0] = a0;
   out[1] = a1;
   out[2] = a2;
+}
+
+// resident-matrix storage; RW == 0 means "stream from HBM"
+template <int RW>
+struct RowBlocks {
+  float val[RW][9];
+  int col[RW];
+};
+template <>
+struct RowBlocks<0> {};
+
+template <int CPW, int RW>
+__device__ __forceinline__ void spmv_rows(const mfem_solver& S, const int (&chunk)[CPW], const bool (&own)[CPW],
+                                          int lane, const RowBlocks<RW>& rb, const float* vec, float (&y)[CPW][3]) {
+  if constexpr (RW > 0) {
+    static_assert(CPW == 1, "resident rows need one chunk per wave");
+    spmv_row_resident<RW>(rb.val, rb.col, vec, y[0]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+      y[k][0] = y[k][1] = y[k][2] = 0.f;
+      if (own[k]) spmv_row_stream<float, float>(S, chunk[k], lane, vec, y[k], true);
+    }
+  }
+}
+
+// FP64 iterate, FP64 accumulation: only for the true-residual checks, so it
+// always streams the matrix and keeps few registers live
+template <int CPW>
+__device__ __forceinline__ void spmv_rows64(const mfem_solver& S, const int (&chunk)[CPW], const bool (&own)[CPW],
+                                            int lane, const double* vec, double (&y)[CPW][3]) {
+#pragma unroll
+  for (int k = 0; k < CPW; ++k) {
+    y[k][0] = y[k][1] = y[k][2] = 0.0;
+    if (own[k]) spmv_row_stream<double, double, 2>(S, chunk[k], lane, vec, y[k], true);
+  }
 }
 
 template <int CPW>
+__device__ __forceinline__ void apply_M(const float (&M)[CPW][9], int k, const double* r, float* z) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) z[a] = (float)(M[k][3 * a] * r[0] + M[k][3 * a + 1] * r[1] + M[k][3 * a + 2] * r[2]);
+}
+
+template <int CPW, int RW, bool PIPE>
 __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) {
   GridSync gs{S.red, S.flags, S.status, (uint32_t)S.status[MFEM_STATUS_EPOCH], false};
   const int lane = threadIdx.x & 63;
   const int gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
   const int nw = gridDim.x * kWaves;
+  const size_t nrow_pad = (size_t)S.n_chunks * 64;
   int chunk[CPW];
   bool live[CPW], own[CPW];
-  double x[CPW][3];
-  float r[CPW][3], p[CPW][3], q[CPW][3], z[CPW][3], M[CPW][9], b[CPW][3];
+  double x[CPW][3], r[CPW][3];
+  float M[CPW][9], b[CPW][3];
+  // standard: p, q, z.  pipelined: u, w (FP64) and p, s, q, z (FP32)
+  float p[CPW][3], q[CPW][3], z[CPW][3], sv[CPW][3];
+  double u[CPW][3], w[CPW][3];
+  RowBlocks<RW> rb;
+  int pub = 0;  // publish counter: buffer = pub & 1
+  auto pzbuf = [&](int k) { return S.pz + (size_t)(k & 1) * nrow_pad * 3; };
 
   double bb = 0.0;
 #pragma unroll
@@ -568,9 +669,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
     live[k] = own[k] && row < S.n_verts;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      x[k][a] = 0.0;
+      x[k][a] = r[k][a] = u[k][a] = w[k][a] = 0.0;
       b[k][a] = 0.f;
-      p[k][a] = q[k][a] = r[k][a] = z[k][a] = 0.f;
+      p[k][a] = q[k][a] = z[k][a] = sv[k][a] = 0.f;
     }
 #pragma unroll
     for (int a = 0; a < 9; ++a) M[k][a] = (a % 4 == 0) ? 1.f : 0.f;
@@ -590,151 +691,262 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
       bb += (double)b[k][0] * b[k][0] + (double)b[k][1] * b[k][1] + (double)b[k][2] * b[k][2];
     }
     if (own[k]) {
-      const int row2 = chunk[k] * 64 + lane;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) S.px[3 * row2 + a] = x[k][a];
+      for (int a = 0; a < 3; ++a) pub_store(&S.px[3 * row + a], x[k][a]);
     }
+  }
+  if constexpr (RW > 0) {
+    // this wave's matrix rows, resident for the whole solve (padding: 0, col 0)
+    const int c = chunk[0];
+    const int s0 = own[0] ? S.chunk_ptr[c] : 0;
+    const int wd = own[0] ? S.chunk_ptr[c + 1] - s0 : 0;
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const bool ok = k < wd;
+      rb.col[k] = ok ? S.sell_col[(s0 + k) * 64 + lane] : 0;
+      const float* vb = S.sell_val + (size_t)(ok ? s0 + k : 0) * 576 + lane;
+#pragma unroll
+      for (int a = 0; a < 9; ++a) rb.val[k][a] = ok ? vb[64 * a] : 0.f;
+    }
+  }
+  // device-clock profile of the solve phases (wave 0 of workgroup 0 only)
+  const bool prof = blockIdx.x == 0 && threadIdx.x < 64;
+  uint64_t tk[6] = {0, 0, 0, 0, 0, 0};
+  uint64_t tprev = wall_clock64();
+#define PCG_LAP(k)                        \
+  if (prof) {                             \
+    const uint64_t tnow = wall_clock64(); \
+    tk[k] += tnow - tprev;                \
+    tprev = tnow;                         \
   }
   double red1[1] = {bb};
   if (!grid_reduce(gs, red1)) return;
   const double bnorm2 = red1[0];
   const double tol2 = (double)S.cg_tol * (double)S.cg_tol * bnorm2;
+  // the recurrence runs to a 2x tighter residual so that the FP64 re-check
+  // normally passes without a restart
+  const double tol2_rec = 0.25 * tol2;
   int it = 0, restarts = 0;
   double rt2 = 0.0;
 
-  if (bnorm2 > 0.0) {
-    // r0 = g - H x0 (FP64), z0 = M r0
-    double rz = 0.0, rr = 0.0;
+  // (re)start from the current r: z = M r, publish z; PIPE also w = H u
+  // returns false on abort; rz, rr reduced
+  auto restart = [&](double& rz, double& rr) -> bool {
+    rz = 0.0;
+    rr = 0.0;
+    float* zb = pzbuf(pub);
 #pragma unroll
     for (int k = 0; k < CPW; ++k) {
       if (!own[k]) continue;
-      double y[3];
-      spmv_row_f64(S, chunk[k], lane, S.px, y);
       const int row = chunk[k] * 64 + lane;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) r[k][a] = live[k] ? (float)((double)b[k][a] - y[a]) : 0.f;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) z[k][a] = M[k][3 * a] * r[k][0] + M[k][3 * a + 1] * r[k][1] + M[k][3 * a + 2] * r[k][2];
+      apply_M<CPW>(M, k, r[k], z[k]);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        S.pz[3 * row + a] = z[k][a];
-        rz += (double)r[k][a] * z[k][a];
-        rr += (double)r[k][a] * r[k][a];
+        pub_store(&zb[3 * row + a], z[k][a]);
+        u[k][a] = (double)z[k][a];
+        rz += r[k][a] * z[k][a];
+        rr += r[k][a] * r[k][a];
       }
     }
+    ++pub;
     double red2[2] = {rz, rr};
-    if (!grid_reduce(gs, red2)) return;
+    if (!grid_reduce(gs, red2)) return false;
     rz = red2[0];
     rr = red2[1];
-    double beta = 0.0;
+    if constexpr (PIPE) {
+      float y[CPW][3];
+      spmv_rows<CPW, RW>(S, chunk, own, lane, rb, zb, y);
+#pragma unroll
+      for (int k = 0; k < CPW; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) w[k][a] = own[k] ? (double)y[k][a] : 0.0;
+    }
+    return true;
+  };
+
+  if (bnorm2 > 0.0) {
+    double rz = 0.0, rr = 0.0;
+    {
+      double y[CPW][3];
+      spmv_rows64<CPW>(S, chunk, own, lane, S.px, y);
+#pragma unroll
+      for (int k = 0; k < CPW; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) r[k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
+    }
+    if (!restart(rz, rr)) return;
+    bool first = true;
+    double gamma_prev = 1.0, alpha_prev = 1.0, beta = 0.0;
+    PCG_LAP(5);
     for (;;) {
-      if (rr <= tol2 || it >= S.cg_maxiter) {
-        // true residual with FP64 iterate
+      double gamma = 0.0, delta = 0.0;
+      if constexpr (PIPE) {
+        // gamma = r.u, delta = w.u, rr = r.r; m = M w published with them
+        rr = 0.0;
+        float* mb = pzbuf(pub);
+#pragma unroll
+        for (int k = 0; k < CPW; ++k) {
+          if (!own[k]) continue;
+          const int row = chunk[k] * 64 + lane;
+          float m[3];
+          apply_M<CPW>(M, k, w[k], m);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            pub_store(&mb[3 * row + a], m[a]);
+            gamma += r[k][a] * u[k][a];
+            delta += w[k][a] * u[k][a];
+            rr += r[k][a] * r[k][a];
+          }
+        }
+        ++pub;
+        PCG_LAP(2);
+        double red[3] = {gamma, delta, rr};
+        if (!grid_reduce(gs, red)) return;
+        gamma = red[0];
+        delta = red[1];
+        rr = red[2];
+        PCG_LAP(1);
+      }
+      if (rr <= tol2_rec || it >= S.cg_maxiter) {
+        // true residual with the FP64 iterate
 #pragma unroll
         for (int k = 0; k < CPW; ++k) {
           if (!own[k]) continue;
           const int row = chunk[k] * 64 + lane;
 #pragma unroll
-          for (int a = 0; a < 3; ++a) S.px[3 * row + a] = x[k][a];
+          for (int a = 0; a < 3; ++a) pub_store(&S.px[3 * row + a], x[k][a]);
         }
         double red0[1] = {0.0};
         if (!grid_reduce(gs, red0)) return;
         double acc = 0.0;
-        float rt[CPW][3];
+        double rt[CPW][3];
+        {
+          double y[CPW][3];
+          spmv_rows64<CPW>(S, chunk, own, lane, S.px, y);
 #pragma unroll
-        for (int k = 0; k < CPW; ++k) {
+          for (int k = 0; k < CPW; ++k)
 #pragma unroll
-          for (int a = 0; a < 3; ++a) rt[k][a] = 0.f;
-          if (!live[k]) continue;
-          double y[3];
-          spmv_row_f64(S, chunk[k], lane, S.px, y);
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            double d = (double)b[k][a] - y[a];
-            rt[k][a] = (float)d;
-            acc += d * d;
-          }
+            for (int a = 0; a < 3; ++a) {
+              rt[k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
+              acc += rt[k][a] * rt[k][a];
+            }
         }
         double redt[1] = {acc};
         if (!grid_reduce(gs, redt)) return;
         rt2 = redt[0];
+        PCG_LAP(4);
         if (rt2 <= tol2 || it >= S.cg_maxiter || restarts >= 16) break;
         ++restarts;
-        rz = 0.0;
+#pragma unroll
+        for (int k = 0; k < CPW; ++k)
+#pragma unroll
+          for (int a = 0; a < 3; ++a) r[k][a] = rt[k][a];
+        if (!restart(rz, rr)) return;
+        first = true;
+        beta = 0.0;
+        PCG_LAP(4);
+        continue;
+      }
+      if constexpr (PIPE) {
+        // n = H m; scalars; all vector recurrences local
+        float n[CPW][3];
+        spmv_rows<CPW, RW>(S, chunk, own, lane, rb, pzbuf(pub - 1), n);
+        PCG_LAP(0);
+        double alpha;
+        if (first) {
+          beta = 0.0;
+          alpha = gamma / delta;
+        } else {
+          beta = gamma / gamma_prev;
+          alpha = gamma / (delta - beta * gamma / alpha_prev);
+        }
+        if (!(alpha > 0.0) || !isfinite(alpha)) {  // breakdown: fall to the residual check
+          it = S.cg_maxiter;
+          continue;
+        }
+        const float fb = (float)beta;
+#pragma unroll
+        for (int k = 0; k < CPW; ++k) {
+          if (!own[k]) continue;
+          float m[3];
+          apply_M<CPW>(M, k, w[k], m);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            z[k][a] = n[k][a] + fb * z[k][a];
+            q[k][a] = m[a] + fb * q[k][a];
+            sv[k][a] = (float)w[k][a] + fb * sv[k][a];
+            p[k][a] = (float)u[k][a] + fb * p[k][a];
+            x[k][a] += alpha * (double)p[k][a];
+            r[k][a] -= alpha * (double)sv[k][a];
+            u[k][a] -= alpha * (double)q[k][a];
+            w[k][a] -= alpha * (double)z[k][a];
+          }
+        }
+        PCG_LAP(2);
+        gamma_prev = gamma;
+        alpha_prev = alpha;
+        first = false;
+        ++it;
+        continue;
+      } else {
+        // phase 1: q = H p via y = H z
+        double pq = 0.0;
+        const float fb = (float)beta;
+        {
+          float y[CPW][3];
+          spmv_rows<CPW, RW>(S, chunk, own, lane, rb, pzbuf(pub - 1), y);
+#pragma unroll
+          for (int k = 0; k < CPW; ++k) {
+            if (!own[k]) continue;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              q[k][a] = y[k][a] + fb * q[k][a];
+              p[k][a] = z[k][a] + fb * p[k][a];
+              pq += (double)p[k][a] * q[k][a];
+            }
+          }
+        }
+        PCG_LAP(0);
+        double redp[1] = {pq};
+        if (!grid_reduce(gs, redp)) return;
+        pq = redp[0];
+        PCG_LAP(1);
+        if (!(pq > 0.0)) {  // breakdown (non-SPD or exact solution): fall to the residual check
+          it = S.cg_maxiter;
+          continue;
+        }
+        const double alpha = rz / pq;
+        double rz2 = 0.0;
         rr = 0.0;
+        float* zb = pzbuf(pub);
 #pragma unroll
         for (int k = 0; k < CPW; ++k) {
           if (!own[k]) continue;
           const int row = chunk[k] * 64 + lane;
 #pragma unroll
-          for (int a = 0; a < 3; ++a) r[k][a] = rt[k][a];
-#pragma unroll
-          for (int a = 0; a < 3; ++a) z[k][a] = M[k][3 * a] * r[k][0] + M[k][3 * a + 1] * r[k][1] + M[k][3 * a + 2] * r[k][2];
+          for (int a = 0; a < 3; ++a) {
+            x[k][a] += alpha * (double)p[k][a];
+            r[k][a] -= alpha * (double)q[k][a];
+          }
+          apply_M<CPW>(M, k, r[k], z[k]);
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
-            S.pz[3 * row + a] = z[k][a];
-            rz += (double)r[k][a] * z[k][a];
-            rr += (double)r[k][a] * r[k][a];
+            pub_store(&zb[3 * row + a], z[k][a]);
+            rz2 += r[k][a] * z[k][a];
+            rr += r[k][a] * r[k][a];
           }
         }
-        double red3[2] = {rz, rr};
-        if (!grid_reduce(gs, red3)) return;
-        rz = red3[0];
-        rr = red3[1];
-        beta = 0.0;
-        continue;
+        ++pub;
+        PCG_LAP(2);
+        double redz[2] = {rz2, rr};
+        if (!grid_reduce(gs, redz)) return;
+        rz2 = redz[0];
+        rr = redz[1];
+        PCG_LAP(3);
+        beta = rz2 / rz;
+        rz = rz2;
+        ++it;
       }
-      // phase 1: q = H p via y = H z
-      double pq = 0.0;
-      const float fb = (float)beta;
-#pragma unroll
-      for (int k = 0; k < CPW; ++k) {
-        if (!own[k]) continue;
-        float y[3];
-        spmv_row_f32(S, chunk[k], lane, S.pz, y);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          q[k][a] = y[a] + fb * q[k][a];
-          p[k][a] = z[k][a] + fb * p[k][a];
-          pq += (double)p[k][a] * q[k][a];
-        }
-      }
-      double redp[1] = {pq};
-      if (!grid_reduce(gs, redp)) return;
-      pq = redp[0];
-      if (!(pq > 0.0)) {  // breakdown (non-SPD or exact solution): fall to the residual check
-        it = S.cg_maxiter;
-        continue;
-      }
-      const double alpha = rz / pq;
-      const float fa = (float)alpha;
-      double rz2 = 0.0;
-      rr = 0.0;
-#pragma unroll
-      for (int k = 0; k < CPW; ++k) {
-        if (!own[k]) continue;
-        const int row = chunk[k] * 64 + lane;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          x[k][a] += alpha * (double)p[k][a];
-          r[k][a] -= fa * q[k][a];
-        }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) z[k][a] = M[k][3 * a] * r[k][0] + M[k][3 * a + 1] * r[k][1] + M[k][3 * a + 2] * r[k][2];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          S.pz[3 * row + a] = z[k][a];
-          rz2 += (double)r[k][a] * z[k][a];
-          rr += (double)r[k][a] * r[k][a];
-        }
-      }
-      double redz[2] = {rz2, rr};
-      if (!grid_reduce(gs, redz)) return;
-      rz2 = redz[0];
-      rr = redz[1];
-      beta = rz2 / rz;
-      rz = rz2;
-      ++it;
     }
   }
   // write the solution (g == 0 gives dx = 0)
@@ -751,9 +963,12 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
       st[0] = (double)it;
       st[1] = bnorm2 > 0.0 ? sqrt(rt2 / bnorm2) : 0.0;
       st[7] = (double)restarts;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) st[8 + k] = (double)tk[k];
     }
     S.status[MFEM_STATUS_EPOCH] = (int32_t)gs.epoch;
   }
+#undef PCG_LAP
 }
 
 // ---------------------------------------------------------------------------
@@ -973,7 +1188,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv(mfem_solver S, const float* xin
   int row = blockIdx.x * kBlock + threadIdx.x;
   if (row >= S.n_verts) return;
   float y[3];
-  spmv_row_f32(S, row >> 6, row & 63, xin, y);
+  spmv_row_stream<float, float>(S, row >> 6, row & 63, xin, y, false);
   yout[3 * row] = y[0];
   yout[3 * row + 1] = y[1];
   yout[3 * row + 2] = y[2];
@@ -1000,8 +1215,23 @@ __global__ void __launch_bounds__(kBlock) k_merit(mfem_solver S, float dt, float
 // ---------------------------------------------------------------------------
 namespace {
 
+struct PcgVariant {
+  int cpw, rw;
+  bool pipe;
+  const void* fn;
+};
+#define PCG_V(C, R, P) {C, R, P, reinterpret_cast<const void*>(&k_pcg<C, R, P>)}
+// resident-row variants first (one chunk per wave, rows in VGPRs), then the
+// streaming ones with more chunks per wave for larger systems
+const PcgVariant kPcgVariants[] = {
+    PCG_V(1, 16, false), PCG_V(1, 0, false), PCG_V(2, 0, false), PCG_V(4, 0, false), PCG_V(8, 0, false),
+    PCG_V(1, 16, true),  PCG_V(1, 0, true),  PCG_V(2, 0, true),  PCG_V(4, 0, true),  PCG_V(8, 0, true),
+};
+#undef PCG_V
+constexpr int kNumPcgVariants = sizeof(kPcgVariants) / sizeof(kPcgVariants[0]);
+
 struct CoopInfo {
-  int max_blocks_pcg[4] = {0, 0, 0, 0};
+  int max_blocks_pcg[kNumPcgVariants] = {};
   int max_blocks_ls = 0;
   bool ready = false;
 };
@@ -1016,11 +1246,9 @@ int coop_info(CoopInfo** out) {
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
     if (!coop) return fail(MFEM_EHIP, "device does not support cooperative launches");
-    const void* fns[4] = {reinterpret_cast<const void*>(&k_pcg<1>), reinterpret_cast<const void*>(&k_pcg<2>),
-                          reinterpret_cast<const void*>(&k_pcg<4>), reinterpret_cast<const void*>(&k_pcg<8>)};
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kNumPcgVariants; ++k) {
       int nb = 0;
-      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fns[k], kBlock, 0));
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kPcgVariants[k].fn, kBlock, 0));
       ci.max_blocks_pcg[k] = nb * cus;
     }
     int nb = 0;
@@ -1037,6 +1265,7 @@ int check_solver(const mfem_solver* S) {
   if (S->n_verts <= 0 || S->n_tets <= 0) return fail(MFEM_EINVAL, "empty mesh");
   if (S->n_chunks != (S->n_verts + 63) / 64) return fail(MFEM_EINVAL, "n_chunks != ceil(n_verts/64)");
   if (S->n_slots < S->n_chunks) return fail(MFEM_EINVAL, "n_slots < n_chunks");
+  if (S->max_width <= 0) return fail(MFEM_EINVAL, "max_width must be the widest chunk (slots)");
   if (S->n_shapes < 0 || S->n_tris < 0) return fail(MFEM_EINVAL, "negative count");
   if (S->iterations > MFEM_MAX_STATS_ITERS) return fail(MFEM_EINVAL, "iterations > MFEM_MAX_STATS_ITERS");
   if (!S->x || !S->v || !S->s || !S->lam || !S->tets || !S->sell_val || !S->status || !S->stats)
@@ -1057,7 +1286,7 @@ int launched() {
 #define FIELD_TABLE(X)                                                                                   \
   X(n_verts) X(n_tets) X(n_tris) X(n_shapes) X(n_chunks) X(n_slots) X(energy) X(iterations) X(cg_maxiter) \
   X(precond) X(line_search) X(ls_max) X(tri_contact) X(friction_on) X(cg_tol) X(ls_alpha0) X(ls_tau)      \
-  X(ls_c) X(ls_thr) X(sing_thr) X(attach_k) X(d0) X(d1) X(kappa) X(eps_v) X(qb) X(gravity) X(pad0) X(tets) \
+  X(ls_c) X(ls_thr) X(sing_thr) X(attach_k) X(d0) X(d1) X(kappa) X(eps_v) X(qb) X(gravity) X(max_width) X(pcg_flags) X(pad1) X(tets) \
   X(dminv) X(materials) X(vol) X(mass) X(inv_mass) X(pin_pos) X(tris) X(shape_type) X(shape_xf)            \
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
@@ -1077,6 +1306,13 @@ int64_t mfem_solver_offsetof(const char* field) {
 }
 
 int mfem_version(void) { return MFEM_VERSION; }
+
+int mfem_wall_clock_khz(void) {
+  int dev = 0, khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return -1;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) return -1;
+  return khz;
+}
 
 const char* mfem_last_error(void) { return g_last_error.c_str(); }
 
@@ -1116,11 +1352,13 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
   if (int rc = check_solver(S)) return rc;
   CoopInfo* ci = nullptr;
   if (int rc = coop_info(&ci)) return rc;
-  static const int cpw_opts[4] = {1, 2, 4, 8};
   int sel = -1;
   unsigned nblocks = 0;
-  for (int k = 0; k < 4; ++k) {
-    long waves = (S->n_chunks + cpw_opts[k] - 1) / cpw_opts[k];
+  for (int k = 0; k < kNumPcgVariants; ++k) {
+    const PcgVariant& v = kPcgVariants[k];
+    if (v.pipe != ((S->pcg_flags & MFEM_PCG_PIPELINED) != 0)) continue;
+    if (v.rw > 0 && ((S->pcg_flags & MFEM_PCG_STREAM_ROWS) || S->max_width > v.rw)) continue;
+    long waves = (S->n_chunks + v.cpw - 1) / v.cpw;
     long nb = (waves + kWaves - 1) / kWaves;
     if (nb <= ci->max_blocks_pcg[k] && nb <= MFEM_RED_MAX_BLOCKS) {
       sel = k;
@@ -1131,11 +1369,7 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
   if (sel < 0) return fail(MFEM_ECAPACITY, "system too large for the persistent PCG kernel");
   mfem_solver Sc = *S;
   void* args[] = {(void*)&Sc, (void*)&newton_iter};
-  const void* fn = sel == 0 ? reinterpret_cast<const void*>(&k_pcg<1>)
-                 : sel == 1 ? reinterpret_cast<const void*>(&k_pcg<2>)
-                 : sel == 2 ? reinterpret_cast<const void*>(&k_pcg<4>)
-                            : reinterpret_cast<const void*>(&k_pcg<8>);
-  HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(kBlock), args, 0, (hipStream_t)stream));
+  HIP_TRY(hipLaunchCooperativeKernel(kPcgVariants[sel].fn, dim3(nblocks), dim3(kBlock), args, 0, (hipStream_t)stream));
   return launched();
 }
 

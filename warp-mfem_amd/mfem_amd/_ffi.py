@@ -21,7 +21,7 @@ FIELDS = [
     ("line_search", c_i32), ("ls_max", c_i32), ("tri_contact", c_i32), ("friction_on", c_i32),
     ("cg_tol", c_f32), ("ls_alpha0", c_f32), ("ls_tau", c_f32), ("ls_c", c_f32), ("ls_thr", c_f32),
     ("sing_thr", c_f32), ("attach_k", c_f32), ("d0", c_f32), ("d1", c_f32), ("kappa", c_f32), ("eps_v", c_f32),
-    ("qb", c_f32 * 3), ("gravity", c_f32 * 3), ("pad0", c_f32),
+    ("qb", c_f32 * 3), ("gravity", c_f32 * 3), ("max_width", c_i32), ("pcg_flags", c_i32), ("pad1", c_i32),
 ] + [(n, c_ptr) for n in (
     "tets", "dminv", "materials", "vol", "mass", "inv_mass", "pin_pos", "tris", "shape_type", "shape_xf",
     "shape_scale", "shape_mu", "chunk_ptr", "sell_col", "sell_val", "diag_lane", "contrib_ptr", "contrib_src",
@@ -36,13 +36,15 @@ class MfemSolverStruct(ctypes.Structure):
 
 
 # mirrors of the header's constants
-MFEM_STATS_PER_ITER = 8
+MFEM_STATS_PER_ITER = 16
 MFEM_MAX_STATS_ITERS = 64
 MFEM_RED_MAX_BLOCKS = 2048
 SHAPE_PLANE, SHAPE_CAPSULE = 0, 1
+PCG_PIPELINED, PCG_STREAM_ROWS = 1, 2
 
 ENTRY_POINTS = [
-    "mfem_solver_sizeof", "mfem_solver_offsetof", "mfem_version", "mfem_last_error", "mfem_init_stretch",
+    "mfem_solver_sizeof", "mfem_solver_offsetof", "mfem_version", "mfem_wall_clock_khz", "mfem_last_error",
+    "mfem_init_stretch",
     "mfem_begin_step", "mfem_update_system", "mfem_assemble", "mfem_global_solve", "mfem_local_solve",
     "mfem_line_search", "mfem_end_step", "mfem_step", "mfem_spmv", "mfem_merit",
 ]
@@ -78,7 +80,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mfem_line_search.argtypes = [P, ctypes.c_float, ctypes.c_int, vp]
     lib.mfem_spmv.argtypes = [P, vp, vp, vp]
     lib.mfem_merit.argtypes = [P, ctypes.c_float, ctypes.c_float, vp, vp]
-    for name in ENTRY_POINTS[4:]:
+    lib.mfem_wall_clock_khz.restype = ctypes.c_int
+    for name in ENTRY_POINTS[5:]:
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib

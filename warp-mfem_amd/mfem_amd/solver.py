@@ -94,6 +94,8 @@ class RefinementSolver:
         self.refine_every_n_steps = refine_every_n_steps
         self.min_refine_score = min_refine_score
         self.device = torch.device(device)
+        if self.device.type == "cuda" and self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         self._step = -1
         self._timings: dict[str, list[float]] = {}
 
@@ -170,7 +172,7 @@ class RefinementSolver:
             b[n] = zf(nv)
         b["dx"] = torch.zeros(nv, 3, dtype=torch.float64, device=dev)
         b["ds"] = zf(nt, 6)
-        b["pz"] = zf(pat.n_chunks * 64, 3)
+        b["pz"] = zf(2, pat.n_chunks * 64, 3)
         b["px"] = torch.zeros(pat.n_chunks * 64, 3, dtype=torch.float64, device=dev)
         b["red"] = torch.zeros(2 * _ffi.MFEM_RED_MAX_BLOCKS * 4, dtype=torch.float64, device=dev)
         b["flags"] = torch.zeros(_ffi.MFEM_RED_MAX_BLOCKS, dtype=torch.int32, device=dev)
@@ -181,6 +183,10 @@ class RefinementSolver:
         S = _ffi.MfemSolverStruct()
         S.n_verts, S.n_tets, S.n_tris, S.n_shapes = nv, nt, len(tris), n_shapes
         S.n_chunks, S.n_slots = pat.n_chunks, pat.n_slots
+        S.max_width = int(np.diff(pat.chunk_ptr).max())
+        # solver-specific knobs (not reference kwargs): pipelined PCG, resident rows
+        S.pcg_flags = (_ffi.PCG_PIPELINED if kwargs.get("pcg_pipelined", True) else 0) | \
+                      (0 if kwargs.get("pcg_resident_rows", True) else _ffi.PCG_STREAM_ROWS)
         S.energy = _ENERGIES[energy]
         S.iterations = self.iterations
         S.cg_maxiter = int(kwargs.get("cg_max_iterations", 1000))
