@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="octopus", choices=["octopus", "beam"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--pcg", default="standard", choices=["standard", "pipelined"])
+    ap.add_argument("--stream-rows", action="store_true", help="stream matrix rows instead of keeping them in VGPRs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -58,7 +60,8 @@ def main():
         wl = "cantilever_beam_1920_tets"
     kw = {k: v for k, v in m.params.items() if k != "iterations"}
     solver = RefinementSolver(m, m.params["iterations"], max_tets=m.tet_count, enable_refinement=False,
-                              device="cuda:0", **kw)
+                              device="cuda:0", pcg_pipelined=args.pcg == "pipelined",
+                              pcg_resident_rows=not args.stream_rows, **kw)
     state = model_state(m, "cuda:0")
     out = state.clone()
     print(f"[bench] {wl}: {m.tet_count} tets, {m.particle_count} verts, warmup {args.warmup}", flush=True)

@@ -146,7 +146,7 @@ typedef struct mfem_solver {
   float* ds;    /* [n_tets][6] stretch update */
   /* persistent-kernel scratch */
   float* pz;    /* [2][n_chunks*64][3] published PCG vector (double-buffered) */
-  double* px;   /* [n_chunks*64][3] published PCG iterate (residual check) */
+  double* px;   /* [2][n_chunks*64][3] published FP64 PCG vector (residual checks, pipelined variant) */
   double* red;  /* [2][MFEM_RED_MAX_BLOCKS][4] grid reduction partials */
   uint32_t* flags; /* [MFEM_RED_MAX_BLOCKS] grid barrier arrival epochs */
   int32_t* status; /* [8] see MFEM_STATUS_* */

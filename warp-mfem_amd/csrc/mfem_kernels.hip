@@ -540,7 +540,8 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
 // m = M w for the single SpMV n = H m.
 // Both: when the recurrence residual passes (at tol/2), the true residual
 // g - H x is recomputed with the FP64 iterate and FP64 accumulation; if it
-// fails tol, the iteration restarts from it (residual replacement, <= 16).
+// fails tol, the recurrence residual is replaced by it (at most 16 times):
+// the standard variant keeps its search direction, the pipelined one restarts.
 // ---------------------------------------------------------------------------
 template <typename T, typename A, int GRP = 8>
 __device__ __forceinline__ void spmv_row_stream(const mfem_solver& S, int c, int lane, const T* vec, A* out,
@@ -575,14 +576,13 @@ __device__ __forceinline__ void spmv_row_stream(const mfem_solver& S, int c, int
   out[2] = a2;
 }
 
-template <int RW>
-__device__ __forceinline__ void spmv_row_resident(const float (&val)[RW][9], const int (&col)[RW], const float* vec,
-                                                  float* out) {
-  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-  constexpr int GRP = 8;  // gathers in flight per group
+template <int RW, typename T, int GRP>
+__device__ __forceinline__ void spmv_row_resident(const float (&val)[RW][9], const int (&col)[RW], const T* vec,
+                                                  T* out) {
+  T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
   for (int kb = 0; kb < RW; kb += GRP) {
-    float g[GRP][3];
+    T g[GRP][3];  // gathers in flight per group
 #pragma unroll
     for (int k = 0; k < GRP; ++k)
 #pragma unroll
@@ -590,9 +590,9 @@ __device__ __forceinline__ void spmv_row_resident(const float (&val)[RW][9], con
 #pragma unroll
     for (int k = 0; k < GRP; ++k) {
       const float* v = val[kb + k];
-      a0 += v[0] * g[k][0] + v[1] * g[k][1] + v[2] * g[k][2];
-      a1 += v[3] * g[k][0] + v[4] * g[k][1] + v[5] * g[k][2];
-      a2 += v[6] * g[k][0] + v[7] * g[k][1] + v[8] * g[k][2];
+      a0 += (T)v[0] * g[k][0] + (T)v[1] * g[k][1] + (T)v[2] * g[k][2];
+      a1 += (T)v[3] * g[k][0] + (T)v[4] * g[k][1] + (T)v[5] * g[k][2];
+      a2 += (T)v[6] * g[k][0] + (T)v[7] * g[k][1] + (T)v[8] * g[k][2];
     }
   }
   out[0] = a0;
@@ -614,7 +614,7 @@ __device__ __forceinline__ void spmv_rows(const mfem_solver& S, const int (&chun
                                           int lane, const RowBlocks<RW>& rb, const float* vec, float (&y)[CPW][3]) {
   if constexpr (RW > 0) {
     static_assert(CPW == 1, "resident rows need one chunk per wave");
-    spmv_row_resident<RW>(rb.val, rb.col, vec, y[0]);
+    spmv_row_resident<RW, float, 8>(rb.val, rb.col, vec, y[0]);
   } else {
 #pragma unroll
     for (int k = 0; k < CPW; ++k) {
@@ -624,22 +624,27 @@ __device__ __forceinline__ void spmv_rows(const mfem_solver& S, const int (&chun
   }
 }
 
-// FP64 iterate, FP64 accumulation: only for the true-residual checks, so it
-// always streams the matrix and keeps few registers live
-template <int CPW>
+// FP64 vector, FP64 accumulation: the true-residual checks and the pipelined
+// variant's single SpMV per iteration (resident rows when available)
+template <int CPW, int RW, bool RESIDENT>
 __device__ __forceinline__ void spmv_rows64(const mfem_solver& S, const int (&chunk)[CPW], const bool (&own)[CPW],
-                                            int lane, const double* vec, double (&y)[CPW][3]) {
+                                            int lane, const RowBlocks<RW>& rb, const double* vec,
+                                            double (&y)[CPW][3]) {
+  if constexpr (RW > 0 && RESIDENT) {
+    spmv_row_resident<RW, double, 2>(rb.val, rb.col, vec, y[0]);
+  } else {
 #pragma unroll
-  for (int k = 0; k < CPW; ++k) {
-    y[k][0] = y[k][1] = y[k][2] = 0.0;
-    if (own[k]) spmv_row_stream<double, double, 2>(S, chunk[k], lane, vec, y[k], true);
+    for (int k = 0; k < CPW; ++k) {
+      y[k][0] = y[k][1] = y[k][2] = 0.0;
+      if (own[k]) spmv_row_stream<double, double, 2>(S, chunk[k], lane, vec, y[k], true);
+    }
   }
 }
 
-template <int CPW>
-__device__ __forceinline__ void apply_M(const float (&M)[CPW][9], int k, const double* r, float* z) {
+template <int CPW, typename T>
+__device__ __forceinline__ void apply_M(const float (&M)[CPW][9], int k, const double* r, T* z) {
 #pragma unroll
-  for (int a = 0; a < 3; ++a) z[a] = (float)(M[k][3 * a] * r[0] + M[k][3 * a + 1] * r[1] + M[k][3 * a + 2] * r[2]);
+  for (int a = 0; a < 3; ++a) z[a] = (T)(M[k][3 * a] * r[0] + M[k][3 * a + 1] * r[1] + M[k][3 * a + 2] * r[2]);
 }
 
 template <int CPW, int RW, bool PIPE>
@@ -653,14 +658,17 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
   bool live[CPW], own[CPW];
   double x[CPW][3], r[CPW][3];
   float M[CPW][9], b[CPW][3];
-  // standard: p, q, z.  pipelined: u, w (FP64) and p, s, q, z (FP32)
-  float p[CPW][3], q[CPW][3], z[CPW][3], sv[CPW][3];
-  double u[CPW][3], w[CPW][3];
+  // standard variant: p, q, z in FP32
+  float p[CPW][3], q[CPW][3], z[CPW][3];
+  // pipelined variant: u, w, p, s, q, z all FP64
+  double u[CPW][3], w[CPW][3], pd[CPW][3], sd[CPW][3], qd[CPW][3], zd[CPW][3];
   RowBlocks<RW> rb;
-  int pub = 0;  // publish counter: buffer = pub & 1
+  int pub = 0, pubx = 0;  // publish counters: buffer = count & 1
   auto pzbuf = [&](int k) { return S.pz + (size_t)(k & 1) * nrow_pad * 3; };
+  auto pxbuf = [&](int k) { return S.px + (size_t)(k & 1) * nrow_pad * 3; };
 
   double bb = 0.0;
+  double* px0 = pxbuf(pubx);
 #pragma unroll
   for (int k = 0; k < CPW; ++k) {
     chunk[k] = gw + k * nw;
@@ -670,8 +678,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       x[k][a] = r[k][a] = u[k][a] = w[k][a] = 0.0;
+      pd[k][a] = sd[k][a] = qd[k][a] = zd[k][a] = 0.0;
       b[k][a] = 0.f;
-      p[k][a] = q[k][a] = z[k][a] = sv[k][a] = 0.f;
+      p[k][a] = q[k][a] = z[k][a] = 0.f;
     }
 #pragma unroll
     for (int a = 0; a < 9; ++a) M[k][a] = (a % 4 == 0) ? 1.f : 0.f;
@@ -692,9 +701,10 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
     }
     if (own[k]) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) pub_store(&S.px[3 * row + a], x[k][a]);
+      for (int a = 0; a < 3; ++a) pub_store(&px0[3 * row + a], x[k][a]);
     }
   }
+  ++pubx;
   if constexpr (RW > 0) {
     // this wave's matrix rows, resident for the whole solve (padding: 0, col 0)
     const int c = chunk[0];
@@ -729,37 +739,48 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
   int it = 0, restarts = 0;
   double rt2 = 0.0;
 
-  // (re)start from the current r: z = M r, publish z; PIPE also w = H u
-  // returns false on abort; rz, rr reduced
+  // (re)start from the current r.  Standard: z = M r published, (r.z, r.r)
+  // reduced.  Pipelined: u = M r published (FP64), then w = H u.
   auto restart = [&](double& rz, double& rr) -> bool {
     rz = 0.0;
     rr = 0.0;
-    float* zb = pzbuf(pub);
-#pragma unroll
-    for (int k = 0; k < CPW; ++k) {
-      if (!own[k]) continue;
-      const int row = chunk[k] * 64 + lane;
-      apply_M<CPW>(M, k, r[k], z[k]);
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        pub_store(&zb[3 * row + a], z[k][a]);
-        u[k][a] = (double)z[k][a];
-        rz += r[k][a] * z[k][a];
-        rr += r[k][a] * r[k][a];
-      }
-    }
-    ++pub;
-    double red2[2] = {rz, rr};
-    if (!grid_reduce(gs, red2)) return false;
-    rz = red2[0];
-    rr = red2[1];
     if constexpr (PIPE) {
-      float y[CPW][3];
-      spmv_rows<CPW, RW>(S, chunk, own, lane, rb, zb, y);
+      double* ub = pxbuf(pubx);
 #pragma unroll
-      for (int k = 0; k < CPW; ++k)
+      for (int k = 0; k < CPW; ++k) {
+        if (!own[k]) continue;
+        const int row = chunk[k] * 64 + lane;
+        apply_M<CPW, double>(M, k, r[k], u[k]);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) w[k][a] = own[k] ? (double)y[k][a] : 0.0;
+        for (int a = 0; a < 3; ++a) {
+          pub_store(&ub[3 * row + a], u[k][a]);
+          rr += r[k][a] * r[k][a];
+        }
+      }
+      ++pubx;
+      double red2[1] = {rr};
+      if (!grid_reduce(gs, red2)) return false;
+      rr = red2[0];
+      spmv_rows64<CPW, RW, PIPE>(S, chunk, own, lane, rb, ub, w);
+    } else {
+      float* zb = pzbuf(pub);
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) {
+        if (!own[k]) continue;
+        const int row = chunk[k] * 64 + lane;
+        apply_M<CPW, float>(M, k, r[k], z[k]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          pub_store(&zb[3 * row + a], z[k][a]);
+          rz += r[k][a] * z[k][a];
+          rr += r[k][a] * r[k][a];
+        }
+      }
+      ++pub;
+      double red2[2] = {rz, rr};
+      if (!grid_reduce(gs, red2)) return false;
+      rz = red2[0];
+      rr = red2[1];
     }
     return true;
   };
@@ -768,7 +789,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
     double rz = 0.0, rr = 0.0;
     {
       double y[CPW][3];
-      spmv_rows64<CPW>(S, chunk, own, lane, S.px, y);
+      spmv_rows64<CPW, RW, false>(S, chunk, own, lane, rb, px0, y);
 #pragma unroll
       for (int k = 0; k < CPW; ++k)
 #pragma unroll
@@ -783,13 +804,13 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
       if constexpr (PIPE) {
         // gamma = r.u, delta = w.u, rr = r.r; m = M w published with them
         rr = 0.0;
-        float* mb = pzbuf(pub);
+        double* mb = pxbuf(pubx);
 #pragma unroll
         for (int k = 0; k < CPW; ++k) {
           if (!own[k]) continue;
           const int row = chunk[k] * 64 + lane;
-          float m[3];
-          apply_M<CPW>(M, k, w[k], m);
+          double m[3];
+          apply_M<CPW, double>(M, k, w[k], m);
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
             pub_store(&mb[3 * row + a], m[a]);
@@ -798,7 +819,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
             rr += r[k][a] * r[k][a];
           }
         }
-        ++pub;
+        ++pubx;
         PCG_LAP(2);
         double red[3] = {gamma, delta, rr};
         if (!grid_reduce(gs, red)) return;
@@ -809,20 +830,22 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
       }
       if (rr <= tol2_rec || it >= S.cg_maxiter) {
         // true residual with the FP64 iterate
+        double* xb = pxbuf(pubx);
 #pragma unroll
         for (int k = 0; k < CPW; ++k) {
           if (!own[k]) continue;
           const int row = chunk[k] * 64 + lane;
 #pragma unroll
-          for (int a = 0; a < 3; ++a) pub_store(&S.px[3 * row + a], x[k][a]);
+          for (int a = 0; a < 3; ++a) pub_store(&xb[3 * row + a], x[k][a]);
         }
+        ++pubx;
         double red0[1] = {0.0};
         if (!grid_reduce(gs, red0)) return;
         double acc = 0.0;
         double rt[CPW][3];
         {
           double y[CPW][3];
-          spmv_rows64<CPW>(S, chunk, own, lane, S.px, y);
+          spmv_rows64<CPW, RW, false>(S, chunk, own, lane, rb, xb, y);
 #pragma unroll
           for (int k = 0; k < CPW; ++k)
 #pragma unroll
@@ -841,16 +864,20 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
         for (int k = 0; k < CPW; ++k)
 #pragma unroll
           for (int a = 0; a < 3; ++a) r[k][a] = rt[k][a];
+        const double rz_old = rz;
         if (!restart(rz, rr)) return;
+        // standard variant: residual replacement that keeps the search
+        // direction (beta from the replaced residual); the pipelined variant
+        // rebuilds its auxiliary vectors and starts a new Krylov sequence
         first = true;
-        beta = 0.0;
+        beta = PIPE ? 0.0 : rz / rz_old;
         PCG_LAP(4);
         continue;
       }
       if constexpr (PIPE) {
-        // n = H m; scalars; all vector recurrences local
-        float n[CPW][3];
-        spmv_rows<CPW, RW>(S, chunk, own, lane, rb, pzbuf(pub - 1), n);
+        // n = H m; scalars; all vector recurrences local (FP64)
+        double n[CPW][3];
+        spmv_rows64<CPW, RW, true>(S, chunk, own, lane, rb, pxbuf(pubx - 1), n);
         PCG_LAP(0);
         double alpha;
         if (first) {
@@ -864,22 +891,21 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
           it = S.cg_maxiter;
           continue;
         }
-        const float fb = (float)beta;
 #pragma unroll
         for (int k = 0; k < CPW; ++k) {
           if (!own[k]) continue;
-          float m[3];
-          apply_M<CPW>(M, k, w[k], m);
+          double m[3];
+          apply_M<CPW, double>(M, k, w[k], m);
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
-            z[k][a] = n[k][a] + fb * z[k][a];
-            q[k][a] = m[a] + fb * q[k][a];
-            sv[k][a] = (float)w[k][a] + fb * sv[k][a];
-            p[k][a] = (float)u[k][a] + fb * p[k][a];
-            x[k][a] += alpha * (double)p[k][a];
-            r[k][a] -= alpha * (double)sv[k][a];
-            u[k][a] -= alpha * (double)q[k][a];
-            w[k][a] -= alpha * (double)z[k][a];
+            zd[k][a] = n[k][a] + beta * zd[k][a];
+            qd[k][a] = m[a] + beta * qd[k][a];
+            sd[k][a] = w[k][a] + beta * sd[k][a];
+            pd[k][a] = u[k][a] + beta * pd[k][a];
+            x[k][a] += alpha * pd[k][a];
+            r[k][a] -= alpha * sd[k][a];
+            u[k][a] -= alpha * qd[k][a];
+            w[k][a] -= alpha * zd[k][a];
           }
         }
         PCG_LAP(2);
@@ -928,7 +954,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
             x[k][a] += alpha * (double)p[k][a];
             r[k][a] -= alpha * (double)q[k][a];
           }
-          apply_M<CPW>(M, k, r[k], z[k]);
+          apply_M<CPW, float>(M, k, r[k], z[k]);
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
             pub_store(&zb[3 * row + a], z[k][a]);
@@ -1225,7 +1251,8 @@ struct PcgVariant {
 // streaming ones with more chunks per wave for larger systems
 const PcgVariant kPcgVariants[] = {
     PCG_V(1, 16, false), PCG_V(1, 0, false), PCG_V(2, 0, false), PCG_V(4, 0, false), PCG_V(8, 0, false),
-    PCG_V(1, 16, true),  PCG_V(1, 0, true),  PCG_V(2, 0, true),  PCG_V(4, 0, true),  PCG_V(8, 0, true),
+    // the pipelined variant's FP64 recurrences leave no room for resident rows
+    PCG_V(1, 0, true), PCG_V(2, 0, true), PCG_V(4, 0, true), PCG_V(8, 0, true),
 };
 #undef PCG_V
 constexpr int kNumPcgVariants = sizeof(kPcgVariants) / sizeof(kPcgVariants[0]);

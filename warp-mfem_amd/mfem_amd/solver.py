@@ -173,7 +173,7 @@ class RefinementSolver:
         b["dx"] = torch.zeros(nv, 3, dtype=torch.float64, device=dev)
         b["ds"] = zf(nt, 6)
         b["pz"] = zf(2, pat.n_chunks * 64, 3)
-        b["px"] = torch.zeros(pat.n_chunks * 64, 3, dtype=torch.float64, device=dev)
+        b["px"] = torch.zeros(2, pat.n_chunks * 64, 3, dtype=torch.float64, device=dev)
         b["red"] = torch.zeros(2 * _ffi.MFEM_RED_MAX_BLOCKS * 4, dtype=torch.float64, device=dev)
         b["flags"] = torch.zeros(_ffi.MFEM_RED_MAX_BLOCKS, dtype=torch.int32, device=dev)
         b["status"] = torch.zeros(8, dtype=torch.int32, device=dev)
@@ -185,7 +185,7 @@ class RefinementSolver:
         S.n_chunks, S.n_slots = pat.n_chunks, pat.n_slots
         S.max_width = int(np.diff(pat.chunk_ptr).max())
         # solver-specific knobs (not reference kwargs): pipelined PCG, resident rows
-        S.pcg_flags = (_ffi.PCG_PIPELINED if kwargs.get("pcg_pipelined", True) else 0) | \
+        S.pcg_flags = (_ffi.PCG_PIPELINED if kwargs.get("pcg_pipelined", False) else 0) | \
                       (0 if kwargs.get("pcg_resident_rows", True) else _ffi.PCG_STREAM_ROWS)
         S.energy = _ENERGIES[energy]
         S.iterations = self.iterations
