@@ -540,8 +540,7 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
 // m = M w for the single SpMV n = H m.
 // Both: when the recurrence residual passes (at tol/2), the true residual
 // g - H x is recomputed with the FP64 iterate and FP64 accumulation; if it
-// fails tol, the recurrence residual is replaced by it (at most 16 times):
-// the standard variant keeps its search direction, the pipelined one restarts.
+// fails tol, the iteration restarts from it (residual replacement, <= 16).
 // ---------------------------------------------------------------------------
 template <typename T, typename A, int GRP = 8>
 __device__ __forceinline__ void spmv_row_stream(const mfem_solver& S, int c, int lane, const T* vec, A* out,
@@ -864,13 +863,11 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
         for (int k = 0; k < CPW; ++k)
 #pragma unroll
           for (int a = 0; a < 3; ++a) r[k][a] = rt[k][a];
-        const double rz_old = rz;
         if (!restart(rz, rr)) return;
-        // standard variant: residual replacement that keeps the search
-        // direction (beta from the replaced residual); the pipelined variant
-        // rebuilds its auxiliary vectors and starts a new Krylov sequence
+        // restart the Krylov sequence from the replaced residual (keeping the
+        // old direction diverges when the residual gap is large)
         first = true;
-        beta = PIPE ? 0.0 : rz / rz_old;
+        beta = 0.0;
         PCG_LAP(4);
         continue;
       }
