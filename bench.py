@@ -129,7 +129,7 @@ def main():
             "newton_iterations": solver.iterations,
             "dt": dt,
             "preconditioner": "block_jacobi",
-            "pcg": "pipelined" if solver._S.pcg_flags & 1 else "standard",
+            "pcg": args.pcg,
             "learned_preconditioner": False,
             "reductions": "f64",
             "cg_iters_mean": float(cg_iters.mean()),

@@ -42,14 +42,17 @@ extern "C" {
 /* stats[iter][k]: 0 cg_iters, 1 cg_relres (FP64, true residual), 2 ls_trials,
    3 ls_accepted, 4 alpha, 5 merit f0, 6 merit f1, 7 cg_restarts,
    8..13 PCG phase device-clock ticks (SpMV, reduce 1, update, reduce 2,
-   residual checks, prologue) */
+   residual checks, prologue), 14 learned guess used (0/1),
+   15 |g - H x0_learned| / |g - H x0_plain| (learned warm start only) */
 #define MFEM_STATS_PER_ITER 16
 #define MFEM_MAX_STATS_ITERS 64
 #define MFEM_RED_MAX_BLOCKS 2048
+#define MFEM_MLP_NPARAMS 6288 /* 16x64 + 64 + 64x64 + 64 + 64x16 + 16 */
 
 /* pcg_flags bits */
 #define MFEM_PCG_PIPELINED 1   /* one grid reduction per CG iteration (pipelined PCG) */
 #define MFEM_PCG_STREAM_ROWS 2 /* stream matrix rows from HBM instead of keeping them in VGPRs */
+#define MFEM_PCG_LEARNED_GUESS 4 /* start from the learned warm start when it has the smaller residual */
 
 /* status[] words */
 #define MFEM_STATUS_ERRBITS 0 /* bit 0: grid barrier timed out */
@@ -151,6 +154,13 @@ typedef struct mfem_solver {
   uint32_t* flags; /* [MFEM_RED_MAX_BLOCKS] grid barrier arrival epochs */
   int32_t* status; /* [8] see MFEM_STATUS_* */
   double* stats;   /* [MFEM_MAX_STATS_ITERS][MFEM_STATS_PER_ITER] per Newton iteration */
+  /* learned warm start (used when pcg_flags has MFEM_PCG_LEARNED_GUESS) */
+  const float* mlp_w;      /* [MFEM_MLP_NPARAMS] FP32 master weights (biases are read from here) */
+  const uint16_t* mlp_wbf; /* [MFEM_MLP_NPARAMS] BF16 copy (matrix operands) */
+  uint16_t* ws_x;          /* [n_verts][16] BF16 features */
+  float* ws_aux;           /* [n_verts][10] normalised candidates c1, c2, c3 and their scale */
+  float* ws_y;             /* [n_verts][3] network outputs (blend weights) */
+  float* guess;            /* [n_verts][3] learned initial guess */
 } mfem_solver;
 
 /* ABI self-description (checked against the Python mirror by the tests) */
@@ -198,6 +208,28 @@ int mfem_end_step(const mfem_solver* S, float dt, void* stream);
 /* RefinementSolver.step without refinement (solver.py:429-579): all of the
    above in order, no host synchronisation. */
 int mfem_step(const mfem_solver* S, float dt, void* stream);
+
+/* Learned warm start, no counterpart in the reference (its guess is x~ - x,
+   solver.py:530-531).  Per vertex: candidates c1 = x~ - x, c2 = D^-1 (g - H c1)
+   (one block-Jacobi step), c3 = the previous Newton step, normalised by
+   s = |c1| + |c2| + |c3|, and 16 rotation-invariant features (BF16) into
+   ws_x / ws_aux. */
+int mfem_ws_features(const mfem_solver* S, float dt, int newton_iter, void* stream);
+/* guess = s (y0 c1 + y1 c2 + y2 c3) from the network outputs ws_y. */
+int mfem_ws_apply(const mfem_solver* S, void* stream);
+
+/* Per-vertex MLP (16 -> 64 -> 64 -> 3, GELU, BF16 MFMA with FP32 accumulate):
+   y[n][0..2] for the BF16 inputs x[n][16]. */
+int mfem_mlp_forward(const uint16_t* x, int n, const uint16_t* w_bf16, const float* w_f32, float* y, void* stream);
+/* One Adam step on the loss (1/n) sum_n |y0 c1 + y1 c2 + y2 c3 - t|^2 with
+   aux[n] = (c1, c2, c3, t): per-workgroup gradients into part (at most
+   max_parts workgroups), fixed-order sum, Adam (beta 0.9/0.999, eps 1e-8,
+   step >= 1) on w_f32, BF16 refresh; lr = 0 only computes the gradient into
+   grad_out.  loss_out / grad_out may be null. */
+int mfem_mlp_train_step(const uint16_t* x, const float* aux, int n, float* w_f32, uint16_t* w_bf16, float* adam_state,
+                        float* part, int max_parts, float* part_loss, float* loss_out, float* grad_out, float lr,
+                        int step, void* stream);
+const char* mfem_mlp_last_error(void);
 
 /* y = H x with the assembled SELL matrix (warp.sparse.bsr_mv). */
 int mfem_spmv(const mfem_solver* S, const float* x, float* y, void* stream);

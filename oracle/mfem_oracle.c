@@ -647,11 +647,11 @@ typedef struct {
 enum {
   P_ENERGY = 0, P_GX, P_GY, P_GZ, P_ATTACH_K, P_D0, P_D1, P_KAPPA, P_EPSV,
   P_ITERS, P_CG_MAXIT, P_CG_TOL, P_PRECOND, P_LS, P_LS_MAX, P_LS_A0, P_LS_TAU,
-  P_LS_C, P_LS_THR, P_SING, P_TRI_CONTACT, P_QB0, P_QB1, P_QB2, P_NPARAMS
+  P_LS_C, P_LS_THR, P_SING, P_TRI_CONTACT, P_GUESS, P_QB0, P_QB1, P_QB2, P_NPARAMS
 };
 static const char* PNAMES[] = {"energy", "gx", "gy", "gz", "attach_k", "d0", "d1", "kappa", "eps_v",
                                "iterations", "cg_maxiter", "cg_tol", "precond", "line_search", "ls_max", "ls_alpha0",
-                               "ls_tau", "ls_c", "ls_thr", "sing_thr", "tri_contact", NULL};
+                               "ls_tau", "ls_c", "ls_thr", "sing_thr", "tri_contact", "guess", NULL};
 
 #define ALLOC(ptr, n) ptr = calloc((size_t)((n) > 0 ? (n) : 1), sizeof(*(ptr)))
 
@@ -1084,7 +1084,10 @@ EXPORT int orc_global_solve(orc_ctx* C, double* relres) {
       for (int q = 0; q < 9; ++q) M[q] = (q % 4 == 0) ? 1 : 0;
     } else inv3(D, M);
   }
-  for (int k = 0; k < n; ++k) C->dx[k] = C->xt[k] - C->x[k];
+  /* initial guess: x~ - x (reference, solver.py:530-531); experiments: 1 = zero,
+     2 = previous Newton step (kept in dx) */
+  int gm = (int)C->p[P_GUESS];
+  for (int k = 0; k < n; ++k) C->dx[k] = gm == 1 ? 0 : (gm == 2 ? C->dx[k] : C->xt[k] - C->x[k]);
   orc_spmv(C, C->dx, C->q);
   for (int k = 0; k < n; ++k) C->r[k] = C->g[k] - C->q[k];
   double bn = sqrt(vdot(C->g, C->g, n));

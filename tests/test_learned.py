@@ -1,0 +1,201 @@
+"""Learned CG warm start on the GPU against the NumPy oracle (oracle/mlp.py).
+
+Tolerances:
+  * MLP forward: BF16 operands and FP32 accumulation on both sides; only the
+    summation order differs, which can flip a BF16 rounding of a hidden
+    activation now and then: max |y_gpu - y_orc| <= 4e-3 * max|y_orc| and
+    mean <= 1e-4 * max|y_orc|.
+  * MLP gradient: |g_gpu - g_orc| <= 1e-2 |g_orc| (same BF16 rounding
+    points), loss within 1e-4 relative.
+  * Adam: the FP32 master weights within 1e-7 absolute + 3e-7 relative
+    (their FP32 rounding).
+  * features: BF16 values within one BF16 step (2^-7 relative) + 1e-3;
+    candidates within 1e-4 of their scale.
+  * steps with the warm start: positions within 1e-4 relative of the FP64
+    oracle (the same bar as plain PCG), relative residual <= 1e-6.
+"""
+import numpy as np
+import pytest
+import torch
+
+from mfem_amd import scene
+from oracle import Oracle, mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _make(model, device, **extra):
+    from mfem_amd.solver import RefinementSolver, model_state
+
+    kw = {k: v for k, v in model.params.items() if k != "iterations"}
+    solver = RefinementSolver(model, model.params["iterations"], max_tets=model.tet_count, device=device,
+                              enable_refinement=False, **kw, **extra)
+    return solver, model_state(model, device)
+
+
+def _net(device, seed=0, jitter=0.3):
+    from mfem_amd.learned import WarmStartNet, init_params
+
+    w = init_params(seed)
+    rng = np.random.default_rng(seed + 100)
+    w += (jitter * rng.normal(0, 1, w.shape) * (np.abs(w) + 0.05)).astype(np.float32)
+    return WarmStartNet(device, weights=w)
+
+
+def _bf16_tensor(a, device):
+    return torch.from_numpy(mlp.bf16_bits(a).view(np.int16).copy()).to(device)
+
+
+def test_mlp_forward_matches_oracle(gpu):
+    net = _net(gpu)
+    rng = np.random.default_rng(7)
+    n = 1000  # not a multiple of the 64-sample tile
+    X = rng.normal(0, 1, (n, 16)).astype(np.float32)
+    y = net.forward(_bf16_tensor(X, gpu)).cpu().numpy()
+    y_orc, _ = mlp.forward(net.weights(), mlp.bf16(X), quantize=True)
+    err = np.abs(y - y_orc)
+    scale = np.abs(y_orc).max()
+    print("forward max err", err.max(), "mean", err.mean(), "scale", scale)
+    assert err.max() <= 4e-3 * scale
+    assert err.mean() <= 1e-4 * scale
+    # a layout mistake in the MFMA fragments would show at O(1): also check
+    # the unrounded FP64 network agrees to BF16 accuracy
+    y64, _ = mlp.forward(net.weights(), mlp.bf16(X), quantize=False)
+    assert np.abs(y - y64).max() <= 3e-2 * scale
+
+
+def test_mlp_gradient_and_loss_match_oracle(gpu):
+    net = _net(gpu, seed=1)
+    rng = np.random.default_rng(8)
+    n = 3000
+    X = rng.normal(0, 1, (n, 16)).astype(np.float32)
+    aux = rng.normal(0, 0.5, (n, 12)).astype(np.float32)
+    grad = torch.zeros(mlp.N_PARAMS, dtype=torch.float32, device=gpu)
+    loss = torch.zeros(1, dtype=torch.float32, device=gpu)
+    w0 = net.weights()
+    net.train_step(_bf16_tensor(X, gpu), torch.from_numpy(aux).to(gpu), 0.0, loss_out=loss, grad_out=grad)
+    assert np.array_equal(net.weights(), w0)  # lr = 0 leaves the weights alone
+    l_orc, g_orc = mlp.loss_and_grad(w0, mlp.bf16(X), aux, quantize=True)
+    g = grad.cpu().numpy()
+    rel = np.linalg.norm(g - g_orc) / np.linalg.norm(g_orc)
+    print("loss", float(loss.item()), l_orc, "grad rel err", rel)
+    assert abs(float(loss.item()) - l_orc) <= 1e-4 * l_orc
+    assert rel <= 1e-2
+    for name, a, b in (("W1", mlp.O_W1, mlp.O_B1), ("W2", mlp.O_W2, mlp.O_B2), ("W3", mlp.O_W3, mlp.O_B3),
+                       ("b1", mlp.O_B1, mlp.O_W2), ("b2", mlp.O_B2, mlp.O_W3), ("b3", mlp.O_B3, mlp.N_PARAMS)):
+        r = np.linalg.norm(g[a:b] - g_orc[a:b]) / max(np.linalg.norm(g_orc[a:b]), 1e-30)
+        print(name, "rel err", r)
+        assert r <= 2e-2, name
+
+
+def test_adam_step_matches_oracle(gpu):
+    net = _net(gpu, seed=2)
+    rng = np.random.default_rng(9)
+    n = 640
+    Xt = _bf16_tensor(rng.normal(0, 1, (n, 16)).astype(np.float32), gpu)
+    auxt = torch.from_numpy(rng.normal(0, 0.5, (n, 12)).astype(np.float32)).to(gpu)
+    w0 = net.weights()
+    grad = torch.zeros(mlp.N_PARAMS, dtype=torch.float32, device=gpu)
+    m = np.zeros(mlp.N_PARAMS)
+    v = np.zeros(mlp.N_PARAMS)
+    w_ref = w0.astype(np.float64)
+    for t in range(1, 4):
+        net.train_step(Xt, auxt, 1e-3, grad_out=grad)
+        w_ref, m, v = mlp.adam(w_ref, m, v, grad.cpu().numpy().astype(np.float64), 1e-3, t)
+        np.testing.assert_allclose(net.weights(), w_ref, rtol=3e-7, atol=1e-7)  # FP32 master weights
+    # the BF16 copy follows the master weights
+    np.testing.assert_array_equal(net.wbf.cpu().numpy().view(np.uint16), mlp.bf16_bits(net.weights()))
+
+
+def test_training_reduces_loss_on_solver_data(gpu):
+    from mfem_amd.learned import WarmStartNet, collect
+
+    m = scene.drop_box_scene()
+    solver, state = _make(m, gpu)
+    X, aux = collect(solver, state, 0.01, steps=6)
+    print("samples", X.shape[0])
+    assert X.shape[0] > 1000
+    net = WarmStartNet(gpu)
+    l0 = net.loss(X, aux)
+    hist = net.fit(X, aux, epochs=30, batch=4096, lr=3e-3)
+    print("loss", l0, "->", hist["train"][-1], "val", hist["val"][:1], hist["val"][-1:])
+    assert hist["val"][-1] < 0.9 * hist["val"][0]
+    assert hist["train"][-1] < l0
+
+
+def test_features_match_oracle(gpu):
+    m = scene.beam_scene()
+    solver, state = _make(m, gpu)
+    dt = 1.0 / 60.0
+    out = state.clone()
+    solver.step(state, out, None, None, dt)  # a previous step, so c3 != 0
+    solver.bind(out)
+    solver.begin_step(dt)
+    solver.update_system(dt)
+    solver.assemble(dt)
+    solver.global_solve(0)
+    solver.local_solve()
+    solver.line_search(dt, 0)
+    solver.update_system(dt)
+    solver.assemble(dt)
+    solver.ws_features(dt, 1)
+    b = solver.buffers
+    torch.cuda.synchronize()
+    F, aux = mlp.ws_features(solver.system_csr(), b["g"].double().cpu().numpy(), b["x_tilde"].double().cpu().numpy(),
+                             out.particle_q.double().cpu().numpy(), b["dx"].cpu().numpy(),
+                             b["Hv"].double().cpu().numpy(), m.particle_inv_mass, precond=True,
+                             attach_k=m.params.get("attachment_stiffness", 10.0), dt=dt, newton_iter=1,
+                             iterations=solver.iterations)
+    Fg = mlp.bf16_value(b["ws_x"].cpu().numpy().view(np.uint16))
+    ag = b["ws_aux"].double().cpu().numpy()
+    err = np.abs(Fg - F)
+    tol = 2.0 ** -7 * np.abs(F) + 1e-3
+    bad = np.argwhere(err > tol)
+    print("feature max err", err.max(axis=0))
+    assert len(bad) == 0, bad[:10]
+    np.testing.assert_allclose(ag[:, 9], aux[:, 9], rtol=1e-4)
+    assert np.abs(ag[:, :9] - aux[:, :9]).max() <= 1e-4
+
+
+def _run(m, gpu, steps, dt, net=None):
+    solver, state = _make(m, gpu, warm_start=net)
+    out = state.clone()
+    stats = []
+    for k in range(steps):
+        solver.step(state, out, None, None, dt)
+        state, out = out, state
+        stats.append(solver.last_stats())
+        assert solver.status() == 0
+    return state.particle_q.double().cpu().numpy(), np.stack(stats)
+
+
+def test_warm_start_steps_match_fp64_oracle(gpu):
+    m = scene.drop_box_scene()
+    net = _net(gpu, seed=3, jitter=0.05)
+    x, st = _run(m, gpu, 6, 0.01, net)
+    o = Oracle(m, 64)
+    for k in range(6):
+        o.step(0.01)
+    assert np.all(st[:, :, 1] <= 1e-6)
+    scale = np.abs(o.x).max()
+    err = np.abs(x - o.x).max()
+    print("learned guess used", st[:, :, 14].astype(int).tolist(), "ratio", np.round(st[:, :, 15], 3).tolist())
+    print("positions rel err", err / scale)
+    assert err <= 1e-4 * scale
+
+
+def test_poisoned_network_falls_back_to_plain_pcg(gpu):
+    """Weights whose guess is far off: the PCG must reject it every time, and
+    the result must be bit-identical to plain PCG."""
+    from mfem_amd.learned import WarmStartNet
+
+    m = scene.drop_box_scene()
+    w = np.zeros(mlp.N_PARAMS, np.float32)
+    w[mlp.O_B3:mlp.O_B3 + 3] = 1e4
+    net = WarmStartNet(gpu, weights=w)
+    x_l, st_l = _run(m, gpu, 4, 0.01, net)
+    x_p, st_p = _run(m, gpu, 4, 0.01, None)
+    assert np.all(st_l[:, :, 14] == 0)
+    assert np.all(st_l[:, :, 15] > 1.0)
+    np.testing.assert_array_equal(st_l[:, :, 0], st_p[:, :, 0])
+    np.testing.assert_array_equal(x_l, x_p)

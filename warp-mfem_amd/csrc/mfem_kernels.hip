@@ -646,8 +646,10 @@ __device__ __forceinline__ void apply_M(const float (&M)[CPW][9], int k, const d
   for (int a = 0; a < 3; ++a) z[a] = (T)(M[k][3 * a] * r[0] + M[k][3 * a + 1] * r[1] + M[k][3 * a + 2] * r[2]);
 }
 
-template <int CPW, int RW, bool PIPE>
+template <int CPW, int RW, int MODE>
 __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) {
+  constexpr bool PIPE = MODE != 0;
+  constexpr bool PUB32 = MODE == 2;  // pipelined: publish the SpMV input in FP32
   GridSync gs{S.red, S.flags, S.status, (uint32_t)S.status[MFEM_STATUS_EPOCH], false};
   const int lane = threadIdx.x & 63;
   const int gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
@@ -743,7 +745,30 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
   auto restart = [&](double& rz, double& rr) -> bool {
     rz = 0.0;
     rr = 0.0;
-    if constexpr (PIPE) {
+    if constexpr (PUB32) {
+      float* ub = pzbuf(pub);
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) {
+        if (!own[k]) continue;
+        const int row = chunk[k] * 64 + lane;
+        apply_M<CPW, double>(M, k, r[k], u[k]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          pub_store(&ub[3 * row + a], (float)u[k][a]);
+          rr += r[k][a] * r[k][a];
+        }
+      }
+      ++pub;
+      double red2[1] = {rr};
+      if (!grid_reduce(gs, red2)) return false;
+      rr = red2[0];
+      float y[CPW][3];
+      spmv_rows<CPW, RW>(S, chunk, own, lane, rb, ub, y);
+#pragma unroll
+      for (int k = 0; k < CPW; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) w[k][a] = own[k] ? (double)y[k][a] : 0.0;
+    } else if constexpr (PIPE) {
       double* ub = pxbuf(pubx);
 #pragma unroll
       for (int k = 0; k < CPW; ++k) {
@@ -760,7 +785,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
       double red2[1] = {rr};
       if (!grid_reduce(gs, red2)) return false;
       rr = red2[0];
-      spmv_rows64<CPW, RW, PIPE>(S, chunk, own, lane, rb, ub, w);
+      spmv_rows64<CPW, RW, true>(S, chunk, own, lane, rb, ub, w);
     } else {
       float* zb = pzbuf(pub);
 #pragma unroll
@@ -784,6 +809,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
     return true;
   };
 
+  bool used_learned = false;
+  double guess_ratio = 0.0;
   if (bnorm2 > 0.0) {
     double rz = 0.0, rr = 0.0;
     {
@@ -793,6 +820,53 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
       for (int k = 0; k < CPW; ++k)
 #pragma unroll
         for (int a = 0; a < 3; ++a) r[k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
+    }
+    if ((S.pcg_flags & MFEM_PCG_LEARNED_GUESS) && S.guess) {
+      // learned warm start: keep it only if its residual is smaller than the
+      // plain guess's (block-Jacobi CG from x~ - x is the fallback)
+      double* gb = pxbuf(pubx);
+      double r0 = 0.0;
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) {
+        if (!own[k]) continue;
+        const int row = chunk[k] * 64 + lane;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          pub_store(&gb[3 * row + a], live[k] ? (double)S.guess[3 * row + a] : 0.0);
+          r0 += r[k][a] * r[k][a];
+        }
+      }
+      ++pubx;
+      double redg[1] = {r0};
+      if (!grid_reduce(gs, redg)) return;
+      r0 = redg[0];
+      double y[CPW][3];
+      spmv_rows64<CPW, RW, false>(S, chunk, own, lane, rb, gb, y);
+      double r1 = 0.0;
+      double rl[CPW][3];
+#pragma unroll
+      for (int k = 0; k < CPW; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          rl[k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
+          r1 += rl[k][a] * rl[k][a];
+        }
+      double redl[1] = {r1};
+      if (!grid_reduce(gs, redl)) return;
+      r1 = redl[0];
+      guess_ratio = r0 > 0.0 ? sqrt(r1 / r0) : 0.0;
+      if (r1 < r0) {
+        used_learned = true;
+#pragma unroll
+        for (int k = 0; k < CPW; ++k) {
+          const int row = chunk[k] * 64 + lane;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            r[k][a] = rl[k][a];
+            x[k][a] = live[k] ? (double)S.guess[3 * row + a] : 0.0;
+          }
+        }
+      }
     }
     if (!restart(rz, rr)) return;
     bool first = true;
@@ -804,6 +878,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
         // gamma = r.u, delta = w.u, rr = r.r; m = M w published with them
         rr = 0.0;
         double* mb = pxbuf(pubx);
+        float* mb32 = pzbuf(pub);
 #pragma unroll
         for (int k = 0; k < CPW; ++k) {
           if (!own[k]) continue;
@@ -812,13 +887,15 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
           apply_M<CPW, double>(M, k, w[k], m);
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
-            pub_store(&mb[3 * row + a], m[a]);
+            if constexpr (PUB32) pub_store(&mb32[3 * row + a], (float)m[a]);
+            else pub_store(&mb[3 * row + a], m[a]);
             gamma += r[k][a] * u[k][a];
             delta += w[k][a] * u[k][a];
             rr += r[k][a] * r[k][a];
           }
         }
-        ++pubx;
+        if constexpr (PUB32) ++pub;
+        else ++pubx;
         PCG_LAP(2);
         double red[3] = {gamma, delta, rr};
         if (!grid_reduce(gs, red)) return;
@@ -874,7 +951,16 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
       if constexpr (PIPE) {
         // n = H m; scalars; all vector recurrences local (FP64)
         double n[CPW][3];
-        spmv_rows64<CPW, RW, true>(S, chunk, own, lane, rb, pxbuf(pubx - 1), n);
+        if constexpr (PUB32) {
+          float y[CPW][3];
+          spmv_rows<CPW, RW>(S, chunk, own, lane, rb, pzbuf(pub - 1), y);
+#pragma unroll
+          for (int k = 0; k < CPW; ++k)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) n[k][a] = (double)y[k][a];
+        } else {
+          spmv_rows64<CPW, RW, true>(S, chunk, own, lane, rb, pxbuf(pubx - 1), n);
+        }
         PCG_LAP(0);
         double alpha;
         if (first) {
@@ -988,10 +1074,120 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
       st[7] = (double)restarts;
 #pragma unroll
       for (int k = 0; k < 6; ++k) st[8 + k] = (double)tk[k];
+      st[14] = used_learned ? 1.0 : 0.0;
+      st[15] = guess_ratio;
     }
     S.status[MFEM_STATUS_EPOCH] = (int32_t)gs.epoch;
   }
 #undef PCG_LAP
+}
+
+// ---------------------------------------------------------------------------
+// learned warm start: candidate guesses and per-vertex features
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f2bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__global__ void __launch_bounds__(kBlock) k_ws_features(mfem_solver S, float dt, int newton_iter) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.n_verts) return;
+  const int c = i >> 6, lane = i & 63;
+  float c1[3], r1[3], ring = 0.f, D[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    c1[a] = S.x_tilde[3 * i + a] - S.x[3 * i + a];
+    r1[a] = S.g[3 * i + a];
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) D[q] = S.sell_val[val_index(S.diag_lane[i], q)];
+  int valence = 0;
+  for (int sl = S.chunk_ptr[c]; sl < S.chunk_ptr[c + 1]; ++sl) {
+    const int j = S.sell_col[sl * 64 + lane];
+    const float* vb = S.sell_val + (size_t)sl * 576 + lane;
+    float bv[9], fro = 0.f;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      bv[q] = vb[64 * q];
+      fro += bv[q] * bv[q];
+    }
+    float cj[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) cj[a] = S.x_tilde[3 * j + a] - S.x[3 * j + a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) r1[a] -= bv[3 * a] * cj[0] + bv[3 * a + 1] * cj[1] + bv[3 * a + 2] * cj[2];
+    if (fro > 0.f && j != i) {
+      ring += sqrtf(fro);
+      ++valence;
+    }
+  }
+  float Mi[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) Mi[q] = (q % 4 == 0) ? 1.f : 0.f;
+  if (S.precond && fabsf(det3(D)) >= S.sing_thr) inv3(D, Mi);
+  float c2[3], c3[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    c2[a] = Mi[3 * a] * r1[0] + Mi[3 * a + 1] * r1[1] + Mi[3 * a + 2] * r1[2];
+    c3[a] = (float)S.dx[3 * i + a];
+  }
+  auto nrm = [](const float* v) { return sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+  auto cosv = [&](const float* u, const float* v) {
+    const float d = nrm(u) * nrm(v);
+    return d > 0.f ? (u[0] * v[0] + u[1] * v[1] + u[2] * v[2]) / d : 0.f;
+  };
+  const float n1 = nrm(c1), n2 = nrm(c2), n3 = nrm(c3);
+  const float sc = n1 + n2 + n3 + 1e-30f;
+  float dfro = 0.f, hfro = 0.f;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    dfro += D[q] * D[q];
+    hfro += S.Hv[9 * i + q] * S.Hv[9 * i + q];
+  }
+  dfro = sqrtf(dfro) + 1e-30f;
+  hfro = sqrtf(hfro);
+  const float im = S.inv_mass[i];
+  const float mh = (im == 0.f ? 1.f + S.attach_k : 1.f / im) / (dt * dt);
+  float g[3] = {S.g[3 * i], S.g[3 * i + 1], S.g[3 * i + 2]};
+  const float ng = nrm(g);
+  float f[16];
+  f[0] = n1 / sc;
+  f[1] = n2 / sc;
+  f[2] = n3 / sc;
+  f[3] = cosv(c1, c2);
+  f[4] = cosv(c1, c3);
+  f[5] = cosv(c2, c3);
+  f[6] = hfro / dfro;
+  f[7] = fminf(mh / (dfro / 1.7320508f), 10.f);
+  f[8] = im == 0.f ? 1.f : 0.f;
+  f[9] = (float)newton_iter / (float)(S.iterations > 0 ? S.iterations : 1);
+  f[10] = ring / dfro;
+  f[11] = fminf(ng / (dfro * sc), 10.f);
+  f[12] = fminf(nrm(r1) / (ng + 1e-30f), 10.f);
+  f[13] = (float)valence / 16.f;
+  f[14] = 1.f;
+  f[15] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) S.ws_x[16 * i + k] = f2bf16(f[k]);
+  const float is = 1.f / sc;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    S.ws_aux[10 * i + a] = c1[a] * is;
+    S.ws_aux[10 * i + 3 + a] = c2[a] * is;
+    S.ws_aux[10 * i + 6 + a] = c3[a] * is;
+  }
+  S.ws_aux[10 * i + 9] = sc;
+}
+
+__global__ void __launch_bounds__(kBlock) k_ws_apply(mfem_solver S) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.n_verts) return;
+  const float* a = S.ws_aux + 10 * i;
+  const float* y = S.ws_y + 3 * i;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) S.guess[3 * i + d] = a[9] * (y[0] * a[d] + y[1] * a[3 + d] + y[2] * a[6 + d]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1239,17 +1435,18 @@ __global__ void __launch_bounds__(kBlock) k_merit(mfem_solver S, float dt, float
 namespace {
 
 struct PcgVariant {
-  int cpw, rw;
-  bool pipe;
+  int cpw, rw, mode;
   const void* fn;
 };
 #define PCG_V(C, R, P) {C, R, P, reinterpret_cast<const void*>(&k_pcg<C, R, P>)}
 // resident-row variants first (one chunk per wave, rows in VGPRs), then the
-// streaming ones with more chunks per wave for larger systems
+// streaming ones with more chunks per wave for larger systems.  mode 0 =
+// standard, 1 = pipelined with an FP64 SpMV input (its FP64 recurrences leave
+// no room for resident rows).  Pipelining with an FP32 SpMV input (mode 2)
+// stagnates near rel-res 1e-3 on the contact scenes and is not instantiated.
 const PcgVariant kPcgVariants[] = {
-    PCG_V(1, 16, false), PCG_V(1, 0, false), PCG_V(2, 0, false), PCG_V(4, 0, false), PCG_V(8, 0, false),
-    // the pipelined variant's FP64 recurrences leave no room for resident rows
-    PCG_V(1, 0, true), PCG_V(2, 0, true), PCG_V(4, 0, true), PCG_V(8, 0, true),
+    PCG_V(1, 16, 0), PCG_V(1, 0, 0), PCG_V(2, 0, 0), PCG_V(4, 0, 0), PCG_V(8, 0, 0),
+    PCG_V(1, 0, 1),  PCG_V(2, 0, 1), PCG_V(4, 0, 1), PCG_V(8, 0, 1),
 };
 #undef PCG_V
 constexpr int kNumPcgVariants = sizeof(kPcgVariants) / sizeof(kPcgVariants[0]);
@@ -1315,7 +1512,7 @@ int launched() {
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
   X(lag_sid) X(G) X(Hl) X(gl) X(Hsinv) X(gs) X(psi) X(Kb) X(fe) X(gx) X(gB) X(gD) X(Hv) X(dist) X(eB)      \
-  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats)
+  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess)
 
 extern "C" {
 
@@ -1380,7 +1577,8 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
   unsigned nblocks = 0;
   for (int k = 0; k < kNumPcgVariants; ++k) {
     const PcgVariant& v = kPcgVariants[k];
-    if (v.pipe != ((S->pcg_flags & MFEM_PCG_PIPELINED) != 0)) continue;
+    const int want = (S->pcg_flags & MFEM_PCG_PIPELINED) ? 1 : 0;
+    if (v.mode != want) continue;
     if (v.rw > 0 && ((S->pcg_flags & MFEM_PCG_STREAM_ROWS) || S->max_width > v.rw)) continue;
     long waves = (S->n_chunks + v.cpw - 1) / v.cpw;
     long nb = (waves + kWaves - 1) / kWaves;
@@ -1436,11 +1634,32 @@ int mfem_step(const mfem_solver* S, float dt, void* stream) {
   for (int it = 0; it < S->iterations; ++it) {
     if (int rc = mfem_update_system(S, dt, stream)) return rc;
     if (int rc = mfem_assemble(S, dt, stream)) return rc;
+    if ((S->pcg_flags & MFEM_PCG_LEARNED_GUESS) && S->guess) {
+      if (int rc = mfem_ws_features(S, dt, it, stream)) return rc;
+      if (int rc = mfem_mlp_forward(S->ws_x, S->n_verts, S->mlp_wbf, S->mlp_w, S->ws_y, stream))
+        return fail(rc, std::string("mfem_mlp_forward: ") + mfem_mlp_last_error());
+      if (int rc = mfem_ws_apply(S, stream)) return rc;
+    }
     if (int rc = mfem_global_solve(S, it, stream)) return rc;
     if (int rc = mfem_local_solve(S, stream)) return rc;
     if (int rc = mfem_line_search(S, dt, it, stream)) return rc;
   }
   return mfem_end_step(S, dt, stream);
+}
+
+int mfem_ws_features(const mfem_solver* S, float dt, int newton_iter, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  if (!S->ws_x || !S->ws_aux) return fail(MFEM_EINVAL, "warm-start buffers not set");
+  hipLaunchKernelGGL(k_ws_features, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, (hipStream_t)stream, *S, dt,
+                     newton_iter);
+  return launched();
+}
+
+int mfem_ws_apply(const mfem_solver* S, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  if (!S->ws_aux || !S->ws_y || !S->guess) return fail(MFEM_EINVAL, "warm-start buffers not set");
+  hipLaunchKernelGGL(k_ws_apply, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, (hipStream_t)stream, *S);
+  return launched();
 }
 
 int mfem_spmv(const mfem_solver* S, const float* x, float* y, void* stream) {

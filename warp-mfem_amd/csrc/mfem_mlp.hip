@@ -1,0 +1,399 @@
+// mfem_mlp.hip -- per-vertex MLP for the learned CG warm start (gfx950).
+//
+// Network: 16 inputs -> 64 -> 64 -> 3 outputs (stored padded to 16), GELU
+// (tanh form) after the two hidden layers.  Weights are BF16 (copies of FP32
+// master weights), products run on MFMA v_mfma_f32_16x16x16_bf16 with FP32
+// accumulation, activations are re-quantised to BF16 between layers.
+//
+// One wave processes a tile of 64 samples (4 row blocks of 16).  Operands are
+// staged in LDS as row-major BF16 and read into MFMA fragments:
+//   A (16 x 16, M x K): lane l holds A[l % 16][4 (l / 16) + i], i = 0..3
+//   B (16 x 16, K x N): lane l holds B[4 (l / 16) + i][l % 16]
+//   D (16 x 16, M x N): lane l holds D[4 (l / 16) + i][l % 16]
+// (the CDNA3/4 layout of the 16x16x16 BF16 MFMA; tests/test_learned.py
+// checks the products against a NumPy model of the same network.)
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/mfem_amd.h"
+
+namespace {
+
+constexpr int kIn = 16, kHid = 64, kOut = 16;  // kOut = 3 used outputs padded to one MFMA tile
+constexpr int kTile = 64;                       // samples per wave
+constexpr int kBlockW = 4;                      // waves per workgroup
+// parameter layout (FP32 master and BF16 copy share it):
+//   W1 [16][64], b1 [64], W2 [64][64], b2 [64], W3 [64][16], b3 [16]
+constexpr int oW1 = 0, oB1 = oW1 + kIn * kHid, oW2 = oB1 + kHid, oB2 = oW2 + kHid * kHid, oW3 = oB2 + kHid,
+              oB3 = oW3 + kHid * kOut, kNParams = oB3 + kOut;
+static_assert(kNParams == MFEM_MLP_NPARAMS, "parameter count must match the header");
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// GELU, tanh form (Hendrycks & Gimpel 2016) and its derivative
+__device__ __forceinline__ float gelu(float x) {
+  const float k = 0.7978845608028654f;  // sqrt(2/pi)
+  float t = tanhf(k * (x + 0.044715f * x * x * x));
+  return 0.5f * x * (1.f + t);
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float k = 0.7978845608028654f;
+  float u = k * (x + 0.044715f * x * x * x);
+  float t = tanhf(u);
+  return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * k * (1.f + 3.f * 0.044715f * x * x);
+}
+
+// A fragment from a row-major [rows][ld] BF16 LDS array, rows r0.., cols k0..
+__device__ __forceinline__ v4s frag_a(const uint16_t* A, int ld, int r0, int k0, int lane) {
+  const uint16_t* p = A + (r0 + (lane & 15)) * ld + k0 + 4 * (lane >> 4);
+  v4s v;
+  v[0] = (short)p[0]; v[1] = (short)p[1]; v[2] = (short)p[2]; v[3] = (short)p[3];
+  return v;
+}
+// A fragment of A^T where A is row-major [K][ld] (so A^T[i][k] = A[k][i])
+__device__ __forceinline__ v4s frag_at(const uint16_t* A, int ld, int r0, int k0, int lane) {
+  const int i = r0 + (lane & 15), k = k0 + 4 * (lane >> 4);
+  v4s v;
+  v[0] = (short)A[(k + 0) * ld + i]; v[1] = (short)A[(k + 1) * ld + i];
+  v[2] = (short)A[(k + 2) * ld + i]; v[3] = (short)A[(k + 3) * ld + i];
+  return v;
+}
+// B fragment from row-major [K][ld] BF16 (B[k][j])
+__device__ __forceinline__ v4s frag_b(const uint16_t* B, int ld, int k0, int c0, int lane) {
+  const int j = c0 + (lane & 15), k = k0 + 4 * (lane >> 4);
+  v4s v;
+  v[0] = (short)B[(k + 0) * ld + j]; v[1] = (short)B[(k + 1) * ld + j];
+  v[2] = (short)B[(k + 2) * ld + j]; v[3] = (short)B[(k + 3) * ld + j];
+  return v;
+}
+// B fragment of B^T where B is row-major [N][ld] (B^T[k][j] = B[j][k])
+__device__ __forceinline__ v4s frag_bt(const uint16_t* B, int ld, int k0, int c0, int lane) {
+  const uint16_t* p = B + (c0 + (lane & 15)) * ld + k0 + 4 * (lane >> 4);
+  v4s v;
+  v[0] = (short)p[0]; v[1] = (short)p[1]; v[2] = (short)p[2]; v[3] = (short)p[3];
+  return v;
+}
+__device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+struct FwdLds {
+  uint16_t x[kTile][kIn];    // input tile (BF16)
+  uint16_t h1[kTile][kHid];  // gelu(z1) (BF16)
+  uint16_t h2[kTile][kHid];  // gelu(z2) (BF16)
+  float y[kTile][kOut];      // outputs
+};
+struct TrainLds {
+  FwdLds f;
+  float z1[kTile][kHid];  // pre-activations (FP32) for gelu'
+  float z2[kTile][kHid];
+  uint16_t dz2[kTile][kHid];  // BF16 backward operands
+  uint16_t dz1[kTile][kHid];
+  uint16_t dy[kTile][kOut];
+};
+struct WeightLds {
+  uint16_t w1[kIn][kHid];
+  uint16_t w2[kHid][kHid];
+  uint16_t w3[kHid][kOut];
+  float b1[kHid], b2[kHid], b3[kOut];
+};
+
+__device__ void load_weights(WeightLds& W, const uint16_t* wbf, const float* wf32) {
+  for (int i = threadIdx.x; i < kIn * kHid; i += blockDim.x) (&W.w1[0][0])[i] = wbf[oW1 + i];
+  for (int i = threadIdx.x; i < kHid * kHid; i += blockDim.x) (&W.w2[0][0])[i] = wbf[oW2 + i];
+  for (int i = threadIdx.x; i < kHid * kOut; i += blockDim.x) (&W.w3[0][0])[i] = wbf[oW3 + i];
+  for (int i = threadIdx.x; i < kHid; i += blockDim.x) {
+    W.b1[i] = wf32[oB1 + i];
+    W.b2[i] = wf32[oB2 + i];
+  }
+  for (int i = threadIdx.x; i < kOut; i += blockDim.x) W.b3[i] = wf32[oB3 + i];
+}
+
+// forward pass of one 64-sample tile already staged in L.x; z1 / z2 (may be
+// null) receive the FP32 pre-activations
+__device__ void forward_tile(const WeightLds& W, FwdLds& L, int lane, float (*z1)[kHid], float (*z2)[kHid]) {
+  // layer 1: z1 = x W1 + b1 (64 x 64)
+  for (int rb = 0; rb < 4; ++rb)
+    for (int cb = 0; cb < 4; ++cb) {
+      v4f acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mfma(frag_a(&L.x[0][0], kIn, 16 * rb, 0, lane), frag_b(&W.w1[0][0], kHid, 0, 16 * cb, lane), acc);
+      const int j = 16 * cb + (lane & 15);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = 16 * rb + 4 * (lane >> 4) + i;
+        const float z = acc[i] + W.b1[j];
+        if (z1) z1[r][j] = z;
+        L.h1[r][j] = f2bf(gelu(z));
+      }
+    }
+  __builtin_amdgcn_wave_barrier();
+  // layer 2: z2 = h1 W2 + b2
+  for (int rb = 0; rb < 4; ++rb)
+    for (int cb = 0; cb < 4; ++cb) {
+      v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+        acc = mfma(frag_a(&L.h1[0][0], kHid, 16 * rb, 16 * kb, lane), frag_b(&W.w2[0][0], kHid, 16 * kb, 16 * cb, lane), acc);
+      const int j = 16 * cb + (lane & 15);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = 16 * rb + 4 * (lane >> 4) + i;
+        const float z = acc[i] + W.b2[j];
+        if (z2) z2[r][j] = z;
+        L.h2[r][j] = f2bf(gelu(z));
+      }
+    }
+  __builtin_amdgcn_wave_barrier();
+  // layer 3: y = h2 W3 + b3 (64 x 16)
+  for (int rb = 0; rb < 4; ++rb) {
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+      acc = mfma(frag_a(&L.h2[0][0], kHid, 16 * rb, 16 * kb, lane), frag_b(&W.w3[0][0], kOut, 16 * kb, 0, lane), acc);
+    const int j = lane & 15;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) L.y[16 * rb + 4 * (lane >> 4) + i][j] = acc[i] + W.b3[j];
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Inference: Y[n][0..2] for n < N
+__global__ void __launch_bounds__(256) k_mlp_forward(const uint16_t* X, int N, const uint16_t* wbf, const float* wf32,
+                                                     float* Y) {
+  __shared__ WeightLds W;
+  __shared__ FwdLds Ls[kBlockW];
+  load_weights(W, wbf, wf32);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  FwdLds& L = Ls[wv];
+  const int ntiles = (N + kTile - 1) / kTile;
+  for (int t = blockIdx.x * kBlockW + wv; t < ntiles; t += gridDim.x * kBlockW) {
+    const int n = t * kTile + lane;
+#pragma unroll
+    for (int c = 0; c < kIn; ++c) L.x[lane][c] = n < N ? X[(size_t)n * kIn + c] : (uint16_t)0;
+    __builtin_amdgcn_wave_barrier();
+    forward_tile(W, L, lane, nullptr, nullptr);
+    if (n < N)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Y[(size_t)n * 3 + c] = L.y[lane][c];
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Training: loss and gradients for samples [0, N).  Each sample n has
+// aux[n][0..8] = candidates c1, c2, c3 (normalised) and aux[n][9..11] =
+// target t (normalised); prediction p = y0 c1 + y1 c2 + y2 c3 and loss
+// (1/N) sum |p - t|^2.  One wave per workgroup (the FP32 pre-activations of a
+// 64-sample tile fill most of the LDS); per-workgroup partial gradients go to
+// part[blockIdx.x][kNParams] and the partial loss to part_loss[blockIdx.x].
+__global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t* X, const float* aux, int N, const uint16_t* wbf,
+                                                 const float* wf32, float* part, float* part_loss) {
+  __shared__ WeightLds W;
+  __shared__ TrainLds T;
+  __shared__ float gacc[kNParams];
+  load_weights(W, wbf, wf32);
+  for (int i = threadIdx.x; i < kNParams; i += blockDim.x) gacc[i] = 0.f;
+  __syncthreads();
+  const int lane = threadIdx.x;
+  FwdLds& L = T.f;
+  const float inv_n = 1.f / (float)N;
+  float loss = 0.f;
+  // per-wave weight-gradient accumulators stay in MFMA accumulators across tiles
+  v4f gW1[4] = {}, gW2[16] = {}, gW3[4] = {};
+  float gb1 = 0.f, gb2 = 0.f, gb3 = 0.f;  // lane j < 64 (b1, b2) / < 3 (b3)
+  const int ntiles = (N + kTile - 1) / kTile;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int n = t * kTile + lane;
+    const bool ok = n < N;
+#pragma unroll
+    for (int c = 0; c < kIn; ++c) L.x[lane][c] = ok ? X[(size_t)n * kIn + c] : (uint16_t)0;
+    __builtin_amdgcn_wave_barrier();
+    forward_tile(W, L, lane, T.z1, T.z2);
+    // output gradient dL/dy_k = (2/N) (p - t) . c_k
+    float dy[3] = {0.f, 0.f, 0.f};
+    if (ok) {
+      const float* a = aux + (size_t)n * 12;
+      float e[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        e[d] = L.y[lane][0] * a[d] + L.y[lane][1] * a[3 + d] + L.y[lane][2] * a[6 + d] - a[9 + d];
+        loss += e[d] * e[d] * inv_n;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) dy[k] = 2.f * inv_n * (e[0] * a[3 * k] + e[1] * a[3 * k + 1] + e[2] * a[3 * k + 2]);
+    }
+#pragma unroll
+    for (int c = 0; c < kOut; ++c) T.dy[lane][c] = f2bf(c < 3 ? dy[c] : 0.f);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float s2 = dy[c];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) s2 += __shfl_xor(s2, off, 64);
+      if (lane == c) gb3 += s2;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // dW3 += h2^T dy  (64 x 16), K = 64 samples
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+        gW3[rb] = mfma(frag_at(&L.h2[0][0], kHid, 16 * rb, 16 * kb, lane), frag_b(&T.dy[0][0], kOut, 16 * kb, 0, lane), gW3[rb]);
+    // dz2 = (dy W3^T) * gelu'(z2)  (64 x 64)
+    for (int rb = 0; rb < 4; ++rb)
+      for (int cb = 0; cb < 4; ++cb) {
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma(frag_a(&T.dy[0][0], kOut, 16 * rb, 0, lane), frag_bt(&W.w3[0][0], kOut, 0, 16 * cb, lane), acc);
+        const int j = 16 * cb + (lane & 15);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = 16 * rb + 4 * (lane >> 4) + i;
+          T.dz2[r][j] = f2bf(acc[i] * gelu_grad(T.z2[r][j]));
+        }
+      }
+    __builtin_amdgcn_wave_barrier();
+    for (int r = 0; r < kTile; ++r) gb2 += bf2f(T.dz2[r][lane]);
+    // dW2 += h1^T dz2 (64 x 64)
+    for (int rb = 0; rb < 4; ++rb)
+      for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+          gW2[rb * 4 + cb] = mfma(frag_at(&L.h1[0][0], kHid, 16 * rb, 16 * kb, lane),
+                                  frag_b(&T.dz2[0][0], kHid, 16 * kb, 16 * cb, lane), gW2[rb * 4 + cb]);
+    // dz1 = (dz2 W2^T) * gelu'(z1)
+    for (int rb = 0; rb < 4; ++rb)
+      for (int cb = 0; cb < 4; ++cb) {
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+          acc = mfma(frag_a(&T.dz2[0][0], kHid, 16 * rb, 16 * kb, lane), frag_bt(&W.w2[0][0], kHid, 16 * kb, 16 * cb, lane), acc);
+        const int j = 16 * cb + (lane & 15);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = 16 * rb + 4 * (lane >> 4) + i;
+          T.dz1[r][j] = f2bf(acc[i] * gelu_grad(T.z1[r][j]));
+        }
+      }
+    __builtin_amdgcn_wave_barrier();
+    for (int r = 0; r < kTile; ++r) gb1 += bf2f(T.dz1[r][lane]);
+    // dW1 += x^T dz1 (16 x 64)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+        gW1[cb] = mfma(frag_at(&L.x[0][0], kIn, 0, 16 * kb, lane), frag_b(&T.dz1[0][0], kHid, 16 * kb, 16 * cb, lane), gW1[cb]);
+    __builtin_amdgcn_wave_barrier();
+  }
+  // this workgroup's gradient (one wave: plain stores, no races)
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) gacc[oW1 + (4 * (lane >> 4) + i) * kHid + 16 * cb + (lane & 15)] = gW1[cb][i];
+  for (int rb = 0; rb < 4; ++rb)
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        gacc[oW2 + (16 * rb + 4 * (lane >> 4) + i) * kHid + 16 * cb + (lane & 15)] = gW2[rb * 4 + cb][i];
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) gacc[oW3 + (16 * rb + 4 * (lane >> 4) + i) * kOut + (lane & 15)] = gW3[rb][i];
+  gacc[oB1 + lane] = gb1;
+  gacc[oB2 + lane] = gb2;
+  if (lane < kOut) gacc[oB3 + lane] = lane < 3 ? gb3 : 0.f;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kNParams; i += blockDim.x) part[(size_t)blockIdx.x * kNParams + i] = gacc[i];
+  if (threadIdx.x == 0) part_loss[blockIdx.x] = loss;
+}
+
+// Sum the per-workgroup gradients in a fixed order, Adam update of the FP32
+// master weights (Kingma & Ba 2015), refresh of the BF16 copies.
+// state[0..P) = m, state[P..2P) = v.  One thread per parameter.
+__global__ void __launch_bounds__(256) k_mlp_adam(const float* part, int nparts, float* wf32, uint16_t* wbf,
+                                                  float* state, float lr, float beta1, float beta2, float eps, int t,
+                                                  float* grad_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kNParams) return;
+  float g = 0.f;
+  for (int b = 0; b < nparts; ++b) g += part[(size_t)b * kNParams + i];
+  if (grad_out) grad_out[i] = g;
+  if (lr > 0.f) {
+    float m = beta1 * state[i] + (1.f - beta1) * g;
+    float v = beta2 * state[kNParams + i] + (1.f - beta2) * g * g;
+    state[i] = m;
+    state[kNParams + i] = v;
+    const float mh = m / (1.f - powf(beta1, (float)t));
+    const float vh = v / (1.f - powf(beta2, (float)t));
+    wf32[i] -= lr * mh / (sqrtf(vh) + eps);
+  }
+  wbf[i] = f2bf(wf32[i]);
+}
+
+__global__ void k_mlp_sum_loss(const float* part_loss, int nparts, float* loss) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    float s = 0.f;
+    for (int b = 0; b < nparts; ++b) s += part_loss[b];
+    *loss = s;
+  }
+}
+
+thread_local std::string g_mlp_err;
+
+}  // namespace
+
+extern "C" {
+
+const char* mfem_mlp_last_error(void) { return g_mlp_err.c_str(); }
+
+int mfem_mlp_forward(const uint16_t* X, int n, const uint16_t* w_bf16, const float* w_f32, float* y, void* stream) {
+  if (n < 0 || !X || !w_bf16 || !w_f32 || !y) {
+    g_mlp_err = "bad argument";
+    return MFEM_EINVAL;
+  }
+  if (n == 0) return MFEM_OK;
+  const int ntiles = (n + kTile - 1) / kTile;
+  int nb = (ntiles + kBlockW - 1) / kBlockW;
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(k_mlp_forward, dim3(nb), dim3(256), 0, (hipStream_t)stream, X, n, w_bf16, w_f32, y);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    g_mlp_err = hipGetErrorString(e);
+    return MFEM_EHIP;
+  }
+  return MFEM_OK;
+}
+
+int mfem_mlp_train_step(const uint16_t* X, const float* aux, int n, float* w_f32, uint16_t* w_bf16, float* adam_state,
+                        float* part, int max_parts, float* part_loss, float* loss_out, float* grad_out, float lr,
+                        int step, void* stream) {
+  if (n <= 0 || !X || !aux || !w_f32 || !w_bf16 || !adam_state || !part || !part_loss || max_parts <= 0) {
+    g_mlp_err = "bad argument";
+    return MFEM_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  int nb = (n + kTile - 1) / kTile;
+  if (nb > max_parts) nb = max_parts;
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(64), 0, st, X, aux, n, w_bf16, w_f32, part, part_loss);
+  hipLaunchKernelGGL(k_mlp_adam, dim3((kNParams + 255) / 256), dim3(256), 0, st, part, nb, w_f32, w_bf16, adam_state,
+                     lr, 0.9f, 0.999f, 1e-8f, step, grad_out);
+  if (loss_out) hipLaunchKernelGGL(k_mlp_sum_loss, dim3(1), dim3(64), 0, st, part_loss, nb, loss_out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    g_mlp_err = hipGetErrorString(e);
+    return MFEM_EHIP;
+  }
+  return MFEM_OK;
+}
+
+}  // extern "C"

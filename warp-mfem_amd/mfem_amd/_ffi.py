@@ -27,7 +27,7 @@ FIELDS = [
     "shape_scale", "shape_mu", "chunk_ptr", "sell_col", "sell_val", "diag_lane", "contrib_ptr", "contrib_src",
     "vinc_ptr", "vinc_src", "x", "v", "s", "lam", "x_tilde", "x_start", "lag_n", "lag_fn", "lag_sid",
     "G", "Hl", "gl", "Hsinv", "gs", "psi", "Kb", "fe", "gx", "gB", "gD", "Hv", "dist", "eB", "eD", "g",
-    "dx", "ds", "pz", "px", "red", "flags", "status", "stats",
+    "dx", "ds", "pz", "px", "red", "flags", "status", "stats", "mlp_w", "mlp_wbf", "ws_x", "ws_aux", "ws_y", "guess",
 )]
 
 
@@ -40,13 +40,15 @@ MFEM_STATS_PER_ITER = 16
 MFEM_MAX_STATS_ITERS = 64
 MFEM_RED_MAX_BLOCKS = 2048
 SHAPE_PLANE, SHAPE_CAPSULE = 0, 1
-PCG_PIPELINED, PCG_STREAM_ROWS = 1, 2
+PCG_PIPELINED, PCG_STREAM_ROWS, PCG_LEARNED_GUESS = 1, 2, 4
+MFEM_MLP_NPARAMS = 6288
 
 ENTRY_POINTS = [
     "mfem_solver_sizeof", "mfem_solver_offsetof", "mfem_version", "mfem_wall_clock_khz", "mfem_last_error",
     "mfem_init_stretch",
     "mfem_begin_step", "mfem_update_system", "mfem_assemble", "mfem_global_solve", "mfem_local_solve",
-    "mfem_line_search", "mfem_end_step", "mfem_step", "mfem_spmv", "mfem_merit",
+    "mfem_line_search", "mfem_end_step", "mfem_step", "mfem_spmv", "mfem_merit", "mfem_ws_features",
+    "mfem_ws_apply", "mfem_mlp_forward", "mfem_mlp_train_step", "mfem_mlp_last_error",
 ]
 
 
@@ -80,9 +82,15 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mfem_line_search.argtypes = [P, ctypes.c_float, ctypes.c_int, vp]
     lib.mfem_spmv.argtypes = [P, vp, vp, vp]
     lib.mfem_merit.argtypes = [P, ctypes.c_float, ctypes.c_float, vp, vp]
+    lib.mfem_ws_features.argtypes = [P, ctypes.c_float, ctypes.c_int, vp]
+    lib.mfem_ws_apply.argtypes = [P, vp]
+    lib.mfem_mlp_forward.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
+    lib.mfem_mlp_train_step.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp, vp, vp,
+                                        ctypes.c_float, ctypes.c_int, vp]
     lib.mfem_wall_clock_khz.restype = ctypes.c_int
     for name in ENTRY_POINTS[5:]:
         getattr(lib, name).restype = ctypes.c_int
+    lib.mfem_mlp_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 

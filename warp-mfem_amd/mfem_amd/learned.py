@@ -1,0 +1,205 @@
+"""Learned CG warm start: per-vertex MLP trained on the solver's own solves.
+
+The reference starts every CG solve from x~ - x (src/mfem/refinement/
+kernels.py ``compute_initial_guess``, called at solver.py:530-531) and has no
+learned component.  Here a small per-vertex network (16 -> 64 -> 64 -> 3,
+GELU, BF16 weights, FP32 accumulation on MFMA; csrc/mfem_mlp.hip) blends
+three candidate guesses per vertex,
+
+    dx0_i = s_i (y0 c1_i + y1 c2_i + y2 c3_i),
+    c1 = x~ - x,  c2 = D^-1 (g - H c1),  c3 = previous Newton step,
+
+from 16 rotation-invariant features (candidate norms and angles, diagonal /
+off-diagonal stiffness ratios, mass term, pinning, Newton iteration).  The
+PCG keeps the learned guess only when its residual |g - H dx0| is below the
+plain guess's, so block-Jacobi CG from x~ - x stays the fallback and the
+judge of convergence (the FP64 relative residual).
+
+Training data are (features, candidates, PCG solution) triples logged by
+``collect`` from plain-PCG steps; ``WarmStartNet.fit`` trains on the GPU with
+the project's forward/backward kernels and Adam.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _ffi
+
+N_PARAMS = _ffi.MFEM_MLP_NPARAMS
+N_IN, N_HID, N_OUT = 16, 64, 16
+O_W1 = 0
+O_B1 = O_W1 + N_IN * N_HID
+O_W2 = O_B1 + N_HID
+O_B2 = O_W2 + N_HID * N_HID
+O_W3 = O_B2 + N_HID
+O_B3 = O_W3 + N_HID * N_OUT
+MAX_PARTS = 1024  # workgroups of the gradient kernel (per-workgroup partial gradients)
+
+
+def _ptr(t: torch.Tensor | None):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _mlp_call(lib, name, *args):
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        raise RuntimeError(f"{name} failed ({rc}): {lib.mfem_mlp_last_error().decode()}")
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """Initial FP32 parameters: scaled normal hidden layers, a small output
+    layer and output bias (1, 0, 0), so the untrained network already
+    returns (about) the plain guess x~ - x."""
+    rng = np.random.default_rng(seed)
+    w = np.zeros(N_PARAMS, np.float32)
+    w[O_W1:O_B1] = rng.normal(0, 1.0 / np.sqrt(N_IN), N_IN * N_HID)
+    w[O_W2:O_B2] = rng.normal(0, 1.0 / np.sqrt(N_HID), N_HID * N_HID)
+    W3 = np.zeros((N_HID, N_OUT), np.float32)
+    W3[:, :3] = rng.normal(0, 1e-2, (N_HID, 3))
+    w[O_W3:O_B3] = W3.ravel()
+    w[O_B3] = 1.0
+    return w
+
+
+class WarmStartNet:
+    """FP32 master weights, their BF16 copy and Adam state, on one device."""
+
+    def __init__(self, device="cuda", weights: np.ndarray | None = None, seed: int = 0):
+        self.lib = _ffi.load()
+        self.device = torch.device(device)
+        if self.device.type == "cuda" and self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        w = init_params(seed) if weights is None else np.asarray(weights, np.float32)
+        if w.shape != (N_PARAMS,):
+            raise ValueError(f"weights must hold {N_PARAMS} values")
+        self.w = torch.as_tensor(w.copy(), device=self.device)
+        self.wbf = torch.empty(N_PARAMS, dtype=torch.int16, device=self.device)
+        self.adam = torch.zeros(2 * N_PARAMS, dtype=torch.float32, device=self.device)
+        self.t = 0
+        self._part = None
+        self._part_loss = None
+        self.refresh()
+
+    def refresh(self) -> None:
+        """BF16 copy of the FP32 weights (round to nearest even)."""
+        self.wbf.copy_(self.w.to(torch.bfloat16).view(torch.int16))
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def forward(self, X: torch.Tensor) -> torch.Tensor:
+        """y (n, 3) for BF16 features X (n, 16) stored as int16."""
+        X = self._features(X)
+        y = torch.empty(X.shape[0], 3, dtype=torch.float32, device=self.device)
+        _mlp_call(self.lib, "mfem_mlp_forward", _ptr(X), ctypes.c_int(X.shape[0]), _ptr(self.wbf), _ptr(self.w),
+                  _ptr(y), self._stream())
+        return y
+
+    def _features(self, X):
+        if X.dtype == torch.bfloat16:
+            X = X.view(torch.int16)
+        if X.dtype != torch.int16 or X.device != self.device or X.dim() != 2 or X.shape[1] != N_IN:
+            raise ValueError("features must be a (n, 16) BF16 (or int16 bit-pattern) tensor on the net's device")
+        return X.contiguous()
+
+    def _scratch(self):
+        if self._part is None:
+            self._part = torch.empty(MAX_PARTS * N_PARAMS, dtype=torch.float32, device=self.device)
+            self._part_loss = torch.empty(MAX_PARTS, dtype=torch.float32, device=self.device)
+
+    def train_step(self, X, aux, lr: float, loss_out: torch.Tensor | None = None,
+                   grad_out: torch.Tensor | None = None) -> None:
+        """One Adam step on the mean blend loss of the batch (aux (n, 12) =
+        normalised c1, c2, c3 and target).  lr = 0 computes the gradient only
+        (into grad_out) and leaves the weights unchanged."""
+        X = self._features(X)
+        aux = aux.to(device=self.device, dtype=torch.float32).contiguous()
+        if aux.shape != (X.shape[0], 12):
+            raise ValueError("aux must be (n, 12)")
+        self._scratch()
+        if lr > 0:
+            self.t += 1
+        _mlp_call(self.lib, "mfem_mlp_train_step", _ptr(X), _ptr(aux), ctypes.c_int(X.shape[0]), _ptr(self.w),
+                  _ptr(self.wbf), _ptr(self.adam), _ptr(self._part), ctypes.c_int(MAX_PARTS), _ptr(self._part_loss),
+                  _ptr(loss_out), _ptr(grad_out), ctypes.c_float(lr), ctypes.c_int(max(self.t, 1)), self._stream())
+
+    def loss(self, X, aux) -> float:
+        out = torch.zeros(1, dtype=torch.float32, device=self.device)
+        self.train_step(X, aux, 0.0, loss_out=out)
+        return float(out.item())
+
+    def fit(self, X, aux, epochs: int = 20, batch: int = 65536, lr: float = 2e-3, seed: int = 0,
+            val_fraction: float = 0.1) -> dict:
+        """Mini-batch Adam on the GPU.  Returns train / validation loss per
+        epoch (validation split fixed by ``seed``)."""
+        X = self._features(X)
+        aux = aux.to(device=self.device, dtype=torch.float32)
+        n = X.shape[0]
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        perm = torch.randperm(n, generator=g).to(self.device)
+        n_val = int(n * val_fraction)
+        val, tr = perm[:n_val], perm[n_val:]
+        hist = {"train": [], "val": []}
+        loss_buf = torch.zeros(1, dtype=torch.float32, device=self.device)
+        for ep in range(epochs):
+            order = tr[torch.randperm(len(tr), generator=g).to(self.device)]
+            tot, nb = 0.0, 0
+            for b0 in range(0, len(order), batch):
+                idx = order[b0:b0 + batch]
+                if len(idx) < 64:
+                    continue
+                self.train_step(X[idx], aux[idx], lr, loss_out=loss_buf)
+                tot += float(loss_buf.item())
+                nb += 1
+            hist["train"].append(tot / max(nb, 1))
+            if n_val >= 64:
+                hist["val"].append(self.loss(X[val], aux[val]))
+        return hist
+
+    def weights(self) -> np.ndarray:
+        return self.w.cpu().numpy().copy()
+
+    def save(self, path: str) -> None:
+        np.save(path, self.weights())
+
+    @classmethod
+    def load(cls, path: str, device="cuda") -> "WarmStartNet":
+        return cls(device, weights=np.load(path, allow_pickle=False))
+
+
+def collect(solver, state, dt: float, steps: int, every: int = 1, max_target: float = 8.0):
+    """Runs ``steps`` plain-PCG steps stage by stage on ``state`` (advanced in
+    place) and logs, per Newton iteration, the warm-start features, the
+    normalised candidates and the PCG solution as training pairs.
+
+    Returns (X (n, 16) int16 BF16 bits, aux (n, 12) float32) on the device:
+    aux = (c1, c2, c3, dx / s) with s the candidates' scale; vertices with no
+    scale and targets beyond ``max_target`` (in units of s) are left out.
+    """
+    Xs, As = [], []
+    out = state.clone()
+    b = solver.buffers
+    for k in range(steps):
+        out.assign(state)
+        solver.bind(out)
+        solver.begin_step(dt)
+        for it in range(solver.iterations):
+            solver.update_system(dt)
+            solver.assemble(dt)
+            solver.ws_features(dt, it)
+            solver.global_solve(it, learned=False)
+            if k % every == 0:
+                s = b["ws_aux"][:, 9:10]
+                t = b["dx"].float() / s
+                # drop vertices at rest (no scale) and outliers far from every candidate
+                keep = (s[:, 0] > 1e-12 * float(s.max())) & (t.abs().amax(dim=1) < max_target)
+                Xs.append(b["ws_x"][keep].clone())
+                As.append(torch.cat([b["ws_aux"][keep, :9], t[keep]], dim=1))
+            solver.local_solve()
+            solver.line_search(dt, it)
+        solver.end_step(dt)
+        state.assign(out)
+    return torch.cat(Xs), torch.cat(As)

@@ -178,6 +178,11 @@ class RefinementSolver:
         b["flags"] = torch.zeros(_ffi.MFEM_RED_MAX_BLOCKS, dtype=torch.int32, device=dev)
         b["status"] = torch.zeros(8, dtype=torch.int32, device=dev)
         b["stats"] = torch.zeros(_ffi.MFEM_MAX_STATS_ITERS * _ffi.MFEM_STATS_PER_ITER, dtype=torch.float64, device=dev)
+        # learned warm start workspace (features, candidates, outputs, guess)
+        b["ws_x"] = torch.zeros(nv, 16, dtype=torch.int16, device=dev)
+        b["ws_aux"] = zf(nv, 10)
+        b["ws_y"] = zf(nv, 3)
+        b["guess"] = zf(nv, 3)
         self.buffers = b
 
         S = _ffi.MfemSolverStruct()
@@ -210,6 +215,8 @@ class RefinementSolver:
         for name, t in b.items():
             setattr(S, name, t.data_ptr())
         self._S = S
+        self.warm_start_net = None
+        self.set_warm_start(kwargs.get("warm_start"))
 
         # AdditionalState.from_model: rest stretch from the model's positions
         x0 = f32(np.asarray(model.particle_q).reshape(-1, 3))
@@ -239,6 +246,21 @@ class RefinementSolver:
         if bq.shape[0] != self._S.n_shapes:
             raise ValueError("body_q must hold one (p, q) pose per shape")
         self.buffers["shape_xf"].copy_(bq)
+
+    def set_warm_start(self, net) -> None:
+        """Attach a trained ``learned.WarmStartNet`` (or None for plain PCG).
+        With a network, every PCG solve first evaluates it and starts from the
+        learned guess when that has the smaller residual."""
+        self.warm_start_net = net
+        if net is None:
+            self._S.pcg_flags &= ~_ffi.PCG_LEARNED_GUESS
+            self._S.mlp_w = self._S.mlp_wbf = None
+            return
+        if net.device != self.device:
+            raise ValueError("the warm-start network must live on the solver's device")
+        self._S.mlp_w = net.w.data_ptr()
+        self._S.mlp_wbf = net.wbf.data_ptr()
+        self._S.pcg_flags |= _ffi.PCG_LEARNED_GUESS
 
     # -- API ----------------------------------------------------------------
     def step(self, state_in: State, state_out: State, control=None, contacts=None, dt: float = 1.0 / 60.0) -> None:
@@ -275,6 +297,8 @@ class RefinementSolver:
         for it in range(self.iterations):
             timed("update_system", self.update_system, dt)
             timed("assemble", self.assemble, dt)
+            if self.warm_start_net is not None:
+                timed("warm_start", self.warm_start, dt, it)
             timed("global_solve", self.global_solve, it)
             timed("local_solve", self.local_solve)
             timed("line_search", self.line_search, dt, it)
@@ -294,8 +318,38 @@ class RefinementSolver:
     def assemble(self, dt):
         self._call("mfem_assemble", ctypes.c_float(dt), self._stream())
 
-    def global_solve(self, newton_iter=0):
-        self._call("mfem_global_solve", ctypes.c_int(newton_iter), self._stream())
+    def ws_features(self, dt, newton_iter=0):
+        """Warm-start features and candidates into buffers ws_x / ws_aux."""
+        self._call("mfem_ws_features", ctypes.c_float(dt), ctypes.c_int(newton_iter), self._stream())
+
+    def warm_start(self, dt, newton_iter=0):
+        """Features, network and blend: the learned initial guess (buffer
+        ``guess``) that the next ``global_solve`` tries first."""
+        net = self.warm_start_net
+        if net is None:
+            raise RuntimeError("no warm-start network attached")
+        self.ws_features(dt, newton_iter)
+        b = self.buffers
+        _ffi_mlp = self.lib.mfem_mlp_forward
+        rc = _ffi_mlp(ctypes.c_void_p(b["ws_x"].data_ptr()), ctypes.c_int(self.max_particles),
+                      ctypes.c_void_p(net.wbf.data_ptr()), ctypes.c_void_p(net.w.data_ptr()),
+                      ctypes.c_void_p(b["ws_y"].data_ptr()), self._stream())
+        if rc != 0:
+            raise RuntimeError(f"mfem_mlp_forward failed ({rc}): {self.lib.mfem_mlp_last_error().decode()}")
+        self._call("mfem_ws_apply", self._stream())
+
+    def global_solve(self, newton_iter=0, learned: bool | None = None):
+        """PCG on H dx = g.  learned=None follows the attached network (the
+        guess from the last ``warm_start``); False forces the plain guess."""
+        flags = self._S.pcg_flags
+        if learned is not None:
+            if learned and self.warm_start_net is None:
+                raise RuntimeError("no warm-start network attached")
+            self._S.pcg_flags = (flags | _ffi.PCG_LEARNED_GUESS) if learned else (flags & ~_ffi.PCG_LEARNED_GUESS)
+        try:
+            self._call("mfem_global_solve", ctypes.c_int(newton_iter), self._stream())
+        finally:
+            self._S.pcg_flags = flags
 
     def local_solve(self):
         self._call("mfem_local_solve", self._stream())
@@ -321,7 +375,8 @@ class RefinementSolver:
     # diagnostics ------------------------------------------------------------
     def last_stats(self) -> np.ndarray:
         """Per Newton iteration of the last step: cg_iters, cg_relres (FP64),
-        ls_trials, ls_accepted, alpha, f0, f1, cg_restarts."""
+        ls_trials, ls_accepted, alpha, f0, f1, cg_restarts, PCG phase ticks
+        (6), learned guess used, learned / plain initial residual ratio."""
         torch.cuda.synchronize(self.device)
         st = self.buffers["stats"].cpu().numpy().reshape(_ffi.MFEM_MAX_STATS_ITERS, _ffi.MFEM_STATS_PER_ITER)
         return st[: self.iterations].copy()
