@@ -1,4 +1,5 @@
-"""Benchmark: tet-steps/s of the per-timestep soft-body solve on one MI355X.
+"""Benchmark: tet-steps/s of the per-timestep soft-body solve on one MI355X,
+learned CG warm start vs plain block-Jacobi PCG.
 
 Workload (default): the ~2e5-tet octopus-like body (ellipsoid mantle + 8
 tapered arms, Neo-Hookean with a log-normal stiffness field) on a ground
@@ -6,16 +7,28 @@ plane with IPC barrier (dhat = 1e-3) and friction (mu = 0.3), dt = 1/100,
 12 Newton iterations per step, block-Jacobi PCG to relative residual 1e-6
 (FP64 residual), line search on.  Synthetic mesh, generated in-process.
 
-One "step" = RefinementSolver.step on the whole body.  value = tets * steps /
-wall time of the timed steps (inputs resident in HBM).  The roofline object
-is for the dominant kernel, the persistent PCG (k_pcg): algorithmic bytes per
-CG iteration = 40 B per nonzero 3x3 block (36 B values + 4 B column) + 24 B
-per vertex (gather and publish of the FP32 direction), times the iterations
-each launch ran, divided by the launch's duration from HIP events on the
-launching stream.  cpu_baseline = the FP64 C oracle (oracle/mfem_oracle.c),
-one step of the same scene from its initial state, one thread.
+Order: W warmup steps (plain PCG); ``--train-steps`` plain steps whose
+solves are logged (features, candidates, right-hand side, matrix); training
+of the warm-start MLP on the GPU on the solver's own objective, the initial
+CG residual (BF16 MFMA forward/backward, SpMV, Adam); then K timed steps with
+plain PCG, K with the learned warm start and, as an ablation, K with the
+network's correction switched off (previous Newton step as the guess), all
+from the same state.  One "step" = RefinementSolver.step on the whole body.
+value = tets * K / wall time of the learned leg (inputs resident in HBM);
+the plain leg's rate is reported beside it (config.plain_value).
+
+The roofline object is for the dominant kernel, the persistent PCG (k_pcg):
+algorithmic bytes per CG iteration = 40 B per nonzero 3x3 block (36 B values
++ 4 B column) + 24 B per vertex (gather and publish of the FP32 direction),
+times the iterations each launch ran, divided by the launch's duration from
+HIP events on the launching stream.  The mlp object is the MFMA figure of the
+network's forward kernel: 2 * (16*64 + 64*64 + 64*3) FLOP per vertex over its
+event-timed duration, against the dense BF16 peak.  cpu_baseline = the FP64 C
+oracle (oracle/mfem_oracle.c), one step of the same scene from its initial
+state, one thread.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--train-steps T] [--epochs E] [--no-learned]
 """
 import argparse
 import json
@@ -29,6 +42,7 @@ for _p in (ROOT, os.path.join(ROOT, "warp-mfem_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA
 
 
 def main():
@@ -40,6 +54,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--pcg", default="standard", choices=["standard", "pipelined"])
     ap.add_argument("--stream-rows", action="store_true", help="stream matrix rows instead of keeping them in VGPRs")
+    ap.add_argument("--train-steps", type=int, default=6, help="plain steps logged as training solves")
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--no-learned", action="store_true", help="time plain PCG only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -72,32 +89,74 @@ def main():
     if solver.status() != 0:
         raise SystemExit("device error during warmup")
 
-    timers: dict = {}
-    hist = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        solver.step_instrumented(state, out, dt, timers)
-        hist.append(solver.buffers["stats"].clone())
-        state, out = out, state
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    print(f"[bench] timed {args.steps} steps in {wall:.3f} s", flush=True)
-    if solver.status() != 0:
-        raise SystemExit("device error during the timed steps")
-
     from mfem_amd import _ffi
+    from mfem_amd.learned import WarmStartNet, collect_solves
 
-    st = np.stack([h.cpu().numpy().reshape(_ffi.MFEM_MAX_STATS_ITERS, _ffi.MFEM_STATS_PER_ITER)[: solver.iterations]
-                   for h in hist])  # (K, iters, stats)
+    learned = not args.no_learned
+    train_info = None
+    net = None
+    if learned:
+        t0 = time.perf_counter()
+        solves = collect_solves(solver, state, dt, steps=args.train_steps)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        net = WarmStartNet("cuda:0")
+        hist = net.fit_residual(solves, solver, epochs=args.epochs, lr=1e-4, seed=0)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        train_info = {"solves": len(solves), "pairs": len(solves) * m.particle_count, "train_steps": args.train_steps,
+                      "epochs": args.epochs, "objective": "log |g - H dx0|^2 per solve", "collect_s": t1 - t0,
+                      "train_s": t2 - t1, "loss_prev_newton_step": hist["base"],
+                      "loss_per_epoch": [round(v, 4) for v in hist["loss"]], "best_epoch": hist["best_epoch"]}
+        print(f"[bench] trained on {len(solves)} solves in {t2 - t1:.2f} s, objective {hist['loss'][0]:.4g} -> "
+              f"{min(hist['loss']):.4g} (kept epoch {hist['best_epoch']}; previous Newton step alone: "
+              f"{hist['base']:.4g})", flush=True)
+        del solves
+
     khz = solver.lib.mfem_wall_clock_khz()
+
+    def timed_leg(s0, use_net):
+        st = s0.clone()
+        o = st.clone()
+        solver.set_warm_start(use_net)
+        timers: dict = {}
+        hist = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            solver.step_instrumented(st, o, dt, timers)
+            hist.append(solver.buffers["stats"].clone())
+            st, o = o, st
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        if solver.status() != 0:
+            raise SystemExit("device error during the timed steps")
+        stats = np.stack([h.cpu().numpy().reshape(_ffi.MFEM_MAX_STATS_ITERS, _ffi.MFEM_STATS_PER_ITER)
+                          [: solver.iterations] for h in hist])  # (K, iters, stats)
+        assert np.all(stats[:, :, 1] <= 1e-6), "a PCG solve missed rel-res 1e-6"
+        return wall, stats, timers, st
+
+    s0 = state.clone()
+    wall_p, st_p, timers_p, end_p = timed_leg(s0, None)
+    print(f"[bench] plain: {args.steps} steps in {wall_p:.3f} s", flush=True)
+    if learned:
+        wall, st, timers, end_l = timed_leg(s0, net)
+        print(f"[bench] learned: {args.steps} steps in {wall:.3f} s", flush=True)
+        pos_diff = float((end_l.particle_q - end_p.particle_q).abs().max().item())
+        # ablation: the same warm start with the network's correction off
+        # (zero output layer: the previous Newton step, same fallback)
+        base_net = WarmStartNet("cuda:0", weights=np.zeros(net.w.numel(), np.float32))
+        wall_b, st_b, _, _ = timed_leg(s0, base_net)
+        print(f"[bench] previous Newton step only: {args.steps} steps in {wall_b:.3f} s", flush=True)
+    else:
+        wall, st, timers = wall_p, st_p, timers_p
+
     ticks = st[:, :, 8:14].sum(axis=(0, 1))
     n_it = max(1.0, float(st[:, :, 0].sum()))
     pcg_phase_us = {name: float(ticks[k] / khz * 1e3 / n_it) for k, name in
                     enumerate(["spmv", "reduce1", "update", "reduce2", "checks", "prologue"])}
     cg_iters = st[:, :, 0]
     relres = st[:, :, 1]
-    assert np.all(relres <= 1e-6), "a PCG solve missed rel-res 1e-6"
     phase_ms = {k: sum(a.elapsed_time(b) for a, b in v) for k, v in timers.items()}
     pcg_ms = np.array([a.elapsed_time(b) for a, b in timers["global_solve"]])
     nnz = int(len(solver.pattern.cols))
@@ -109,6 +168,38 @@ def main():
     traffic = None
 
     value = T * args.steps / wall
+    value_plain = T * args.steps / wall_p
+    config = {
+        "workload": wl,
+        "tets": T,
+        "verts": V,
+        "newton_iterations": solver.iterations,
+        "dt": dt,
+        "preconditioner": "block_jacobi",
+        "pcg": args.pcg,
+        "learned_warm_start": learned,
+        "reductions": "f64",
+        "plain_value": value_plain,
+        "plain_ms_per_step": wall_p * 1e3 / args.steps,
+        "learned_speedup": value / value_plain,
+        "cg_iters_per_step": float(cg_iters.sum(axis=1).mean()),
+        "plain_cg_iters_per_step": float(st_p[:, :, 0].sum(axis=1).mean()),
+        "cg_iters_mean": float(cg_iters.mean()),
+        "cg_relres_max": float(relres.max()),
+        "cg_iters_last_step": cg_iters[-1].astype(int).tolist(),
+        "cg_restarts_last_step": st[-1, :, 7].astype(int).tolist(),
+        "ls_trials_last_step": st[-1, :, 2].astype(int).tolist(),
+        "pcg_phase_us_per_iter": pcg_phase_us,
+        "phase_ms_per_step": {k: v / args.steps for k, v in phase_ms.items()},
+    }
+    if learned:
+        config["learned_guess_used_frac"] = float(st[:, :, 14].mean())
+        config["learned_guess_residual_ratio_median"] = float(np.median(st[:, :, 15]))
+        config["max_abs_position_diff_vs_plain"] = pos_diff
+        config["training"] = train_info
+        config["ablation_prev_newton_step_only"] = {
+            "value": T * args.steps / wall_b, "ms_per_step": wall_b * 1e3 / args.steps,
+            "cg_iters_per_step": float(st_b[:, :, 0].sum(axis=1).mean())}
     result = {
         "metric": "tet-steps/s at CG rel-res 1e-6, learned-precond vs plain PCG, 1 MI355X",
         "value": value,
@@ -122,25 +213,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": wl,
-            "tets": T,
-            "verts": V,
-            "newton_iterations": solver.iterations,
-            "dt": dt,
-            "preconditioner": "block_jacobi",
-            "pcg": args.pcg,
-            "learned_preconditioner": False,
-            "reductions": "f64",
-            "cg_iters_mean": float(cg_iters.mean()),
-            "cg_iters_per_step": float(cg_iters.sum(axis=1).mean()),
-            "cg_relres_max": float(relres.max()),
-            "cg_iters_last_step": cg_iters[-1].astype(int).tolist(),
-            "cg_restarts_last_step": st[-1, :, 7].astype(int).tolist(),
-            "ls_trials_last_step": st[-1, :, 2].astype(int).tolist(),
-            "pcg_phase_us_per_iter": pcg_phase_us,
-            "phase_ms_per_step": {k: v / args.steps for k, v in phase_ms.items()},
-        },
+        "config": config,
         "roofline": {
             "kernel": "k_pcg",
             "bound": "hbm",
@@ -153,9 +226,34 @@ def main():
             "us_per_cg_iter": float(np.sum(pcg_ms[active]) * 1e3 / max(1.0, np.sum(iters_flat[active]))),
         },
     }
+    if learned:
+        result["mlp"] = mlp_mfma(solver, net, V)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(m, dt)
     print(json.dumps(result), flush=True)
+
+
+def mlp_mfma(solver, net, V):
+    """Event-timed forward pass of the warm-start MLP over the body's
+    features (the launch the solver makes once per Newton iteration)."""
+    import torch
+
+    X = solver.buffers["ws_x"]
+    for _ in range(3):
+        net.forward(X)
+    n = 20
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(n):
+        net.forward(X)
+    b.record()
+    torch.cuda.synchronize()
+    us = a.elapsed_time(b) * 1e3 / n
+    flop = 2.0 * V * (16 * 64 + 64 * 64 + 64 * 3)
+    tflops = flop / (us * 1e-6) / 1e12
+    return {"kernel": "k_mlp_forward", "bound": "mfma", "achieved": tflops, "peak": BF16_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": tflops / BF16_PEAK_TFLOPS, "us_per_launch": us, "flop_per_launch": flop}
 
 
 def cpu_baseline(m, dt):

@@ -55,7 +55,8 @@ extern "C" {
 #define MFEM_PCG_LEARNED_GUESS 4 /* start from the learned warm start when it has the smaller residual */
 
 /* status[] words */
-#define MFEM_STATUS_ERRBITS 0 /* bit 0: grid barrier timed out */
+#define MFEM_STATUS_ERRBITS 0 /* bit 0: grid barrier timed out; the persistent kernels (PCG, line search)
+                                 then return at once until status and flags are cleared */
 #define MFEM_STATUS_EPOCH 1   /* grid-barrier epoch counter (internal) */
 
 /*
@@ -213,9 +214,11 @@ int mfem_step(const mfem_solver* S, float dt, void* stream);
    solver.py:530-531).  Per vertex: candidates c1 = x~ - x, c2 = D^-1 (g - H c1)
    (one block-Jacobi step), c3 = the previous Newton step, normalised by
    s = |c1| + |c2| + |c3|, and 16 rotation-invariant features (BF16) into
-   ws_x / ws_aux. */
+   ws_x / ws_aux; feature 15 is the line-search step of Newton iteration
+   newton_iter - 1 (read from stats). */
 int mfem_ws_features(const mfem_solver* S, float dt, int newton_iter, void* stream);
-/* guess = s (y0 c1 + y1 c2 + y2 c3) from the network outputs ws_y. */
+/* guess = c3 + s (y0 c1 + y1 c2 + y2 c3) from the network outputs ws_y:
+   the previous Newton step plus a learned correction. */
 int mfem_ws_apply(const mfem_solver* S, void* stream);
 
 /* Per-vertex MLP (16 -> 64 -> 64 -> 3, GELU, BF16 MFMA with FP32 accumulate):
@@ -229,6 +232,13 @@ int mfem_mlp_forward(const uint16_t* x, int n, const uint16_t* w_bf16, const flo
 int mfem_mlp_train_step(const uint16_t* x, const float* aux, int n, float* w_f32, uint16_t* w_bf16, float* adam_state,
                         float* part, int max_parts, float* part_loss, float* loss_out, float* grad_out, float lr,
                         int step, void* stream);
+/* One Adam step from a given output gradient dy[n][3] = dL/dy (any loss on
+   the network outputs, e.g. the relative initial CG residual): backward pass,
+   fixed-order sum of the per-workgroup gradients, Adam, BF16 refresh.
+   lr = 0 only computes the gradient into grad_out (may be null). */
+int mfem_mlp_backward_step(const uint16_t* x, const float* dy, int n, float* w_f32, uint16_t* w_bf16,
+                           float* adam_state, float* part, int max_parts, float* grad_out, float lr, int step,
+                           void* stream);
 const char* mfem_mlp_last_error(void);
 
 /* y = H x with the assembled SELL matrix (warp.sparse.bsr_mv). */

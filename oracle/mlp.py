@@ -87,36 +87,58 @@ def forward(w, X, quantize: bool = True):
     return y[:, :3], dict(X=X, W1=W1, W2=W2, W3=W3, z1=z1, h1=h1, z2=z2, h2=h2)
 
 
-def loss_and_grad(w, X, aux, quantize: bool = True):
-    """Loss (1/n) sum_n |y0 c1 + y1 c2 + y2 c3 - t|^2 with aux[n] = (c1, c2,
-    c3, t) and its gradient with respect to the parameter vector (the
-    kernels' backward pass: BF16 dy, dz2, dz1 when quantize)."""
-    aux = np.asarray(aux, np.float64).reshape(-1, 12)
-    n = aux.shape[0]
-    y, c = forward(w, X, quantize)
-    C = aux[:, :9].reshape(n, 3, 3)  # C[n, k] = candidate k
-    t = aux[:, 9:12]
-    e = np.einsum("nk,nkd->nd", y.astype(np.float64), C) - t
-    loss = float(np.sum(e * e) / n)
-    dy3 = (2.0 / n) * np.einsum("nd,nkd->nk", e, C)
+def backward(w, X, dy3, quantize: bool = True):
+    """Gradient with respect to the parameter vector for an output gradient
+    dy3 = dL/dy (n, 3): the kernels' backward pass (BF16 dy, dz2, dz1 when
+    quantize; the b3 gradient sums the unrounded dy)."""
+    dy3 = np.asarray(dy3, np.float64)
+    n = dy3.shape[0]
+    _, c = forward(w, X, quantize)
     dy = np.zeros((n, N_OUT))
     dy[:, :3] = dy3
     q = bf16 if quantize else (lambda a: a)
-    dyq = q(dy).astype(np.float64)
-    h1, h2 = c["h1"].astype(np.float64), c["h2"].astype(np.float64)
+    dyq = np.asarray(q(dy), np.float64)
+    h1, h2 = np.asarray(c["h1"], np.float64), np.asarray(c["h2"], np.float64)
     gW3 = h2.T @ dyq
     gb3 = np.zeros(N_OUT)
     gb3[:3] = dy3.sum(axis=0)
-    dz2 = q(dyq @ np.asarray(c["W3"], np.float64).T * gelu_grad(np.asarray(c["z2"], np.float64)))
-    dz2 = np.asarray(dz2, np.float64)
+    dz2 = np.asarray(q(dyq @ np.asarray(c["W3"], np.float64).T * gelu_grad(np.asarray(c["z2"], np.float64))),
+                     np.float64)
     gW2 = h1.T @ dz2
     gb2 = dz2.sum(axis=0)
     dz1 = np.asarray(q(dz2 @ np.asarray(c["W2"], np.float64).T * gelu_grad(np.asarray(c["z1"], np.float64))),
                      np.float64)
     gW1 = np.asarray(c["X"], np.float64).T @ dz1
     gb1 = dz1.sum(axis=0)
-    g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
-    return loss, g
+    return np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
+
+
+def loss_and_grad(w, X, aux, quantize: bool = True):
+    """Loss (1/n) sum_n |y0 c1 + y1 c2 + y2 c3 - t|^2 with aux[n] = (c1, c2,
+    c3, t) and its gradient with respect to the parameter vector."""
+    aux = np.asarray(aux, np.float64).reshape(-1, 12)
+    n = aux.shape[0]
+    y, _ = forward(w, X, quantize)
+    C = aux[:, :9].reshape(n, 3, 3)  # C[n, k] = candidate k
+    t = aux[:, 9:12]
+    e = np.einsum("nk,nkd->nd", y.astype(np.float64), C) - t
+    loss = float(np.sum(e * e) / n)
+    dy3 = (2.0 / n) * np.einsum("nd,nkd->nk", e, C)
+    return loss, backward(w, X, dy3, quantize)
+
+
+def residual_loss_and_dy(y, cand, g, H):
+    """Log of the squared initial CG residual of the corrected guess,
+    L = log |g - H dx0|^2 with dx0 = c3 + y0 c1 + y1 c2 + y2 c3 (absolute
+    candidates cand (V, 9)), and dL/dy (V, 3) = c_k . (-2 H r / |r|^2)."""
+    V = g.shape[0]
+    C = np.asarray(cand, np.float64).reshape(V, 3, 3)
+    y = np.asarray(y, np.float64)
+    dx0 = C[:, 2] + np.einsum("nk,nkd->nd", y, C)
+    r = np.asarray(g, np.float64) - (H @ dx0.reshape(-1)).reshape(V, 3)
+    rr = float(np.sum(r * r))
+    gd = (-2.0 / rr) * (H @ r.reshape(-1)).reshape(V, 3)
+    return float(np.log(rr)), np.einsum("nkd,nd->nk", C, gd)
 
 
 def adam(w, m, v, g, lr, t, beta1=0.9, beta2=0.999, eps=1e-8):
@@ -129,13 +151,15 @@ def adam(w, m, v, g, lr, t, beta1=0.9, beta2=0.999, eps=1e-8):
 
 
 def ws_features(H, g, x_tilde, x, dx_prev, Hv, inv_mass, *, precond=True, sing_thr=1e-20, attach_k=10.0, dt=0.01,
-                newton_iter=0, iterations=1):
+                newton_iter=0, iterations=1, alpha_prev=0.0):
     """Per-vertex features and normalised candidates of the learned warm start.
 
     H: scipy BSR (3V x 3V) system matrix; g: (V, 3) right-hand side.
     Candidates: c1 = x~ - x, c2 = D^-1 (g - H c1) (one block-Jacobi step),
     c3 = dx_prev; s = |c1| + |c2| + |c3|.  Returns (X (V, 16) FP32 before
-    BF16 rounding, aux (V, 10) = c1/s, c2/s, c3/s, s).
+    BF16 rounding, aux (V, 10) = c1/s, c2/s, c3/s, s).  alpha_prev is the
+    line-search step of the previous Newton iteration (feature 15, zero at
+    the first iteration).
     """
     import scipy.sparse as sp
 
@@ -188,12 +212,14 @@ def ws_features(H, g, x_tilde, x, dx_prev, Hv, inv_mass, *, precond=True, sing_t
     F[:, 12] = np.minimum(np.linalg.norm(r1, axis=1) / (ng + 1e-30), 10.0)
     F[:, 13] = valence / 16.0
     F[:, 14] = 1.0
+    F[:, 15] = alpha_prev if newton_iter > 0 else 0.0
     aux = np.concatenate([c1 / s[:, None], c2 / s[:, None], c3 / s[:, None], s[:, None]], axis=1)
     return F, aux
 
 
 def ws_apply(y, aux):
-    """guess = s (y0 c1 + y1 c2 + y2 c3) from normalised candidates."""
+    """guess = c3 + s (y0 c1 + y1 c2 + y2 c3) from normalised candidates:
+    the previous Newton step plus the network's correction."""
     aux = np.asarray(aux, np.float64)
     y = np.asarray(y, np.float64)
-    return aux[:, 9:10] * (y[:, 0:1] * aux[:, 0:3] + y[:, 1:2] * aux[:, 3:6] + y[:, 2:3] * aux[:, 6:9])
+    return aux[:, 9:10] * ((1.0 + y[:, 2:3]) * aux[:, 6:9] + y[:, 0:1] * aux[:, 0:3] + y[:, 1:2] * aux[:, 3:6])

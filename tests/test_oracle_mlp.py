@@ -139,7 +139,8 @@ def test_features_on_a_two_vertex_system():
     dxp = np.array([[0.0, 0.0, 0.3], [0.4, 0.0, 0.0]])
     Hv = np.zeros((2, 9))
     inv_mass = np.array([1.0, 0.0])
-    F, aux = mlp.ws_features(H, g, xt, x, dxp, Hv, inv_mass, dt=0.1, attach_k=10.0, newton_iter=1, iterations=4)
+    F, aux = mlp.ws_features(H, g, xt, x, dxp, Hv, inv_mass, dt=0.1, attach_k=10.0, newton_iter=1, iterations=4,
+                             alpha_prev=0.25)
     c1 = xt - x
     r = g - (np.block([[A, B], [B.T, C]]) @ c1.reshape(-1)).reshape(2, 3)
     c2 = np.stack([np.linalg.solve(A, r[0]), np.linalg.solve(C, r[1])])
@@ -153,12 +154,14 @@ def test_features_on_a_two_vertex_system():
     np.testing.assert_allclose(F[:, 9], 0.25)
     np.testing.assert_allclose(F[0, 10], np.linalg.norm(B) / np.linalg.norm(A))
     np.testing.assert_allclose(F[:, 13], 1.0 / 16.0)
+    np.testing.assert_allclose(F[:, 15], 0.25)
     # mass term: m / dt^2 relative to |D|_F / sqrt(3); pinned vertex uses 1 + k
     np.testing.assert_allclose(F[0, 7], min(1.0 / 0.01 / (np.linalg.norm(A) / np.sqrt(3)), 10.0))
     np.testing.assert_allclose(F[1, 7], min(11.0 / 0.01 / (np.linalg.norm(C) / np.sqrt(3)), 10.0))
-    # y = (1, 0, 0) reproduces the plain guess x~ - x
-    np.testing.assert_allclose(mlp.ws_apply(np.tile([1.0, 0.0, 0.0], (2, 1)), aux), c1, rtol=1e-12)
-    np.testing.assert_allclose(mlp.ws_apply(np.tile([0.0, 1.0, 0.0], (2, 1)), aux), c2, rtol=1e-12)
+    # y = 0 gives the previous Newton step; y = (1, 0, -1) the plain guess x~ - x
+    np.testing.assert_allclose(mlp.ws_apply(np.zeros((2, 3)), aux), dxp, rtol=1e-12)
+    np.testing.assert_allclose(mlp.ws_apply(np.tile([1.0, 0.0, -1.0], (2, 1)), aux), c1, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(mlp.ws_apply(np.tile([0.0, 1.0, -1.0], (2, 1)), aux), c2, rtol=1e-12, atol=1e-15)
 
 
 def _fd_mismatch(w, X, aux, idx, h=1e-6):
@@ -197,3 +200,43 @@ def test_planted_mistakes_fail_the_gradient_pin(monkeypatch):
 
         mp.setattr(mlp, "forward", forward_true)
         assert _fd_mismatch(w, X, aux, idx) > 1e-4
+
+
+def test_residual_objective_gradient_matches_finite_differences():
+    rng = np.random.default_rng(6)
+    V = 12
+    A = rng.normal(0, 1, (3 * V, 3 * V))
+    H = A @ A.T + 3 * V * np.eye(3 * V)  # SPD like the system matrix
+    cand = rng.normal(0, 1, (V, 9))
+    g = rng.normal(0, 1, (V, 3))
+    y = rng.normal(0, 0.3, (V, 3))
+    L, dy = mlp.residual_loss_and_dy(y, cand, g, H)
+    h = 1e-6
+    for n, k in [(0, 0), (3, 1), (7, 2), (11, 0)]:
+        yp, ym = y.copy(), y.copy()
+        yp[n, k] += h
+        ym[n, k] -= h
+        fd = (mlp.residual_loss_and_dy(yp, cand, g, H)[0] - mlp.residual_loss_and_dy(ym, cand, g, H)[0]) / (2 * h)
+        assert abs(fd - dy[n, k]) <= 1e-6 * max(1.0, abs(fd)), (n, k, fd, dy[n, k])
+    # y = 0 is the previous Newton step c3; the exact solution gives r = 0
+    dx = np.linalg.solve(H, g.reshape(-1)).reshape(V, 3)
+    c = cand.reshape(V, 3, 3).copy()
+    c[:, 2] = dx
+    L0, _ = mlp.residual_loss_and_dy(np.zeros((V, 3)), c.reshape(V, 9), g, H)
+    assert L0 < np.log(1e-20)
+
+
+def test_backward_from_output_gradient_matches_chain_rule():
+    """backward(w, X, dL/dy) equals the gradient of L(y(w)) for a linear L."""
+    rng = np.random.default_rng(7)
+    w = _params(rng)
+    X = rng.normal(0, 1, (6, 16))
+    a = rng.normal(0, 1, (6, 3))  # L = sum a * y
+    g = mlp.backward(w, X, a, quantize=False)
+    h = 1e-6
+    for i in [mlp.O_W1 + 5, mlp.O_B1 + 2, mlp.O_W2 + 77, mlp.O_W3 + 64 * 2 + 1, mlp.O_B3 + 1]:
+        wp, wm = w.copy(), w.copy()
+        wp[i] += h
+        wm[i] -= h
+        fd = (np.sum(a * mlp.forward(wp, X, False)[0]) - np.sum(a * mlp.forward(wm, X, False)[0])) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])

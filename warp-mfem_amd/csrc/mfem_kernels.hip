@@ -481,11 +481,14 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
     wait_vmem();
     pub_store(&gs.flags[blockIdx.x], ep);
   }
+  // wait until every workgroup has arrived.  A workgroup that has passed this
+  // barrier may already have stored ep + 1 at the next one before this wave
+  // polls again, so the test is "flag >= ep" (wrap-safe), never "flag == ep".
   bool to = false;
   for (int b = threadIdx.x; b < nb && !to; b += kBlock) {
-    if (pub_load(&gs.flags[b]) == ep) continue;
+    if ((int32_t)(pub_load(&gs.flags[b]) - ep) >= 0) continue;
     const uint64_t t0 = wall_clock64();
-    while (pub_load(&gs.flags[b]) != ep) {
+    while ((int32_t)(pub_load(&gs.flags[b]) - ep) < 0) {
       if (wall_clock64() - t0 > kBarrierTimeout || pub_load(&gs.status[MFEM_STATUS_ERRBITS]) != 0) {
         to = true;
         break;
@@ -650,6 +653,9 @@ template <int CPW, int RW, int MODE>
 __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) {
   constexpr bool PIPE = MODE != 0;
   constexpr bool PUB32 = MODE == 2;  // pipelined: publish the SpMV input in FP32
+  // a barrier timed out earlier: flags and epoch are inconsistent until the
+  // host clears status and flags, so do nothing
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;
   GridSync gs{S.red, S.flags, S.status, (uint32_t)S.status[MFEM_STATUS_EPOCH], false};
   const int lane = threadIdx.x & 63;
   const int gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
@@ -1168,7 +1174,10 @@ __global__ void __launch_bounds__(kBlock) k_ws_features(mfem_solver S, float dt,
   f[12] = fminf(nrm(r1) / (ng + 1e-30f), 10.f);
   f[13] = (float)valence / 16.f;
   f[14] = 1.f;
-  f[15] = 0.f;
+  // step length the line search accepted in the previous Newton iteration
+  f[15] = (newton_iter > 0 && newton_iter <= MFEM_MAX_STATS_ITERS)
+              ? (float)S.stats[(newton_iter - 1) * MFEM_STATS_PER_ITER + 4]
+              : 0.f;
 #pragma unroll
   for (int k = 0; k < 16; ++k) S.ws_x[16 * i + k] = f2bf16(f[k]);
   const float is = 1.f / sc;
@@ -1187,7 +1196,8 @@ __global__ void __launch_bounds__(kBlock) k_ws_apply(mfem_solver S) {
   const float* a = S.ws_aux + 10 * i;
   const float* y = S.ws_y + 3 * i;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) S.guess[3 * i + d] = a[9] * (y[0] * a[d] + y[1] * a[3 + d] + y[2] * a[6 + d]);
+  for (int d = 0; d < 3; ++d)
+    S.guess[3 * i + d] = a[9] * ((1.f + y[2]) * a[6 + d] + y[0] * a[d] + y[1] * a[3 + d]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1325,6 +1335,7 @@ __device__ double tet_slope(const mfem_solver& S, int e) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, int newton_iter) {
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // see k_pcg
   GridSync gs{S.red, S.flags, S.status, (uint32_t)S.status[MFEM_STATUS_EPOCH], false};
   const int tid = blockIdx.x * kBlock + threadIdx.x;
   const int nth = gridDim.x * kBlock;

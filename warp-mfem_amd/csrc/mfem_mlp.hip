@@ -193,11 +193,13 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const uint16_t* X, int N, c
 // Training: loss and gradients for samples [0, N).  Each sample n has
 // aux[n][0..8] = candidates c1, c2, c3 (normalised) and aux[n][9..11] =
 // target t (normalised); prediction p = y0 c1 + y1 c2 + y2 c3 and loss
-// (1/N) sum |p - t|^2.  One wave per workgroup (the FP32 pre-activations of a
+// (1/N) sum |p - t|^2.  With dy_ext the output gradient is given instead
+// (aux unused, partial loss 0): the backward pass of any loss on y.  One wave per workgroup (the FP32 pre-activations of a
 // 64-sample tile fill most of the LDS); per-workgroup partial gradients go to
 // part[blockIdx.x][kNParams] and the partial loss to part_loss[blockIdx.x].
-__global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t* X, const float* aux, int N, const uint16_t* wbf,
-                                                 const float* wf32, float* part, float* part_loss) {
+__global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t* X, const float* aux, const float* dy_ext, int N,
+                                                 const uint16_t* wbf, const float* wf32, float* part,
+                                                 float* part_loss) {
   __shared__ WeightLds W;
   __shared__ TrainLds T;
   __shared__ float gacc[kNParams];
@@ -219,9 +221,13 @@ __global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t* X, const float*
     for (int c = 0; c < kIn; ++c) L.x[lane][c] = ok ? X[(size_t)n * kIn + c] : (uint16_t)0;
     __builtin_amdgcn_wave_barrier();
     forward_tile(W, L, lane, T.z1, T.z2);
-    // output gradient dL/dy_k = (2/N) (p - t) . c_k
+    // output gradient: given (dy_ext[n][0..2]) or, for the blend loss,
+    // dL/dy_k = (2/N) (p - t) . c_k
     float dy[3] = {0.f, 0.f, 0.f};
-    if (ok) {
+    if (ok && dy_ext) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) dy[k] = dy_ext[(size_t)n * 3 + k];
+    } else if (ok) {
       const float* a = aux + (size_t)n * 12;
       float e[3];
 #pragma unroll
@@ -314,7 +320,7 @@ __global__ void __launch_bounds__(64) k_mlp_grad(const uint16_t* X, const float*
   for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
   __syncthreads();
   for (int i = threadIdx.x; i < kNParams; i += blockDim.x) part[(size_t)blockIdx.x * kNParams + i] = gacc[i];
-  if (threadIdx.x == 0) part_loss[blockIdx.x] = loss;
+  if (threadIdx.x == 0 && part_loss) part_loss[blockIdx.x] = loss;
 }
 
 // Sum the per-workgroup gradients in a fixed order, Adam update of the FP32
@@ -384,10 +390,33 @@ int mfem_mlp_train_step(const uint16_t* X, const float* aux, int n, float* w_f32
   hipStream_t st = (hipStream_t)stream;
   int nb = (n + kTile - 1) / kTile;
   if (nb > max_parts) nb = max_parts;
-  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(64), 0, st, X, aux, n, w_bf16, w_f32, part, part_loss);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(64), 0, st, X, aux, (const float*)nullptr, n, w_bf16, w_f32, part,
+                     part_loss);
   hipLaunchKernelGGL(k_mlp_adam, dim3((kNParams + 255) / 256), dim3(256), 0, st, part, nb, w_f32, w_bf16, adam_state,
                      lr, 0.9f, 0.999f, 1e-8f, step, grad_out);
   if (loss_out) hipLaunchKernelGGL(k_mlp_sum_loss, dim3(1), dim3(64), 0, st, part_loss, nb, loss_out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    g_mlp_err = hipGetErrorString(e);
+    return MFEM_EHIP;
+  }
+  return MFEM_OK;
+}
+
+int mfem_mlp_backward_step(const uint16_t* X, const float* dy, int n, float* w_f32, uint16_t* w_bf16,
+                           float* adam_state, float* part, int max_parts, float* grad_out, float lr, int step,
+                           void* stream) {
+  if (n <= 0 || !X || !dy || !w_f32 || !w_bf16 || !adam_state || !part || max_parts <= 0) {
+    g_mlp_err = "bad argument";
+    return MFEM_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  int nb = (n + kTile - 1) / kTile;
+  if (nb > max_parts) nb = max_parts;
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(64), 0, st, X, (const float*)nullptr, dy, n, w_bf16, w_f32, part,
+                     (float*)nullptr);
+  hipLaunchKernelGGL(k_mlp_adam, dim3((kNParams + 255) / 256), dim3(256), 0, st, part, nb, w_f32, w_bf16, adam_state,
+                     lr, 0.9f, 0.999f, 1e-8f, step, grad_out);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     g_mlp_err = hipGetErrorString(e);

@@ -48,7 +48,7 @@ ENTRY_POINTS = [
     "mfem_init_stretch",
     "mfem_begin_step", "mfem_update_system", "mfem_assemble", "mfem_global_solve", "mfem_local_solve",
     "mfem_line_search", "mfem_end_step", "mfem_step", "mfem_spmv", "mfem_merit", "mfem_ws_features",
-    "mfem_ws_apply", "mfem_mlp_forward", "mfem_mlp_train_step", "mfem_mlp_last_error",
+    "mfem_ws_apply", "mfem_mlp_forward", "mfem_mlp_train_step", "mfem_mlp_backward_step", "mfem_mlp_last_error",
 ]
 
 
@@ -87,6 +87,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mfem_mlp_forward.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
     lib.mfem_mlp_train_step.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp, vp, vp,
                                         ctypes.c_float, ctypes.c_int, vp]
+    lib.mfem_mlp_backward_step.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp, ctypes.c_float,
+                                           ctypes.c_int, vp]
     lib.mfem_wall_clock_khz.restype = ctypes.c_int
     for name in ENTRY_POINTS[5:]:
         getattr(lib, name).restype = ctypes.c_int

@@ -4,10 +4,12 @@ The reference starts every CG solve from x~ - x (src/mfem/refinement/
 kernels.py ``compute_initial_guess``, called at solver.py:530-531) and has no
 learned component.  Here a small per-vertex network (16 -> 64 -> 64 -> 3,
 GELU, BF16 weights, FP32 accumulation on MFMA; csrc/mfem_mlp.hip) blends
-three candidate guesses per vertex,
+three candidate guesses per vertex into a correction of the previous Newton
+step,
 
-    dx0_i = s_i (y0 c1_i + y1 c2_i + y2 c3_i),
+    dx0_i = c3_i + s_i (y0 c1_i + y1 c2_i + y2 c3_i) / s_i,
     c1 = x~ - x,  c2 = D^-1 (g - H c1),  c3 = previous Newton step,
+    s_i = |c1_i| + |c2_i| + |c3_i|,
 
 from 16 rotation-invariant features (candidate norms and angles, diagonal /
 off-diagonal stiffness ratios, mass term, pinning, Newton iteration).  The
@@ -50,17 +52,14 @@ def _mlp_call(lib, name, *args):
 
 
 def init_params(seed: int = 0) -> np.ndarray:
-    """Initial FP32 parameters: scaled normal hidden layers, a small output
-    layer and output bias (1, 0, 0), so the untrained network already
-    returns (about) the plain guess x~ - x."""
+    """Initial FP32 parameters: scaled normal hidden layers and a zero output
+    layer, so the untrained network returns exactly the previous Newton step
+    (a random output layer adds per-vertex noise that the CG residual
+    amplifies)."""
     rng = np.random.default_rng(seed)
     w = np.zeros(N_PARAMS, np.float32)
     w[O_W1:O_B1] = rng.normal(0, 1.0 / np.sqrt(N_IN), N_IN * N_HID)
     w[O_W2:O_B2] = rng.normal(0, 1.0 / np.sqrt(N_HID), N_HID * N_HID)
-    W3 = np.zeros((N_HID, N_OUT), np.float32)
-    W3[:, :3] = rng.normal(0, 1e-2, (N_HID, 3))
-    w[O_W3:O_B3] = W3.ravel()
-    w[O_B3] = 1.0
     return w
 
 
@@ -126,6 +125,67 @@ class WarmStartNet:
                   _ptr(self.wbf), _ptr(self.adam), _ptr(self._part), ctypes.c_int(MAX_PARTS), _ptr(self._part_loss),
                   _ptr(loss_out), _ptr(grad_out), ctypes.c_float(lr), ctypes.c_int(max(self.t, 1)), self._stream())
 
+    def backward_step(self, X, dy: torch.Tensor, lr: float, grad_out: torch.Tensor | None = None) -> None:
+        """One Adam step from a given output gradient dy = dL/dy (n, 3);
+        lr = 0 computes the gradient only (into grad_out)."""
+        X = self._features(X)
+        dy = dy.to(device=self.device, dtype=torch.float32).contiguous()
+        if dy.shape != (X.shape[0], 3):
+            raise ValueError("dy must be (n, 3)")
+        self._scratch()
+        if lr > 0:
+            self.t += 1
+        _mlp_call(self.lib, "mfem_mlp_backward_step", _ptr(X), _ptr(dy), ctypes.c_int(X.shape[0]), _ptr(self.w),
+                  _ptr(self.wbf), _ptr(self.adam), _ptr(self._part), ctypes.c_int(MAX_PARTS), _ptr(grad_out),
+                  ctypes.c_float(lr), ctypes.c_int(max(self.t, 1)), self._stream())
+
+    def residual_objective(self, solves: list, solver) -> float:
+        """Mean over solves of log |g - H dx0|^2 for the current weights."""
+        tot = 0.0
+        for sv in solves:
+            C = sv["cand"].view(-1, 3, 3)
+            dx0 = C[:, 2] + torch.einsum("nk,nkd->nd", self.forward(sv["X"]), C)
+            r = (sv["g"] - solver.spmv_with(sv["values"], dx0)).double()
+            tot += float(torch.log(torch.sum(r * r)).item())
+        return tot / len(solves)
+
+    def fit_residual(self, solves: list, solver, epochs: int = 20, lr: float = 1e-4, seed: int = 0) -> dict:
+        """Trains on the solver's own objective: for each logged solve, the
+        log of the squared initial residual |g - H dx0|^2 of the corrected
+        guess dx0 = c3 + y0 c1 + y1 c2 + y2 c3, with H that solve's matrix
+        (applied by the project's SpMV) and dL/ddx0 = -2 H r / |r|^2.  One
+        Adam step per solve.  After every epoch the objective is evaluated
+        with fixed weights and the best weights are kept, the starting point
+        (epoch 0) included, so training never returns a worse guess than it
+        started from.  Returns the per-epoch objective, the objective of the
+        previous Newton step alone (y = 0) and the epoch kept."""
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        hist = {"loss": [self.residual_objective(solves, solver)], "base": None, "best_epoch": 0}
+        base = 0.0
+        for sv in solves:
+            r = (sv["g"] - solver.spmv_with(sv["values"], sv["cand"][:, 6:9])).double()
+            base += float(torch.log(torch.sum(r * r)).item())
+        hist["base"] = base / len(solves)
+        best_w = self.w.clone()
+        for ep in range(epochs):
+            for i in torch.randperm(len(solves), generator=g).tolist():
+                sv = solves[i]
+                C = sv["cand"].view(-1, 3, 3)
+                y = self.forward(sv["X"])
+                dx0 = C[:, 2] + torch.einsum("nk,nkd->nd", y, C)
+                r = sv["g"] - solver.spmv_with(sv["values"], dx0)
+                rr = torch.sum(r.double() * r.double())
+                gd = (-2.0 / rr).float() * solver.spmv_with(sv["values"], r)
+                dy = torch.einsum("nkd,nd->nk", C, gd)
+                self.backward_step(sv["X"], dy, lr)
+            hist["loss"].append(self.residual_objective(solves, solver))
+            if hist["loss"][-1] < hist["loss"][hist["best_epoch"]]:
+                hist["best_epoch"] = ep + 1
+                best_w.copy_(self.w)
+        self.w.copy_(best_w)
+        self.refresh()
+        return hist
+
     def loss(self, X, aux) -> float:
         out = torch.zeros(1, dtype=torch.float32, device=self.device)
         self.train_step(X, aux, 0.0, loss_out=out)
@@ -170,14 +230,19 @@ class WarmStartNet:
         return cls(device, weights=np.load(path, allow_pickle=False))
 
 
-def collect(solver, state, dt: float, steps: int, every: int = 1, max_target: float = 8.0):
+def collect(solver, state, dt: float, steps: int, every: int = 1):
     """Runs ``steps`` plain-PCG steps stage by stage on ``state`` (advanced in
     place) and logs, per Newton iteration, the warm-start features, the
-    normalised candidates and the PCG solution as training pairs.
+    candidates and the PCG solution as training pairs.
 
-    Returns (X (n, 16) int16 BF16 bits, aux (n, 12) float32) on the device:
-    aux = (c1, c2, c3, dx / s) with s the candidates' scale; vertices with no
-    scale and targets beyond ``max_target`` (in units of s) are left out.
+    Returns (X (n, 16) int16 BF16 bits, aux (n, 12) float32) on the device,
+    aux = (c1, c2, c3, dx - c3) / rms(dx) with rms(dx) the root-mean-square
+    solution size of that solve: the loss |sum_k y_k c_k - (dx - c3)|^2 is
+    the squared error of the corrected guess relative to the solve's
+    solution, so every solve weighs the same (each solve's CG iteration
+    count depends on its own relative residual).  Vertices with no scale (at
+    rest, all candidates zero) are left out.
+    Raises if a device error (grid-barrier timeout) occurred.
     """
     Xs, As = [], []
     out = state.clone()
@@ -193,13 +258,45 @@ def collect(solver, state, dt: float, steps: int, every: int = 1, max_target: fl
             solver.global_solve(it, learned=False)
             if k % every == 0:
                 s = b["ws_aux"][:, 9:10]
-                t = b["dx"].float() / s
-                # drop vertices at rest (no scale) and outliers far from every candidate
-                keep = (s[:, 0] > 1e-12 * float(s.max())) & (t.abs().amax(dim=1) < max_target)
+                keep = s[:, 0] > 1e-12 * s.max()
+                dx = b["dx"][keep].float()
+                rms = torch.sqrt(torch.mean(torch.sum(dx * dx, dim=1))).clamp_min(1e-30)
+                cand = b["ws_aux"][keep, :9] * s[keep]
                 Xs.append(b["ws_x"][keep].clone())
-                As.append(torch.cat([b["ws_aux"][keep, :9], t[keep]], dim=1))
+                As.append(torch.cat([cand / rms, (dx - cand[:, 6:9]) / rms], dim=1))
             solver.local_solve()
             solver.line_search(dt, it)
         solver.end_step(dt)
         state.assign(out)
+    if solver.status() != 0:
+        raise RuntimeError("device error while logging training pairs (grid barrier timed out)")
     return torch.cat(Xs), torch.cat(As)
+
+
+def collect_solves(solver, state, dt: float, steps: int) -> list:
+    """Like ``collect`` but keeps whole solves for the residual objective:
+    per Newton iteration the features X (V, 16), the absolute candidates
+    cand (V, 9) = (c1, c2, c3), the right-hand side g (V, 3) and a copy of
+    the assembled matrix values (SELL layout, shared pattern)."""
+    solves = []
+    out = state.clone()
+    b = solver.buffers
+    for k in range(steps):
+        out.assign(state)
+        solver.bind(out)
+        solver.begin_step(dt)
+        for it in range(solver.iterations):
+            solver.update_system(dt)
+            solver.assemble(dt)
+            solver.ws_features(dt, it)
+            s = b["ws_aux"][:, 9:10]
+            solves.append({"X": b["ws_x"].clone(), "cand": (b["ws_aux"][:, :9] * s).contiguous(),
+                           "g": b["g"].clone(), "values": b["sell_val"].clone()})
+            solver.global_solve(it, learned=False)
+            solver.local_solve()
+            solver.line_search(dt, it)
+        solver.end_step(dt)
+        state.assign(out)
+    if solver.status() != 0:
+        raise RuntimeError("device error while logging solves (grid barrier timed out)")
+    return solves

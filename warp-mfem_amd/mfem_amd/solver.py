@@ -366,6 +366,20 @@ class RefinementSolver:
         self._call("mfem_spmv", ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), self._stream())
         return y
 
+    def spmv_with(self, values: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+        """y = H x for stored matrix values in this solver's SELL layout (a
+        copy of ``buffers['sell_val']`` taken at another Newton iteration)."""
+        if values.shape != self.buffers["sell_val"].shape or values.dtype != torch.float32 or \
+                values.device != self.device:
+            raise ValueError("values must match buffers['sell_val'] in shape, dtype and device")
+        x = x.to(device=self.device, dtype=torch.float32).contiguous()
+        y = torch.empty_like(x)
+        S2 = type(self._S).from_buffer_copy(self._S)
+        S2.sell_val = values.data_ptr()
+        _ffi.call("mfem_spmv", ctypes.byref(S2), ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                  self._stream())
+        return y
+
     def merit(self, dt: float, alpha: float = 0.0) -> float:
         out = torch.zeros(1, dtype=torch.float64, device=self.device)
         self._call("mfem_merit", ctypes.c_float(dt), ctypes.c_float(alpha), ctypes.c_void_p(out.data_ptr()),
@@ -382,8 +396,17 @@ class RefinementSolver:
         return st[: self.iterations].copy()
 
     def status(self) -> int:
+        """Device error bits (bit 0: a grid barrier timed out; the persistent
+        kernels then do nothing until ``reset_status``)."""
         torch.cuda.synchronize(self.device)
         return int(self.buffers["status"][0].item())
+
+    def reset_status(self) -> None:
+        """Clears the error bits, the barrier epoch and the arrival flags."""
+        torch.cuda.synchronize(self.device)
+        self.buffers["status"].zero_()
+        self.buffers["flags"].zero_()
+        torch.cuda.synchronize(self.device)
 
     def system_csr(self):
         """Assembled H as a scipy.sparse.bsr_matrix (host copy)."""
