@@ -57,6 +57,11 @@ def main():
     ap.add_argument("--train-steps", type=int, default=6, help="plain steps logged as training solves")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--no-learned", action="store_true", help="time plain PCG only")
+    ap.add_argument("--dump-iters", default=None,
+                    help="write the CG iteration count of every PCG launch, in launch order, to this JSON file "
+                         "(to divide per-dispatch hardware counters by iterations)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run five 1 GiB device-to-device copies (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -65,6 +70,16 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": src.numel() * 4, "copy_probe_bytes_written": dst.numel() * 4,
+                          "copies": 5}), flush=True)
+        return
 
     from mfem_amd import scene
     from mfem_amd.solver import RefinementSolver, model_state
@@ -82,9 +97,12 @@ def main():
     state = model_state(m, "cuda:0")
     out = state.clone()
     print(f"[bench] {wl}: {m.tet_count} tets, {m.particle_count} verts, warmup {args.warmup}", flush=True)
+    launch_iters = []  # CG iterations per PCG launch, in launch order (with --dump-iters)
     for k in range(args.warmup):
         solver.step(state, out, None, None, dt)
         state, out = out, state
+        if args.dump_iters:
+            launch_iters += solver.last_stats()[:, 0].astype(int).tolist()
     torch.cuda.synchronize()
     if solver.status() != 0:
         raise SystemExit("device error during warmup")
@@ -138,6 +156,11 @@ def main():
 
     s0 = state.clone()
     wall_p, st_p, timers_p, end_p = timed_leg(s0, None)
+    if args.dump_iters:
+        launch_iters += st_p[:, :, 0].astype(int).reshape(-1).tolist()
+        with open(args.dump_iters, "w") as f:
+            json.dump({"kernel": "k_pcg", "iters_per_launch": launch_iters, "tets": m.tet_count,
+                       "verts": m.particle_count, "nnz_blocks": int(len(solver.pattern.cols))}, f)
     print(f"[bench] plain: {args.steps} steps in {wall_p:.3f} s", flush=True)
     if learned:
         wall, st, timers, end_l = timed_leg(s0, net)
@@ -165,7 +188,15 @@ def main():
     iters_flat = cg_iters.reshape(-1)
     active = iters_flat > 0
     achieved = float(np.sum(iters_flat[active] * bytes_per_iter) / np.sum(pcg_ms[active] * 1e-3) / 1e9) if active.any() else 0.0
+    # HBM bytes per launch: the per-CG-iteration FETCH_SIZE / WRITE_SIZE of
+    # k_pcg measured by rocprofv3 on this workload (calibrated against a
+    # 1 GiB device copy) times this run's mean iterations per launch
     traffic = None
+    traffic_file = os.path.join(ROOT, "profiles", "hbm_traffic_octopus_204k.json")
+    if args.workload == "octopus" and os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            per_iter = json.load(f)["k_pcg"]["hbm_bytes_per_cg_iter"]
+        traffic = float(per_iter * np.mean(iters_flat[active])) if active.any() else None
 
     value = T * args.steps / wall
     value_plain = T * args.steps / wall_p
@@ -223,6 +254,9 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "bytes_per_cg_iter": bytes_per_iter,
+            "traffic_note": "traffic = measured HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
+                            "profiles/hbm_traffic_octopus_204k.json); below the algorithmic bytes because the "
+                            "matrix rows stay in VGPRs for the whole solve and the vectors in L2 / Infinity Cache",
             "us_per_cg_iter": float(np.sum(pcg_ms[active]) * 1e3 / max(1.0, np.sum(iters_flat[active]))),
         },
     }

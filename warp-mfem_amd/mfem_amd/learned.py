@@ -51,6 +51,11 @@ def _mlp_call(lib, name, *args):
         raise RuntimeError(f"{name} failed ({rc}): {lib.mfem_mlp_last_error().decode()}")
 
 
+def _blend(C: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
+    """dx0 = c3 + y0 c1 + y1 c2 + y2 c3 for candidates C (n, 3, 3)."""
+    return C[:, 2] + y[:, 0:1] * C[:, 0] + y[:, 1:2] * C[:, 1] + y[:, 2:3] * C[:, 2]
+
+
 def init_params(seed: int = 0) -> np.ndarray:
     """Initial FP32 parameters: scaled normal hidden layers and a zero output
     layer, so the untrained network returns exactly the previous Newton step
@@ -144,7 +149,7 @@ class WarmStartNet:
         tot = 0.0
         for sv in solves:
             C = sv["cand"].view(-1, 3, 3)
-            dx0 = C[:, 2] + torch.einsum("nk,nkd->nd", self.forward(sv["X"]), C)
+            dx0 = _blend(C, self.forward(sv["X"]))
             r = (sv["g"] - solver.spmv_with(sv["values"], dx0)).double()
             tot += float(torch.log(torch.sum(r * r)).item())
         return tot / len(solves)
@@ -172,11 +177,11 @@ class WarmStartNet:
                 sv = solves[i]
                 C = sv["cand"].view(-1, 3, 3)
                 y = self.forward(sv["X"])
-                dx0 = C[:, 2] + torch.einsum("nk,nkd->nd", y, C)
+                dx0 = _blend(C, y)
                 r = sv["g"] - solver.spmv_with(sv["values"], dx0)
                 rr = torch.sum(r.double() * r.double())
                 gd = (-2.0 / rr).float() * solver.spmv_with(sv["values"], r)
-                dy = torch.einsum("nkd,nd->nk", C, gd)
+                dy = (C * gd[:, None, :]).sum(dim=2)
                 self.backward_step(sv["X"], dy, lr)
             hist["loss"].append(self.residual_objective(solves, solver))
             if hist["loss"][-1] < hist["loss"][hist["best_epoch"]]:
