@@ -162,6 +162,15 @@ typedef struct mfem_solver {
   float* ws_aux;           /* [n_verts][10] normalised candidates c1, c2, c3 and their scale */
   float* ws_y;             /* [n_verts][3] network outputs (blend weights) */
   float* guess;            /* [n_verts][3] learned initial guess */
+  /* triangle-vs-capsule contact (used when tri_contact != 0) */
+  const int32_t* tcon_ptr; /* [n_slots*64 + 1] per lane-slot list of triangle vertex pairs */
+  const int32_t* tcon_src; /* tri * 9 + 3 * a + b: block (tris[tri][a], tris[tri][b]) */
+  const int32_t* vtri_ptr; /* [n_verts + 1] vertex -> incident triangle corners */
+  const int32_t* vtri_src; /* tri * 3 + corner */
+  float* tri_dist;         /* [n_tris] distance to the nearest capsule minus its radius */
+  float* tri_bary;         /* [n_tris][3] barycentric weights of the closest point */
+  float* tri_nrm;          /* [n_tris][3] contact normal (capsule -> triangle) */
+  int32_t* tri_sid;        /* [n_tris] shape id, -1 when no capsule */
 } mfem_solver;
 
 /* ABI self-description (checked against the Python mirror by the tests) */

@@ -28,8 +28,9 @@ def _solver_kwargs(model):
 def _make(model, device, **extra):
     from mfem_amd.solver import RefinementSolver, model_state
 
-    solver = RefinementSolver(model, model.params["iterations"], max_tets=model.tet_count, device=device,
-                              **_solver_kwargs(model), **extra)
+    kw = _solver_kwargs(model)
+    kw.update(extra)  # explicit arguments override the scene's parameters
+    solver = RefinementSolver(model, model.params["iterations"], max_tets=model.tet_count, device=device, **kw)
     return solver, model_state(model, device)
 
 
@@ -61,6 +62,9 @@ def _perturbed_scene(name):
     elif name == "beam_nh":
         m = scene.beam_scene(energy="neohookean")
         dt = 1.0 / 60.0
+    elif name == "capsule_poke":
+        m = scene.capsule_poke_scene()
+        dt = 0.01
     else:
         m = scene.drop_box_scene()
         dt = 0.01
@@ -74,7 +78,7 @@ def _sync_oracle(o, solver, state):
     o.buf("lam")[:] = solver.buffers["lam"].double().cpu().numpy().ravel()
 
 
-@pytest.mark.parametrize("name", ["beam_arap", "beam_nh", "drop_box"])
+@pytest.mark.parametrize("name", ["beam_arap", "beam_nh", "drop_box", "capsule_poke"])
 def test_newton_stages_match_oracle(gpu, name):
     m, dt = _perturbed_scene(name)
     solver, state = _make(m, gpu)
@@ -162,7 +166,7 @@ def test_newton_stages_match_oracle(gpu, name):
     assert abs(f_gpu - f_orc) <= 1e-4 * max(abs(f_orc), 1e-12) + 1e-7
 
 
-@pytest.mark.parametrize("name,steps", [("beam_arap", 5), ("beam_nh", 5), ("drop_box", 8)])
+@pytest.mark.parametrize("name,steps", [("beam_arap", 5), ("beam_nh", 5), ("drop_box", 8), ("capsule_poke", 5)])
 def test_full_steps_match_fp64_oracle(gpu, name, steps):
     m, dt = _perturbed_scene(name)
     solver, state = _make(m, gpu)
@@ -233,3 +237,44 @@ def test_octopus_full_size_step(gpu):
     assert x[:, 2].min() > -1e-3  # barrier keeps the body on the ground
     # total mass-weighted displacement is downward (falling)
     assert (m.particle_mass * (x[:, 2] - m.particle_q[:, 2])).sum() < 0
+
+
+def test_moving_capsule_triangle_contact_matches_oracle(gpu):
+    """A capsule driven down into a pinned slab through body_q: triangle and
+    vertex barriers against the capsule, friction, per-step poses; positions
+    vs the FP64 oracle, and the triangle term is really active."""
+    m = scene.capsule_poke_scene()
+    dt = 0.01
+    solver, state = _make(m, gpu)
+    o = Oracle(m, 64)
+    out = state.clone()
+    sh = m.shapes[0]
+    pose = np.concatenate([sh.pos, sh.quat]).astype(np.float64)
+    min_tri = []
+    for k in range(6):
+        pose[2] -= 0.0015  # 1.5 mm per step into the slab's top face
+        state.body_q = torch.as_tensor(pose[None, :], dtype=torch.float32, device=gpu)
+        o.buf("shape_xf")[:] = pose
+        solver.step(state, out, None, None, dt)
+        state, out = out, state
+        o.step(dt)
+        st = solver.last_stats()
+        assert np.all(st[:, 1] <= 1e-6), st
+        assert solver.status() == 0
+        min_tri.append(float(solver.buffers["tri_dist"].min().item()))
+    print("min triangle distance per step", min_tri)
+    assert min(min_tri) < m.params["contact_d1"]
+    x = state.particle_q.double().cpu().numpy()
+    _close(x, o.x, 1e-4, "positions with a moving capsule")
+    # the same run without triangle contact ends elsewhere
+    solver2, state2 = _make(m, gpu, tri_contact=False)
+    out2 = state2.clone()
+    pose = np.concatenate([sh.pos, sh.quat]).astype(np.float64)
+    for k in range(6):
+        pose[2] -= 0.0015
+        state2.body_q = torch.as_tensor(pose[None, :], dtype=torch.float32, device=gpu)
+        solver2.step(state2, out2, None, None, dt)
+        state2, out2 = out2, state2
+    x2 = state2.particle_q.double().cpu().numpy()
+    print("triangle term changes positions by", np.abs(x2 - x).max())
+    assert np.abs(x2 - x).max() > 1e-5

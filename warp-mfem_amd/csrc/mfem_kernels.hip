@@ -134,9 +134,97 @@ __device__ float friction_eval(const mfem_solver& S, int i, const float* x, floa
   return sc * friction_f0(y, ev);
 }
 
+// distance of triangle abc to the nearest capsule (minus its radius), the
+// barycentric weights of the closest triangle point and the contact normal
+// (capsule -> triangle; the face normal oriented away from the axis midpoint
+// when the axis touches or pierces the face) (contact.py:553-681)
+__device__ float tri_capsule(const mfem_solver& S, const float* a, const float* b, const float* c, float* bary,
+                             float* nrm, int* sid) {
+  float dmin = 1.0e8f;
+  bary[0] = bary[1] = bary[2] = 1.f / 3.f;
+  nrm[0] = nrm[1] = nrm[2] = 0.f;
+  *sid = -1;
+  for (int k = 0; k < S.n_shapes; ++k) {
+    if (S.shape_type[k] != MFEM_SHAPE_CAPSULE) continue;
+    const float* xf = S.shape_xf + 7 * k;
+    float R[9];
+    quat_to_mat(xf + 3, R);
+    const float rad = S.shape_scale[3 * k], hh = S.shape_scale[3 * k + 1];
+    float p0[3], p1[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      p0[i] = xf[i] - R[i * 3 + 2] * hh;
+      p1[i] = xf[i] + R[i * 3 + 2] * hh;
+    }
+    float w[3], q[3];
+    bool hit;
+    const float dist = closest_point_triangle_segment(a, b, c, p0, p1, w, q, &hit);
+    const float d = dist - rad;
+    if (d < dmin) {
+      float p[3], n[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) p[i] = w[0] * a[i] + w[1] * b[i] + w[2] * c[i];
+      if (!hit && dist > 1e-8f) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) n[i] = (p[i] - q[i]) / dist;
+      } else {
+        const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        cross3f(e1, e2, n);
+        const float nl = sqrtf(dot3f(n, n));
+        if (nl > 1e-20f) {
+          n[0] /= nl; n[1] /= nl; n[2] /= nl;
+        } else {
+          n[0] = 0.f; n[1] = 0.f; n[2] = 1.f;
+        }
+        const float ax[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        const float pp0[3] = {p[0] - p0[0], p[1] - p0[1], p[2] - p0[2]};
+        const float aa = dot3f(ax, ax);
+        const float t = dot3f(pp0, ax) / (aa > 1e-20f ? aa : 1e-20f);
+        const float sgn = t < 0.5f ? -1.f : 1.f;
+        if (sgn * dot3f(n, ax) < 0.f) {
+          n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2];
+        }
+      }
+      dmin = d;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        bary[i] = w[i];
+        nrm[i] = n[i];
+      }
+      *sid = k;
+    }
+  }
+  return dmin;
+}
+
+__device__ __forceinline__ bool tri_contact_on(const mfem_solver& S) {
+  return S.tri_contact && S.n_tris > 0 && S.n_shapes > 0;
+}
+
 // ---------------------------------------------------------------------------
 // per-vertex kernels
 // ---------------------------------------------------------------------------
+
+// per surface triangle: nearest capsule, closest point, normal
+__global__ void __launch_bounds__(kBlock) k_tri_contact(mfem_solver S) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= S.n_tris) return;
+  const int i0 = S.tris[3 * t], i1 = S.tris[3 * t + 1], i2 = S.tris[3 * t + 2];
+  float a[3], b[3], c[3], w[3], n[3];
+  load3(S.x, i0, a);
+  load3(S.x, i1, b);
+  load3(S.x, i2, c);
+  int sid;
+  const float d = tri_capsule(S, a, b, c, w, n, &sid);
+  S.tri_dist[t] = d;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    S.tri_bary[3 * t + k] = w[k];
+    S.tri_nrm[3 * t + k] = n[k];
+  }
+  S.tri_sid[t] = sid;
+}
 __global__ void __launch_bounds__(kBlock) k_begin_step(mfem_solver S, float dt) {
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= S.n_verts) return;
@@ -196,6 +284,21 @@ __global__ void __launch_bounds__(kBlock) k_vertex(mfem_solver S, float dt) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) gb[k] = S.kappa * de * grad[k];
     h2 = S.kappa * fmaxf(d2e, 0.f);
+  }
+  if (tri_contact_on(S)) {
+    // triangle-capsule barrier energy and gradient of the incident
+    // triangles, in incidence order (contact.py:684-775)
+    for (int kk = S.vtri_ptr[i]; kk < S.vtri_ptr[i + 1]; ++kk) {
+      const int src = S.vtri_src[kk], t = src / 3, a = src - 3 * t;
+      const float dt_ = S.tri_dist[t];
+      if (!(dt_ < S.d1)) continue;
+      float e, de, d2e;
+      barrier(dt_, S.d0, S.d1, S.qb, e, de, d2e);
+      const float wa = S.tri_bary[3 * t + a];
+      eb += wa * S.kappa * e;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gb[k] += wa * S.kappa * de * S.tri_nrm[3 * t + k];
+    }
   }
   float gd[3], hd[9];
   float ed = friction_eval(S, i, x, dt, gd, hd);
@@ -375,6 +478,22 @@ __global__ void __launch_bounds__(kBlock) k_assemble(mfem_solver S) {
     } else {
 #pragma unroll
       for (int q = 0; q < 9; ++q) acc[q] += b[q];
+    }
+  }
+  if (tri_contact_on(S)) {
+    // w_a w_b kappa max(b'', 0) n n^T of each triangle on this block
+    for (int k = S.tcon_ptr[l]; k < S.tcon_ptr[l + 1]; ++k) {
+      const int src = S.tcon_src[k], t = src / 9, a = (src / 3) % 3, bb = src % 3;
+      const float d = S.tri_dist[t];
+      if (!(d < S.d1)) continue;
+      float e, de, d2e;
+      barrier(d, S.d0, S.d1, S.qb, e, de, d2e);
+      const float h = S.tri_bary[3 * t + a] * S.tri_bary[3 * t + bb] * S.kappa * fmaxf(d2e, 0.f);
+      const float* n = S.tri_nrm + 3 * t;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[3 * r + c] += h * n[r] * n[c];
     }
   }
 #pragma unroll
@@ -1306,6 +1425,26 @@ __device__ double tet_merit(const mfem_solver& S, int e, float a) {
   return (double)psi - lc;
 }
 
+// triangle-capsule barrier energy at x + a dx, weighted by the barycentric
+// weights of the last full evaluation (contact.py:1050-1088)
+__device__ double tri_merit(const mfem_solver& S, int t, float a) {
+  float v[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int i = S.tris[3 * t + c];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[c][k] = S.x[3 * i + k] + a * (float)S.dx[3 * i + k];
+  }
+  float w[3], n[3];
+  int sid;
+  const float d = tri_capsule(S, v[0], v[1], v[2], w, n, &sid);
+  if (!(d < S.d1)) return 0.0;
+  float e, de, d2e;
+  barrier(d, S.d0, S.d1, S.qb, e, de, d2e);
+  const float* wl = S.tri_bary + 3 * t;
+  return (double)(S.kappa * e) * ((double)wl[0] + (double)wl[1] + (double)wl[2]);
+}
+
 // (g_x + grad B + grad D - G^T lambda) . dx for vertex i
 __device__ double vertex_slope(const mfem_solver& S, int i) {
   float gr[3];
@@ -1348,6 +1487,8 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
     f += tet_merit(S, e, 0.f);
     sl += tet_slope(S, e);
   }
+  if (tri_contact_on(S))
+    for (int t = tid; t < S.n_tris; t += nth) f += tri_merit(S, t, 0.f);
   double r0[2] = {f, sl};
   if (!grid_reduce(gs, r0)) return;
   const double f0 = r0[0];
@@ -1360,6 +1501,8 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
     f = 0.0;
     for (int i = tid; i < S.n_verts; i += nth) f += vertex_merit(S, i, a, dt);
     for (int e = tid; e < S.n_tets; e += nth) f += tet_merit(S, e, a);
+    if (tri_contact_on(S))
+      for (int t = tid; t < S.n_tris; t += nth) f += tri_merit(S, t, a);
     double r1[1] = {f};
     if (!grid_reduce(gs, r1)) return;
     f1 = r1[0];
@@ -1430,6 +1573,7 @@ __global__ void __launch_bounds__(kBlock) k_merit(mfem_solver S, float dt, float
   double f = 0.0;
   if (tid < S.n_verts) f += vertex_merit(S, tid, a, dt);
   if (tid < S.n_tets) f += tet_merit(S, tid, a);
+  if (tid < S.n_tris && tri_contact_on(S)) f += tri_merit(S, tid, a);
   f = wave_sum(f);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = f;
   __syncthreads();
@@ -1502,6 +1646,10 @@ int check_solver(const mfem_solver* S) {
   if (S->iterations > MFEM_MAX_STATS_ITERS) return fail(MFEM_EINVAL, "iterations > MFEM_MAX_STATS_ITERS");
   if (!S->x || !S->v || !S->s || !S->lam || !S->tets || !S->sell_val || !S->status || !S->stats)
     return fail(MFEM_EINVAL, "null buffer");
+  if (S->tri_contact && S->n_tris > 0 &&
+      (!S->tris || !S->tcon_ptr || !S->tcon_src || !S->vtri_ptr || !S->vtri_src || !S->tri_dist || !S->tri_bary ||
+       !S->tri_nrm || !S->tri_sid))
+    return fail(MFEM_EINVAL, "triangle contact enabled without its buffers");
   return MFEM_OK;
 }
 
@@ -1523,7 +1671,7 @@ int launched() {
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
   X(lag_sid) X(G) X(Hl) X(gl) X(Hsinv) X(gs) X(psi) X(Kb) X(fe) X(gx) X(gB) X(gD) X(Hv) X(dist) X(eB)      \
-  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess)
+  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid)
 
 extern "C" {
 
@@ -1565,6 +1713,10 @@ int mfem_update_system(const mfem_solver* S, float dt, void* stream) {
   if (int rc = check_solver(S)) return rc;
   if (!(dt > 0.f)) return fail(MFEM_EINVAL, "dt must be > 0");
   hipStream_t st = (hipStream_t)stream;
+  if (S->tri_contact && S->n_tris > 0 && S->n_shapes > 0) {
+    hipLaunchKernelGGL(k_tri_contact, dim3(blocks_for(S->n_tris)), dim3(kBlock), 0, st, *S);
+    if (int rc = launched()) return rc;
+  }
   hipLaunchKernelGGL(k_vertex, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, st, *S, dt);
   if (int rc = launched()) return rc;
   hipLaunchKernelGGL(k_element, dim3(blocks_for(S->n_tets)), dim3(kBlock), 0, st, *S);
@@ -1686,6 +1838,7 @@ int mfem_merit(const mfem_solver* S, float dt, float alpha, double* out, void* s
   hipStream_t st = (hipStream_t)stream;
   HIP_TRY(hipMemsetAsync(out, 0, sizeof(double), st));
   long n = S->n_verts > S->n_tets ? S->n_verts : S->n_tets;
+  if (S->n_tris > n) n = S->n_tris;
   hipLaunchKernelGGL(k_merit, dim3(blocks_for(n)), dim3(kBlock), 0, st, *S, dt, alpha, out);
   return launched();
 }

@@ -437,4 +437,156 @@ MFEM_DEV void quat_to_mat(const float* q, float* R) {
   R[6] = 2.f * (x * z - y * w); R[7] = 2.f * (y * z + x * w); R[8] = 1.f - 2.f * (x * x + y * y);
 }
 
+// ---------------------------------------------------------------------------
+// triangle / segment closest points (triangle-vs-capsule contact,
+// src/mfem/refinement/contact.py:417-550)
+// ---------------------------------------------------------------------------
+MFEM_DEV float dot3f(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+MFEM_DEV void cross3f(const float* a, const float* b, float* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// barycentric weights of the point of triangle abc closest to x (Voronoi
+// regions of the vertices, then the edges, then the face; Ericson,
+// Real-Time Collision Detection 5.1.5)
+MFEM_DEV void closest_point_triangle_bary(const float* x, const float* a, const float* b, const float* c, float* w) {
+  float ab[3], ac[3], ax[3], bx[3], cx[3];
+  for (int i = 0; i < 3; ++i) {
+    ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i]; ax[i] = x[i] - a[i]; bx[i] = x[i] - b[i]; cx[i] = x[i] - c[i];
+  }
+  const float d1 = dot3f(ab, ax), d2 = dot3f(ac, ax);
+  if (d1 <= 0.f && d2 <= 0.f) { w[0] = 1.f; w[1] = 0.f; w[2] = 0.f; return; }
+  const float d3 = dot3f(ab, bx), d4 = dot3f(ac, bx);
+  if (d3 >= 0.f && d4 <= d3) { w[0] = 0.f; w[1] = 1.f; w[2] = 0.f; return; }
+  const float d5 = dot3f(ab, cx), d6 = dot3f(ac, cx);
+  if (d6 >= 0.f && d5 <= d6) { w[0] = 0.f; w[1] = 0.f; w[2] = 1.f; return; }
+  const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) {
+    const float v = d1 / (d1 - d3);
+    w[0] = 1.f - v; w[1] = v; w[2] = 0.f;
+    return;
+  }
+  const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) {
+    const float t = d2 / (d2 - d6);
+    w[0] = 1.f - t; w[1] = 0.f; w[2] = t;
+    return;
+  }
+  const float va = d3 * d6 - d5 * d4;
+  if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
+    const float t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    w[0] = 0.f; w[1] = 1.f - t; w[2] = t;
+    return;
+  }
+  const float tot = va + vb + vc;
+  if (tot <= 0.f) { w[0] = w[1] = w[2] = 1.f / 3.f; return; }
+  const float v = vb / tot, t = vc / tot;
+  w[0] = 1.f - v - t; w[1] = v; w[2] = t;
+}
+
+// closest points of segments p1q1 and p2q2 (Ericson 5.1.9); returns the
+// parameter on the first segment
+MFEM_DEV float segment_segment(const float* p1, const float* q1, const float* p2, const float* q2, float* c1,
+                               float* c2) {
+  const float d1[3] = {q1[0] - p1[0], q1[1] - p1[1], q1[2] - p1[2]};
+  const float d2[3] = {q2[0] - p2[0], q2[1] - p2[1], q2[2] - p2[2]};
+  const float r[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
+  const float a = dot3f(d1, d1), e = dot3f(d2, d2), f = dot3f(d2, r);
+  const float eps = 1e-12f;
+  float s, t;
+  if (a <= eps && e <= eps) {
+    s = 0.f; t = 0.f;
+  } else if (a <= eps) {
+    s = 0.f; t = fminf(fmaxf(f / e, 0.f), 1.f);
+  } else {
+    const float c = dot3f(d1, r);
+    if (e <= eps) {
+      t = 0.f; s = fminf(fmaxf(-c / a, 0.f), 1.f);
+    } else {
+      const float b = dot3f(d1, d2), den = a * e - b * b;
+      s = den != 0.f ? fminf(fmaxf((b * f - c * e) / den, 0.f), 1.f) : 0.f;
+      t = (b * s + f) / e;
+      if (t < 0.f) {
+        t = 0.f; s = fminf(fmaxf(-c / a, 0.f), 1.f);
+      } else if (t > 1.f) {
+        t = 1.f; s = fminf(fmaxf((b - c) / a, 0.f), 1.f);
+      }
+    }
+  }
+  for (int i = 0; i < 3; ++i) {
+    c1[i] = p1[i] + d1[i] * s;
+    c2[i] = p2[i] + d2[i] * t;
+  }
+  return s;
+}
+
+// does segment p0p1 pierce triangle abc (Moller-Trumbore with the parameter
+// limited to [0, 1])?  w = barycentric weights of the piercing point
+MFEM_DEV bool segment_triangle_intersect(const float* p0, const float* p1, const float* a, const float* b,
+                                         const float* c, float* w) {
+  float dir[3], e1[3], e2[3], pv[3], tv[3], qv[3];
+  for (int i = 0; i < 3; ++i) {
+    dir[i] = p1[i] - p0[i]; e1[i] = b[i] - a[i]; e2[i] = c[i] - a[i]; tv[i] = p0[i] - a[i];
+  }
+  cross3f(dir, e2, pv);
+  const float det = dot3f(e1, pv);
+  const float scale = sqrtf(dot3f(dir, dir)) * sqrtf(dot3f(e1, e1)) * sqrtf(dot3f(e2, e2));
+  if (fabsf(det) <= 1e-7f * scale + 1e-30f) return false;
+  const float inv = 1.f / det;
+  const float u = dot3f(tv, pv) * inv;
+  if (u < 0.f || u > 1.f) return false;
+  cross3f(tv, e1, qv);
+  const float v = dot3f(dir, qv) * inv;
+  if (v < 0.f || u + v > 1.f) return false;
+  const float t = dot3f(e2, qv) * inv;
+  if (t < 0.f || t > 1.f) return false;
+  w[0] = 1.f - u - v; w[1] = u; w[2] = v;
+  return true;
+}
+
+// distance from triangle abc to segment p0p1: the two endpoints against the
+// face, the three edges against the segment, then the pierce test (distance
+// 0).  bary = weights of the triangle point, q = the segment point.
+MFEM_DEV float closest_point_triangle_segment(const float* a, const float* b, const float* c, const float* p0,
+                                              const float* p1, float* bary, float* q, bool* hit) {
+  float w[3], p[3];
+  closest_point_triangle_bary(p0, a, b, c, bary);
+  for (int i = 0; i < 3; ++i) {
+    p[i] = bary[0] * a[i] + bary[1] * b[i] + bary[2] * c[i] - p0[i];
+    q[i] = p0[i];
+  }
+  float best = sqrtf(dot3f(p, p));
+  closest_point_triangle_bary(p1, a, b, c, w);
+  for (int i = 0; i < 3; ++i) p[i] = w[0] * a[i] + w[1] * b[i] + w[2] * c[i] - p1[i];
+  float d = sqrtf(dot3f(p, p));
+  if (d < best) {
+    best = d;
+    for (int i = 0; i < 3; ++i) { bary[i] = w[i]; q[i] = p1[i]; }
+  }
+  const float* E0[3] = {a, b, c};
+  const float* E1[3] = {b, c, a};
+  for (int e = 0; e < 3; ++e) {
+    float c1[3], c2[3];
+    const float s = segment_segment(E0[e], E1[e], p0, p1, c1, c2);
+    const float df[3] = {c1[0] - c2[0], c1[1] - c2[1], c1[2] - c2[2]};
+    d = sqrtf(dot3f(df, df));
+    if (d < best) {
+      best = d;
+      if (e == 0) { bary[0] = 1.f - s; bary[1] = s; bary[2] = 0.f; }
+      else if (e == 1) { bary[0] = 0.f; bary[1] = 1.f - s; bary[2] = s; }
+      else { bary[0] = s; bary[1] = 0.f; bary[2] = 1.f - s; }
+      for (int i = 0; i < 3; ++i) q[i] = c2[i];
+    }
+  }
+  *hit = segment_triangle_intersect(p0, p1, a, b, c, w);
+  if (*hit) {
+    best = 0.f;
+    for (int i = 0; i < 3; ++i) bary[i] = w[i];
+    for (int i = 0; i < 3; ++i) q[i] = w[0] * a[i] + w[1] * b[i] + w[2] * c[i];
+  }
+  return best;
+}
+
 }  // namespace mfem

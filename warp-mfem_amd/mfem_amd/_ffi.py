@@ -28,6 +28,7 @@ FIELDS = [
     "vinc_ptr", "vinc_src", "x", "v", "s", "lam", "x_tilde", "x_start", "lag_n", "lag_fn", "lag_sid",
     "G", "Hl", "gl", "Hsinv", "gs", "psi", "Kb", "fe", "gx", "gB", "gD", "Hv", "dist", "eB", "eD", "g",
     "dx", "ds", "pz", "px", "red", "flags", "status", "stats", "mlp_w", "mlp_wbf", "ws_x", "ws_aux", "ws_y", "guess",
+    "tcon_ptr", "tcon_src", "vtri_ptr", "vtri_src", "tri_dist", "tri_bary", "tri_nrm", "tri_sid",
 )]
 
 

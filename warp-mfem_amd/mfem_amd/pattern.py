@@ -121,6 +121,48 @@ def build_system_pattern(n_verts: int, tets: np.ndarray) -> SystemPattern:
     )
 
 
+def build_triangle_lists(pat: SystemPattern, tris: np.ndarray):
+    """Gather lists of the triangle-capsule contact terms (once per topology).
+
+    Returns (tcon_ptr, tcon_src, vtri_ptr, vtri_src): for every SELL
+    lane-slot, the ordered triangle vertex pairs (tri * 9 + 3 a + b) whose
+    Hessian block w_a w_b k b'' n n^T lands on it; for every vertex, its
+    triangle corners (tri * 3 + a) in triangle order.  Surface triangles are
+    faces of tets, so every pair is a block of the tet pattern; a triangle
+    that is not raises ValueError.
+    """
+    tris = np.asarray(tris, dtype=np.int64).reshape(-1, 3)
+    n, F = pat.n_verts, tris.shape[0]
+    if F and (tris.min() < 0 or tris.max() >= n):
+        raise ValueError("triangle index out of range")
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(pat.rowptr.astype(np.int64)))
+    keys = rows * n + pat.cols.astype(np.int64)
+    dests, srcs = [], []
+    for a in range(3):
+        for b in range(3):
+            k = np.searchsorted(keys, tris[:, a] * n + tris[:, b])
+            k = np.minimum(k, len(keys) - 1)
+            if F and not np.all(keys[k] == tris[:, a] * n + tris[:, b]):
+                raise ValueError("a surface triangle edge is not an edge of the tet mesh")
+            dests.append(pat.csr_lane[k])
+            srcs.append(np.arange(F, dtype=np.int64) * 9 + 3 * a + b)
+    dest = np.concatenate(dests) if F else np.zeros(0, np.int64)
+    src = np.concatenate(srcs) if F else np.zeros(0, np.int64)
+    order = np.lexsort((src, dest))
+    dest, src = dest[order], src[order]
+    tcon_ptr = np.zeros(pat.n_slots * 64 + 1, dtype=np.int64)
+    np.add.at(tcon_ptr, dest + 1, 1)
+    tcon_ptr = np.cumsum(tcon_ptr)
+    vs = tris.reshape(-1)
+    vsrc = np.arange(3 * F, dtype=np.int64)
+    order = np.lexsort((vsrc, vs))
+    vtri_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(vtri_ptr, vs + 1, 1)
+    vtri_ptr = np.cumsum(vtri_ptr)
+    return (tcon_ptr.astype(np.int32), src.astype(np.int32), vtri_ptr.astype(np.int32),
+            vsrc[order].astype(np.int32))
+
+
 def sell_to_csr_values(pat: SystemPattern, sell_val: np.ndarray) -> np.ndarray:
     """(nnz, 3, 3) CSR block values from the lane-minor SELL value array."""
     sv = np.asarray(sell_val).reshape(pat.n_slots, 9, 64)

@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from . import _ffi
-from .pattern import build_system_pattern
+from .pattern import build_system_pattern, build_triangle_lists
 
 _ENERGIES = {"arap": 0, "neohookean": 1}
 
@@ -107,10 +107,6 @@ class RefinementSolver:
         elif mu.shape[0] != n_shapes:
             raise ValueError(f"friction_mu has {mu.shape[0]} entries but the model has {n_shapes} shapes")
         tri_contact = bool(kwargs.get("tri_contact", True))
-        has_capsule = any(sh.type == _ffi.SHAPE_CAPSULE for sh in shapes)
-        if tri_contact and has_capsule:
-            raise NotImplementedError("triangle-vs-capsule contact is not implemented by mfem_amd; "
-                                      "pass tri_contact=False for vertex-only capsule contact")
         d0 = float(kwargs.get("contact_d0", 1.0e-2))
         d1 = float(kwargs.get("contact_d1", 1.0e-1))
         qb = _barrier_at(d0, 0.0, d1)
@@ -178,6 +174,22 @@ class RefinementSolver:
         b["flags"] = torch.zeros(_ffi.MFEM_RED_MAX_BLOCKS, dtype=torch.int32, device=dev)
         b["status"] = torch.zeros(8, dtype=torch.int32, device=dev)
         b["stats"] = torch.zeros(_ffi.MFEM_MAX_STATS_ITERS * _ffi.MFEM_STATS_PER_ITER, dtype=torch.float64, device=dev)
+        # triangle-capsule contact: gather lists and per-triangle state
+        n_tris = len(tris)
+        if n_tris:
+            tcon_ptr, tcon_src, vtri_ptr, vtri_src = build_triangle_lists(pat, tris)
+        else:
+            tcon_ptr = np.zeros(pat.n_slots * 64 + 1, np.int32)
+            tcon_src = vtri_src = np.zeros(1, np.int32)
+            vtri_ptr = np.zeros(nv + 1, np.int32)
+        b["tcon_ptr"] = i32(tcon_ptr)
+        b["tcon_src"] = i32(tcon_src if len(tcon_src) else np.zeros(1, np.int32))
+        b["vtri_ptr"] = i32(vtri_ptr)
+        b["vtri_src"] = i32(vtri_src if len(vtri_src) else np.zeros(1, np.int32))
+        b["tri_dist"] = torch.full((max(n_tris, 1),), 1.0e8, dtype=torch.float32, device=dev)
+        b["tri_bary"] = zf(max(n_tris, 1), 3)
+        b["tri_nrm"] = zf(max(n_tris, 1), 3)
+        b["tri_sid"] = torch.full((max(n_tris, 1),), -1, dtype=torch.int32, device=dev)
         # learned warm start workspace (features, candidates, outputs, guess)
         b["ws_x"] = torch.zeros(nv, 16, dtype=torch.int16, device=dev)
         b["ws_aux"] = zf(nv, 10)
@@ -198,7 +210,8 @@ class RefinementSolver:
         S.precond = int(bool(kwargs.get("preconditioner", True)))
         S.line_search = int(bool(kwargs.get("line_search", True)))
         S.ls_max = int(kwargs.get("line_search_max_iterations", 20))
-        S.tri_contact = 0
+        # Contact(tri_contact=...) keeps it only with triangles (contact.py:844)
+        S.tri_contact = int(tri_contact and len(tris) > 0)
         S.friction_on = int(n_shapes > 0 and float(mu.max(initial=0.0)) > 0.0)
         S.cg_tol = float(kwargs.get("cg_tolerance", 1.0e-6))
         S.ls_alpha0 = float(kwargs.get("line_search_alpha0", 1.0))
