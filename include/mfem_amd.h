@@ -250,6 +250,54 @@ int mfem_mlp_backward_step(const uint16_t* x, const float* dy, int n, float* w_f
                            void* stream);
 const char* mfem_mlp_last_error(void);
 
+/* ---------------------------------------------------------------------
+   Self-contact building blocks (mfem_collide.hip)
+   --------------------------------------------------------------------- */
+#define MFEM_BP_SPAN 1    /* a primitive spans more than max_span cells per axis (not inserted / queried) */
+#define MFEM_BP_ENTRIES 2 /* hash-table entry capacity exceeded */
+#define MFEM_BP_PAIRS 4   /* more candidate pairs than max_pairs (n_pairs holds the full count) */
+
+/* Exclusive (inclusive != 0: inclusive) prefix sum of n int32 values;
+   total (device, may be null) receives the sum.  Replaces TiledScan /
+   wp.utils.array_scan for int32 (src/mfem/scan.py:21-193,
+   refinement/refinement.py:1185-1186). */
+int mfem_scan_i32(const int32_t* in, int32_t* out, int n, int inclusive, int32_t* total, void* workspace,
+                  size_t workspace_bytes, void* stream);
+size_t mfem_scan_workspace_bytes(int n);
+
+/* Spatial-hash broad phase: all point-triangle pairs (points[i], tri) whose
+   boxes inflated by d_hat / 2 overlap, the point not a vertex of the
+   triangle, as (vertex id, tri index) sorted by query order then tri index;
+   edge-edge: all (e1, e2), e1 < e2, no shared vertex.  cell = grid cell
+   size (>= the largest inflated primitive extent keeps max_span = 2);
+   n_pairs (device) = total count, status (device) = MFEM_BP_* bits.  The
+   reference has no live broad phase (its SAP and hash table are unwired,
+   src/mfem/collision.py:840-895, refinement/geometry_hash.py:8-90). */
+int mfem_broadphase_pt(const float* x, const int32_t* points, int n_points, const int32_t* tris, int n_tris,
+                       float d_hat, float cell, int max_span, int32_t* pairs, int max_pairs, int32_t* n_pairs,
+                       int32_t* status, void* workspace, size_t workspace_bytes, void* stream);
+int mfem_broadphase_ee(const float* x, const int32_t* edges, int n_edges, float d_hat, float cell, int max_span,
+                       int32_t* pairs, int max_pairs, int32_t* n_pairs, int32_t* status, void* workspace,
+                       size_t workspace_bytes, void* stream);
+size_t mfem_broadphase_workspace_bytes(int n_insert, int n_query, int max_span);
+
+/* eval_soft_rigid_distance (src/mfem/collision.py:380-476): for rigid/soft
+   candidate triangle pairs, per soft particle the minimum of the 15 feature
+   distances below d_hat (distance pre-filled with a sentinel >= d_hat). */
+int mfem_soft_rigid_distance(const float* points, const int32_t* tris, int rigid_points_count, int rigid_tri_count,
+                             const int32_t* pairs, const int32_t* pair_count, int max_pairs, float d_hat,
+                             float* distance, void* stream);
+
+/* IPC barrier kappa * b(d), b = -(d - d_hat)^2 ln(d / d_hat)
+   (src/mfem/collision.py:529-546), on point-triangle (kind 0, pairs =
+   (vertex, tri)) or edge-edge (kind 1, pairs = (edge, edge)) candidates:
+   per pair the distance, the energy (FP64) and its gradient with respect to
+   the pair's four vertices (grad[k][4][3], vertex ids in vids[k][4]). */
+int mfem_pair_barrier(int kind, const float* x, const int32_t* pairs, const int32_t* n_pairs, int max_pairs,
+                      const int32_t* tris, const int32_t* edges, float kappa, float d_hat, float* dist,
+                      double* energy, float* grad, int32_t* vids, void* stream);
+const char* mfem_collide_last_error(void);
+
 /* y = H x with the assembled SELL matrix (warp.sparse.bsr_mv). */
 int mfem_spmv(const mfem_solver* S, const float* x, float* y, void* stream);
 

@@ -50,6 +50,8 @@ ENTRY_POINTS = [
     "mfem_begin_step", "mfem_update_system", "mfem_assemble", "mfem_global_solve", "mfem_local_solve",
     "mfem_line_search", "mfem_end_step", "mfem_step", "mfem_spmv", "mfem_merit", "mfem_ws_features",
     "mfem_ws_apply", "mfem_mlp_forward", "mfem_mlp_train_step", "mfem_mlp_backward_step", "mfem_mlp_last_error",
+    "mfem_scan_i32", "mfem_scan_workspace_bytes", "mfem_broadphase_pt", "mfem_broadphase_ee",
+    "mfem_broadphase_workspace_bytes", "mfem_soft_rigid_distance", "mfem_pair_barrier", "mfem_collide_last_error",
 ]
 
 
@@ -94,6 +96,17 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     for name in ENTRY_POINTS[5:]:
         getattr(lib, name).restype = ctypes.c_int
     lib.mfem_mlp_last_error.restype = ctypes.c_char_p
+    i, f, sz = ctypes.c_int, ctypes.c_float, ctypes.c_size_t
+    lib.mfem_scan_i32.argtypes = [vp, vp, i, i, vp, vp, sz, vp]
+    lib.mfem_scan_workspace_bytes.argtypes = [i]
+    lib.mfem_scan_workspace_bytes.restype = sz
+    lib.mfem_broadphase_pt.argtypes = [vp, vp, i, vp, i, f, f, i, vp, i, vp, vp, vp, sz, vp]
+    lib.mfem_broadphase_ee.argtypes = [vp, vp, i, f, f, i, vp, i, vp, vp, vp, sz, vp]
+    lib.mfem_broadphase_workspace_bytes.argtypes = [i, i, i]
+    lib.mfem_broadphase_workspace_bytes.restype = sz
+    lib.mfem_soft_rigid_distance.argtypes = [vp, vp, i, i, vp, vp, i, f, vp, vp]
+    lib.mfem_pair_barrier.argtypes = [i, vp, vp, vp, i, vp, vp, f, f, vp, vp, vp, vp, vp]
+    lib.mfem_collide_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 
