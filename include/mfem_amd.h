@@ -298,6 +298,63 @@ int mfem_pair_barrier(int kind, const float* x, const int32_t* pairs, const int3
                       double* energy, float* grad, int32_t* vids, void* stream);
 const char* mfem_collide_last_error(void);
 
+/* ---------------------------------------------------------------------
+   Adaptive refinement, one pass (mfem_refine.hip): replaces refine() of
+   src/mfem/refinement/refinement.py:914-1322 ("legacy" scoring).  The
+   reference's pass is race-dependent; this one is deterministic: greedy
+   selection of split edges by (score, edge key), at most one split edge per
+   tet, computed as iterated local maxima.  Call sequence:
+     mfem_refine_select(phase 0)            scores, candidate table
+     mfem_refine_select(phase 1) repeatedly  until *changed == 0
+     mfem_refine_count                      totals[0..2] = tets, new vertices, triangles after the pass
+     (host: drop the pass when the totals exceed the capacities)
+     mfem_refine_apply                      new vertices appended in place, tets / triangles / per-tet data to O
+   --------------------------------------------------------------------- */
+#define MFEM_REFINE_TABLE_FULL 1
+
+typedef struct mfem_refine_args {
+  const int32_t* tets;         /* [n_tets][4] */
+  const int32_t* tris;         /* [n_tris][3] */
+  const float* x;              /* [n_verts][3] positions (scores, midpoint test) */
+  const float* inv_mass;       /* [n_verts] 0 = pinned; edges between pinned vertices are never split */
+  const float* tet_score;      /* [n_tets] (the solver passes the elastic energy) */
+  const float* vertex_score;   /* [n_verts] (the solver passes the contact barrier energy) */
+  const int32_t* shape_type;   /* [n_shapes] midpoint penetration test */
+  const float* shape_xf;       /* [n_shapes][7] */
+  const float* shape_scale;    /* [n_shapes][3] */
+  const float* stretch;        /* [n_tets][6] */
+  const float* lam;            /* [n_tets][6] */
+  const float* materials;      /* [n_tets][3] */
+  const float* dminv;          /* [n_tets][9] */
+  int32_t n_verts, n_tets, n_tris, n_shapes;
+  int32_t max_tets, max_tris, max_verts;
+  float threshold;             /* min_refine_score */
+  float w_tet, w_vertex;       /* refine_tet_score_weight, refine_vertex_score_weight */
+  float penetrating_score;     /* score of an edge whose midpoint is inside a shape */
+  float split_t;               /* refine_split_position */
+} mfem_refine_args;
+
+typedef struct mfem_refine_out {
+  float* x;           /* [max_verts][3] new vertices appended at n_verts.. (may alias args.x) */
+  float* v;           /* [max_verts][3] */
+  float* rest;        /* [max_verts][3] rest positions */
+  int32_t* tets;      /* [max_tets][4] */
+  float* stretch;     /* [max_tets][6] */
+  float* lam;         /* [max_tets][6] */
+  float* materials;   /* [max_tets][3] */
+  float* dminv;       /* [max_tets][9] */
+  int32_t* tris;      /* [max_tris][3] */
+} mfem_refine_out;
+
+size_t mfem_refine_workspace_bytes(int max_tets, int max_tris);
+int mfem_refine_select(const mfem_refine_args* args, void* workspace, size_t workspace_bytes, int32_t* status,
+                       int32_t* changed, int phase, void* stream);
+int mfem_refine_count(const mfem_refine_args* args, void* workspace, size_t workspace_bytes, int32_t* totals,
+                      void* stream);
+int mfem_refine_apply(const mfem_refine_args* args, void* workspace, size_t workspace_bytes,
+                      const mfem_refine_out* out, void* stream);
+const char* mfem_refine_last_error(void);
+
 /* y = H x with the assembled SELL matrix (warp.sparse.bsr_mv). */
 int mfem_spmv(const mfem_solver* S, const float* x, float* y, void* stream);
 

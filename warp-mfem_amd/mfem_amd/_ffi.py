@@ -52,6 +52,8 @@ ENTRY_POINTS = [
     "mfem_ws_apply", "mfem_mlp_forward", "mfem_mlp_train_step", "mfem_mlp_backward_step", "mfem_mlp_last_error",
     "mfem_scan_i32", "mfem_scan_workspace_bytes", "mfem_broadphase_pt", "mfem_broadphase_ee",
     "mfem_broadphase_workspace_bytes", "mfem_soft_rigid_distance", "mfem_pair_barrier", "mfem_collide_last_error",
+    "mfem_refine_workspace_bytes", "mfem_refine_select", "mfem_refine_count", "mfem_refine_apply",
+    "mfem_refine_last_error",
 ]
 
 
@@ -107,6 +109,12 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mfem_soft_rigid_distance.argtypes = [vp, vp, i, i, vp, vp, i, f, vp, vp]
     lib.mfem_pair_barrier.argtypes = [i, vp, vp, vp, i, vp, vp, f, f, vp, vp, vp, vp, vp]
     lib.mfem_collide_last_error.restype = ctypes.c_char_p
+    lib.mfem_refine_workspace_bytes.argtypes = [i, i]
+    lib.mfem_refine_workspace_bytes.restype = sz
+    lib.mfem_refine_select.argtypes = [vp, vp, sz, vp, vp, i, vp]
+    lib.mfem_refine_count.argtypes = [vp, vp, sz, vp, vp]
+    lib.mfem_refine_apply.argtypes = [vp, vp, sz, vp, vp]
+    lib.mfem_refine_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 

@@ -48,10 +48,19 @@ class State:
                      None if self.body_q is None else self.body_q.clone())
 
 
-def model_state(model, device="cuda") -> State:
-    """A State holding the model's initial positions / velocities."""
-    q = torch.as_tensor(np.asarray(model.particle_q, dtype=np.float32), device=device).contiguous()
-    qd = torch.as_tensor(np.asarray(model.particle_qd, dtype=np.float32), device=device).contiguous()
+def model_state(model, device="cuda", capacity: int | None = None) -> State:
+    """A State holding the model's initial positions / velocities, padded to
+    ``capacity`` rows (the solver's max_particles when refinement may add
+    vertices)."""
+    q0 = np.asarray(model.particle_q, dtype=np.float32).reshape(-1, 3)
+    qd0 = np.asarray(model.particle_qd, dtype=np.float32).reshape(-1, 3)
+    n = len(q0) if capacity is None else int(capacity)
+    if n < len(q0):
+        raise ValueError("capacity < model.particle_count")
+    q = torch.zeros(n, 3, dtype=torch.float32, device=device)
+    qd = torch.zeros(n, 3, dtype=torch.float32, device=device)
+    q[:len(q0)] = torch.as_tensor(q0, device=device)
+    qd[:len(qd0)] = torch.as_tensor(qd0, device=device)
     return State(q, qd)
 
 
@@ -71,26 +80,36 @@ class RefinementSolver:
     def __init__(self, model, iterations: int, max_tets: int, max_particles: int | None = None,
                  max_tris: int | None = None, refine_every_n_steps: int = 10, min_refine_score: float = 10.1,
                  device: str = "cuda", **kwargs):
-        if kwargs.get("enable_refinement", True):
-            raise NotImplementedError(
-                "adaptive refinement is not implemented by mfem_amd; pass enable_refinement=False")
+        self._refine_enabled = bool(kwargs.get("enable_refinement", True))
+        if self._refine_enabled and kwargs.get("refine_scoring", "legacy") != "legacy":
+            if kwargs["refine_scoring"] not in ("legacy", "geometric"):
+                raise ValueError(f"Unknown refine_scoring {kwargs['refine_scoring']!r} (expected 'legacy' or 'geometric')")
+            raise NotImplementedError("refine_scoring='geometric' is not implemented by mfem_amd (use 'legacy')")
         if kwargs.get("shell_mu", 0.0) > 0.0:
             raise NotImplementedError("membrane shell energy (shell_mu > 0) is not implemented by mfem_amd")
         energy = kwargs.get("energy", "arap")
         if energy not in _ENERGIES:
             raise ValueError(f"Unknown energy type: {energy}")
         nv, nt = int(model.particle_count), int(model.tet_count)
-        if max_particles is not None and max_particles != nv:
-            raise ValueError(f"max_particles={max_particles} does not match model.particle_count={nv}")
+        if max_particles is not None and max_particles < nv:
+            raise ValueError(f"max_particles={max_particles} < model.particle_count={nv}")
         if max_tets < nt:
             raise ValueError(f"max_tets={max_tets} < model.tet_count={nt}")
+        tris = np.zeros((0, 3), np.int32) if model.tri_indices is None else np.asarray(model.tri_indices, np.int32)
+        tris = tris.reshape(-1, 3)
+        Vc = nv if max_particles is None else int(max_particles)
+        Tc = int(max_tets)
+        # surface triangles grow with the tets (same headroom when not given)
+        Fc = int(max_tris) if max_tris is not None else int(np.ceil(len(tris) * Tc / max(nt, 1)))
+        if Fc < len(tris):
+            raise ValueError(f"max_tris={Fc} < the model's {len(tris)} surface triangles")
         if iterations > _ffi.MFEM_MAX_STATS_ITERS:
             raise ValueError(f"iterations must be <= {_ffi.MFEM_MAX_STATS_ITERS}")
         self.lib = _ffi.load()
         _ffi.check_layout(self.lib)
         self.model = model
         self.iterations = int(iterations)
-        self.max_tets, self.max_particles = int(max_tets), nv
+        self.max_tets, self.max_particles, self.max_tris = Tc, Vc, Fc
         self.refine_every_n_steps = refine_every_n_steps
         self.min_refine_score = min_refine_score
         self.device = torch.device(device)
@@ -119,88 +138,81 @@ class RefinementSolver:
         tets = np.asarray(model.tet_indices, dtype=np.int32).reshape(-1, 4)
         dminv = np.asarray(model.tet_poses, dtype=np.float64).reshape(-1, 3, 3)
         vol = 1.0 / (6.0 * np.linalg.det(dminv))
-        pat = build_system_pattern(nv, tets)
-        self.pattern = pat
         inv_mass = np.asarray(model.particle_inv_mass, dtype=np.float64)
         pin_pos = np.where((inv_mass == 0)[:, None], np.asarray(model.particle_q, dtype=np.float64), 0.0)
 
+        def pad(a, n, dtype=np.float32, fill=0):
+            a = np.asarray(a, dtype=dtype)
+            out = np.full((n,) + a.shape[1:], fill, dtype=dtype)
+            out[:len(a)] = a
+            return out
+
+        # every per-vertex / per-tet / per-triangle buffer is allocated at its
+        # capacity (refinement appends); the struct carries the active counts
         b = {}
-        b["tets"] = i32(tets)
-        b["dminv"] = f32(dminv.reshape(-1, 9))
-        b["materials"] = f32(np.asarray(model.tet_materials).reshape(-1, 3))
-        b["vol"] = f32(vol)
-        b["mass"] = f32(model.particle_mass)
-        b["inv_mass"] = f32(inv_mass)
-        b["pin_pos"] = f32(pin_pos)
-        tris = np.zeros((0, 3), np.int32) if model.tri_indices is None else np.asarray(model.tri_indices, np.int32)
-        b["tris"] = i32(tris.reshape(-1, 3)) if len(tris) else torch.zeros(1, dtype=torch.int32, device=dev)
+        b["tets"] = i32(pad(tets, Tc, np.int32))
+        b["dminv"] = f32(pad(dminv.reshape(-1, 9), Tc))
+        b["materials"] = f32(pad(np.asarray(model.tet_materials).reshape(-1, 3), Tc))
+        b["vol"] = f32(pad(vol, Tc))
+        b["mass"] = f32(pad(model.particle_mass, Vc))
+        b["inv_mass"] = f32(pad(inv_mass, Vc))
+        b["pin_pos"] = f32(pad(pin_pos, Vc))
+        b["rest"] = f32(pad(np.asarray(model.particle_q).reshape(-1, 3), Vc))  # rest positions (refinement)
+        b["tris"] = i32(pad(tris, max(Fc, 1), np.int32))
         b["shape_type"] = i32([sh.type for sh in shapes] or [0])
         b["shape_xf"] = f32(np.array([np.concatenate([sh.pos, sh.quat]) for sh in shapes]) if shapes else np.zeros((1, 7)))
         b["shape_scale"] = f32(np.array([sh.scale for sh in shapes]) if shapes else np.zeros((1, 3)))
         b["shape_mu"] = f32(mu if n_shapes else np.zeros(1))
-        b["chunk_ptr"] = i32(pat.chunk_ptr)
-        b["sell_col"] = i32(pat.sell_col)
-        b["sell_val"] = zf(pat.n_slots * 576)
-        b["diag_lane"] = i32(pat.diag_lane)
-        b["contrib_ptr"] = i32(pat.contrib_ptr)
-        b["contrib_src"] = i32(pat.contrib_src) if len(pat.contrib_src) else torch.zeros(1, dtype=torch.int32, device=dev)
-        b["vinc_ptr"] = i32(pat.vinc_ptr)
-        b["vinc_src"] = i32(pat.vinc_src)
-        b["s"] = zf(nt, 6)
-        b["lam"] = zf(nt, 6)
-        b["x_tilde"] = zf(nv, 3)
-        b["x_start"] = zf(nv, 3)
-        b["lag_n"] = zf(nv, 3)
-        b["lag_fn"] = zf(nv)
-        b["lag_sid"] = torch.full((nv,), -1, dtype=torch.int32, device=dev)
-        b["G"] = zf(nt, 72)
-        b["Hl"] = zf(nt, 21)
-        b["gl"] = zf(nt, 6)
-        b["Hsinv"] = zf(nt, 21)
-        b["gs"] = zf(nt, 6)
-        b["psi"] = zf(nt)
-        b["Kb"] = zf(10, nt, 9)
-        b["fe"] = zf(nt, 12)
+        b["s"] = zf(Tc, 6)
+        b["lam"] = zf(Tc, 6)
+        b["x_tilde"] = zf(Vc, 3)
+        b["x_start"] = zf(Vc, 3)
+        b["lag_n"] = zf(Vc, 3)
+        b["lag_fn"] = zf(Vc)
+        b["lag_sid"] = torch.full((Vc,), -1, dtype=torch.int32, device=dev)
+        b["G"] = zf(Tc, 72)
+        b["Hl"] = zf(Tc, 21)
+        b["gl"] = zf(Tc, 6)
+        b["Hsinv"] = zf(Tc, 21)
+        b["gs"] = zf(Tc, 6)
+        b["psi"] = zf(Tc)
+        b["Kb"] = zf(10 * Tc * 9)  # [10][n_tets][9] for the active n_tets
+        b["fe"] = zf(Tc, 12)
         for n in ("gx", "gB", "gD", "g"):
-            b[n] = zf(nv, 3)
-        b["Hv"] = zf(nv, 9)
+            b[n] = zf(Vc, 3)
+        b["Hv"] = zf(Vc, 9)
         for n in ("dist", "eB", "eD"):
-            b[n] = zf(nv)
-        b["dx"] = torch.zeros(nv, 3, dtype=torch.float64, device=dev)
-        b["ds"] = zf(nt, 6)
-        b["pz"] = zf(2, pat.n_chunks * 64, 3)
-        b["px"] = torch.zeros(2, pat.n_chunks * 64, 3, dtype=torch.float64, device=dev)
+            b[n] = zf(Vc)
+        b["dx"] = torch.zeros(Vc, 3, dtype=torch.float64, device=dev)
+        b["ds"] = zf(Tc, 6)
         b["red"] = torch.zeros(2 * _ffi.MFEM_RED_MAX_BLOCKS * 4, dtype=torch.float64, device=dev)
         b["flags"] = torch.zeros(_ffi.MFEM_RED_MAX_BLOCKS, dtype=torch.int32, device=dev)
         b["status"] = torch.zeros(8, dtype=torch.int32, device=dev)
         b["stats"] = torch.zeros(_ffi.MFEM_MAX_STATS_ITERS * _ffi.MFEM_STATS_PER_ITER, dtype=torch.float64, device=dev)
-        # triangle-capsule contact: gather lists and per-triangle state
-        n_tris = len(tris)
-        if n_tris:
-            tcon_ptr, tcon_src, vtri_ptr, vtri_src = build_triangle_lists(pat, tris)
-        else:
-            tcon_ptr = np.zeros(pat.n_slots * 64 + 1, np.int32)
-            tcon_src = vtri_src = np.zeros(1, np.int32)
-            vtri_ptr = np.zeros(nv + 1, np.int32)
-        b["tcon_ptr"] = i32(tcon_ptr)
-        b["tcon_src"] = i32(tcon_src if len(tcon_src) else np.zeros(1, np.int32))
-        b["vtri_ptr"] = i32(vtri_ptr)
-        b["vtri_src"] = i32(vtri_src if len(vtri_src) else np.zeros(1, np.int32))
-        b["tri_dist"] = torch.full((max(n_tris, 1),), 1.0e8, dtype=torch.float32, device=dev)
-        b["tri_bary"] = zf(max(n_tris, 1), 3)
-        b["tri_nrm"] = zf(max(n_tris, 1), 3)
-        b["tri_sid"] = torch.full((max(n_tris, 1),), -1, dtype=torch.int32, device=dev)
+        b["tri_dist"] = torch.full((max(Fc, 1),), 1.0e8, dtype=torch.float32, device=dev)
+        b["tri_bary"] = zf(max(Fc, 1), 3)
+        b["tri_nrm"] = zf(max(Fc, 1), 3)
+        b["tri_sid"] = torch.full((max(Fc, 1),), -1, dtype=torch.int32, device=dev)
         # learned warm start workspace (features, candidates, outputs, guess)
-        b["ws_x"] = torch.zeros(nv, 16, dtype=torch.int16, device=dev)
-        b["ws_aux"] = zf(nv, 10)
-        b["ws_y"] = zf(nv, 3)
-        b["guess"] = zf(nv, 3)
+        b["ws_x"] = torch.zeros(Vc, 16, dtype=torch.int16, device=dev)
+        b["ws_aux"] = zf(Vc, 10)
+        b["ws_y"] = zf(Vc, 3)
+        b["guess"] = zf(Vc, 3)
         self.buffers = b
+        self._pinned = torch.as_tensor(pad(inv_mass == 0, Vc, np.bool_), device=dev)
+        self._refine_density = float(kwargs.get("refine_density", 1.0))
+        self._refine_kw = dict(
+            threshold=float(min_refine_score),
+            w_tet=float(kwargs.get("refine_tet_score_weight", 0.0)),
+            w_vertex=float(kwargs.get("refine_vertex_score_weight", 0.1)),
+            penetrating_score=float(kwargs.get("penetrating_edge_score", 1.0e6)),
+            split_t=float(kwargs.get("refine_split_position", 0.5)))
+        self._pass_counter = 0
+        self._refiner = None
+        self.refine_log: list[dict] = []
 
         S = _ffi.MfemSolverStruct()
-        S.n_verts, S.n_tets, S.n_tris, S.n_shapes = nv, nt, len(tris), n_shapes
-        S.n_chunks, S.n_slots = pat.n_chunks, pat.n_slots
-        S.max_width = int(np.diff(pat.chunk_ptr).max())
+        S.n_shapes = n_shapes
         # solver-specific knobs (not reference kwargs): pipelined PCG, resident rows
         S.pcg_flags = (_ffi.PCG_PIPELINED if kwargs.get("pcg_pipelined", False) else 0) | \
                       (0 if kwargs.get("pcg_resident_rows", True) else _ffi.PCG_STREAM_ROWS)
@@ -212,6 +224,7 @@ class RefinementSolver:
         S.ls_max = int(kwargs.get("line_search_max_iterations", 20))
         # Contact(tri_contact=...) keeps it only with triangles (contact.py:844)
         S.tri_contact = int(tri_contact and len(tris) > 0)
+        self._tri_contact_kw = tri_contact
         S.friction_on = int(n_shapes > 0 and float(mu.max(initial=0.0)) > 0.0)
         S.cg_tol = float(kwargs.get("cg_tolerance", 1.0e-6))
         S.ls_alpha0 = float(kwargs.get("line_search_alpha0", 1.0))
@@ -226,16 +239,133 @@ class RefinementSolver:
         S.qb[:] = qb
         S.gravity[:] = [float(g) for g in np.asarray(model.gravity).reshape(3)]
         for name, t in b.items():
-            setattr(S, name, t.data_ptr())
+            if hasattr(S, name):
+                setattr(S, name, t.data_ptr())
         self._S = S
+        self._set_topology(nv, nt, len(tris))
         self.warm_start_net = None
         self.set_warm_start(kwargs.get("warm_start"))
 
         # AdditionalState.from_model: rest stretch from the model's positions
-        x0 = f32(np.asarray(model.particle_q).reshape(-1, 3))
-        self._rest_q = x0
-        self._bind_state(x0, zf(nv, 3))
+        x0 = b["rest"].clone()
+        self._bind_state(x0, zf(Vc, 3))
         self._call("mfem_init_stretch", self._stream())
+
+    # -- topology -------------------------------------------------------------
+    def _set_topology(self, n_verts: int, n_tets: int, n_tris: int) -> None:
+        """(Re)builds the system pattern and the gather lists of the active
+        mesh (host, numpy: once per topology) and points the struct at them."""
+        b, dev = self.buffers, self.device
+        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=dev)
+        tets = b["tets"][:n_tets].cpu().numpy()
+        tris = b["tris"][:n_tris].cpu().numpy()
+        pat = build_system_pattern(n_verts, tets)
+        self.pattern = pat
+        b["chunk_ptr"] = i32(pat.chunk_ptr)
+        b["sell_col"] = i32(pat.sell_col)
+        b["sell_val"] = torch.zeros(pat.n_slots * 576, dtype=torch.float32, device=dev)
+        b["diag_lane"] = i32(pat.diag_lane)
+        b["contrib_ptr"] = i32(pat.contrib_ptr)
+        b["contrib_src"] = i32(pat.contrib_src if len(pat.contrib_src) else np.zeros(1))
+        b["vinc_ptr"] = i32(pat.vinc_ptr)
+        b["vinc_src"] = i32(pat.vinc_src)
+        b["pz"] = torch.zeros(2, pat.n_chunks * 64, 3, dtype=torch.float32, device=dev)
+        b["px"] = torch.zeros(2, pat.n_chunks * 64, 3, dtype=torch.float64, device=dev)
+        if n_tris:
+            tcon_ptr, tcon_src, vtri_ptr, vtri_src = build_triangle_lists(pat, tris)
+        else:
+            tcon_ptr = np.zeros(pat.n_slots * 64 + 1, np.int32)
+            tcon_src = vtri_src = np.zeros(1, np.int32)
+            vtri_ptr = np.zeros(n_verts + 1, np.int32)
+        b["tcon_ptr"] = i32(tcon_ptr)
+        b["tcon_src"] = i32(tcon_src if len(tcon_src) else np.zeros(1))
+        b["vtri_ptr"] = i32(vtri_ptr)
+        b["vtri_src"] = i32(vtri_src if len(vtri_src) else np.zeros(1))
+        S = self._S
+        S.n_verts, S.n_tets, S.n_tris = int(n_verts), int(n_tets), int(n_tris)
+        S.n_chunks, S.n_slots = pat.n_chunks, pat.n_slots
+        S.max_width = int(np.diff(pat.chunk_ptr).max())
+        S.tri_contact = int(self._tri_contact_kw and n_tris > 0)
+        for name in ("chunk_ptr", "sell_col", "sell_val", "diag_lane", "contrib_ptr", "contrib_src", "vinc_ptr",
+                     "vinc_src", "pz", "px", "tcon_ptr", "tcon_src", "vtri_ptr", "vtri_src"):
+            setattr(S, name, b[name].data_ptr())
+
+    @property
+    def active_particle_count(self) -> int:
+        return int(self._S.n_verts)
+
+    @property
+    def active_tet_count(self) -> int:
+        return int(self._S.n_tets)
+
+    @property
+    def active_tri_count(self) -> int:
+        return int(self._S.n_tris)
+
+    def _refresh_masses(self) -> None:
+        """Lumped masses from the rest volumes and refine_density, as
+        refine() rewrites model.particle_mass on every call
+        (refinement.py:1253, scatter_tets :733-824).  The inverse masses of
+        free vertices follow (pinned vertices keep 0); the reference keeps
+        model.particle_inv_mass unchanged in its Hessian mass term."""
+        b = self.buffers
+        nv, nt = self.active_particle_count, self.active_tet_count
+        vol = b["vol"][:nt].double().cpu().numpy()
+        tets = b["tets"][:nt].cpu().numpy()
+        m = np.zeros(nv)
+        np.add.at(m, tets.reshape(-1), np.repeat(self._refine_density * vol / 4.0, 4))
+        pinned = self._pinned[:nv].cpu().numpy()
+        b["mass"][:nv] = torch.as_tensor(m, dtype=torch.float32, device=self.device)
+        b["inv_mass"][:nv] = torch.as_tensor(np.where(pinned, 0.0, 1.0 / np.maximum(m, 1e-30)), dtype=torch.float32,
+                                             device=self.device)
+
+    def _refine_pass(self, state: "State") -> dict:
+        """One refine() call (refinement.py:914-1322): gated by
+        refine_every_n_steps on a pass counter, as in the reference."""
+        from .refine import Refiner
+
+        counter = self._pass_counter
+        self._pass_counter += 1
+        every = max(int(self.refine_every_n_steps), 1)
+        res = {"pass": counter, "fired": counter % every == 0, "new_verts": 0, "dropped": False}
+        changed = False
+        if res["fired"]:
+            b = self.buffers
+            if self._refiner is None:
+                self._refiner = Refiner(self.max_tets, self.max_tris, self.max_particles, self.device)
+                dev = self.device
+                self._ref_out = {"tets": torch.zeros(self.max_tets, 4, dtype=torch.int32, device=dev),
+                                 "stretch": torch.zeros(self.max_tets, 6, device=dev),
+                                 "lam": torch.zeros(self.max_tets, 6, device=dev),
+                                 "materials": torch.zeros(self.max_tets, 3, device=dev),
+                                 "dminv": torch.zeros(self.max_tets, 9, device=dev),
+                                 "tris": torch.zeros(max(self.max_tris, 1), 3, dtype=torch.int32, device=dev)}
+            S = self._S
+            shapes = (b["shape_type"], b["shape_xf"], b["shape_scale"]) if S.n_shapes > 0 else None
+            r = self._refiner.run(tets=b["tets"], tris=b["tris"], x=state.particle_q, v=state.particle_qd,
+                                  rest=b["rest"], inv_mass=b["inv_mass"], tet_score=b["psi"], vertex_score=b["eB"],
+                                  stretch=b["s"], lam=b["lam"], materials=b["materials"], dminv=b["dminv"],
+                                  n_verts=S.n_verts, n_tets=S.n_tets, n_tris=S.n_tris, shapes=shapes,
+                                  out=self._ref_out, **self._refine_kw)
+            res.update(r)
+            if r["new_verts"] > 0 and not r["dropped"]:
+                o, nt, nf = self._ref_out, r["n_tets"], r["n_tris"]
+                b["tets"][:nt] = o["tets"][:nt]
+                b["s"][:nt] = o["stretch"][:nt]
+                b["lam"][:nt] = o["lam"][:nt]
+                b["materials"][:nt] = o["materials"][:nt]
+                b["dminv"][:nt] = o["dminv"][:nt]
+                b["vol"][:nt] = (1.0 / (6.0 * torch.linalg.det(o["dminv"][:nt].double().view(-1, 3, 3)))).float()
+                if nf:
+                    b["tris"][:nf] = o["tris"][:nf]
+                nv0 = self.active_particle_count
+                b["dx"][nv0:r["n_verts"]] = 0.0
+                self._set_topology(r["n_verts"], nt, nf)
+                changed = True
+        if changed or counter == 0:
+            self._refresh_masses()
+        self.refine_log.append(res)
+        return res
 
     # -- plumbing ---------------------------------------------------------
     def _stream(self):
@@ -286,6 +416,8 @@ class RefinementSolver:
             state_out.assign(state_in)
         self._set_body_q(state_out.body_q)
         self._bind_state(state_out.particle_q, state_out.particle_qd)
+        if self._refine_enabled:
+            self._refine_pass(state_out)
         self._call("mfem_step", ctypes.c_float(dt), self._stream())
 
     def step_instrumented(self, state_in: State, state_out: State, dt: float, timers: dict) -> None:
@@ -297,6 +429,8 @@ class RefinementSolver:
             state_out.assign(state_in)
         self._set_body_q(state_out.body_q)
         self._bind_state(state_out.particle_q, state_out.particle_qd)
+        if self._refine_enabled:
+            self._refine_pass(state_out)
 
         def timed(name, fn, *args):
             a = torch.cuda.Event(enable_timing=True)
@@ -344,7 +478,7 @@ class RefinementSolver:
         self.ws_features(dt, newton_iter)
         b = self.buffers
         _ffi_mlp = self.lib.mfem_mlp_forward
-        rc = _ffi_mlp(ctypes.c_void_p(b["ws_x"].data_ptr()), ctypes.c_int(self.max_particles),
+        rc = _ffi_mlp(ctypes.c_void_p(b["ws_x"].data_ptr()), ctypes.c_int(self.active_particle_count),
                       ctypes.c_void_p(net.wbf.data_ptr()), ctypes.c_void_p(net.w.data_ptr()),
                       ctypes.c_void_p(b["ws_y"].data_ptr()), self._stream())
         if rc != 0:
@@ -426,20 +560,32 @@ class RefinementSolver:
         import scipy.sparse as sp
         from .pattern import sell_to_csr_values
         vals = sell_to_csr_values(self.pattern, self.buffers["sell_val"].cpu().numpy())
-        n = self.max_particles
+        n = self.active_particle_count
         return sp.bsr_matrix((vals, self.pattern.cols, self.pattern.rowptr), shape=(3 * n, 3 * n))
 
     @property
     def tet_stretch(self) -> torch.Tensor:
-        return self.buffers["s"]
+        return self.buffers["s"][: self.active_tet_count]
 
     @property
     def tet_lambda(self) -> torch.Tensor:
-        return self.buffers["lam"]
+        return self.buffers["lam"][: self.active_tet_count]
 
     @property
     def elastic_energy(self) -> torch.Tensor:
-        return self.buffers["psi"]
+        return self.buffers["psi"][: self.active_tet_count]
+
+    @property
+    def tet_indices(self) -> torch.Tensor:
+        return self.buffers["tets"][: self.active_tet_count]
+
+    @property
+    def tri_indices(self) -> torch.Tensor:
+        return self.buffers["tris"][: self.active_tri_count]
+
+    @property
+    def rest_particle_q(self) -> torch.Tensor:
+        return self.buffers["rest"][: self.active_particle_count]
 
     def write_timings(self, filename: str) -> None:
         with open(filename, "w") as f:
