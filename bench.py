@@ -60,6 +60,7 @@ def main():
     ap.add_argument("--dump-iters", default=None,
                     help="write the CG iteration count of every PCG launch, in launch order, to this JSON file "
                          "(to divide per-dispatch hardware counters by iterations)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default PCG variant choice")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run five 1 GiB device-to-device copies (counter calibration) and exit")
     args = ap.parse_args()
@@ -109,6 +110,13 @@ def main():
 
     from mfem_amd import _ffi
     from mfem_amd.learned import WarmStartNet, collect_solves
+
+    tune = None
+    if not args.no_autotune:
+        tune = solver.autotune(state, dt)
+        c = tune["chosen"]
+        print(f"[bench] autotuned PCG: variant {c['variant']} ({c['chunks_per_wave']} chunk(s)/wave, "
+              f"{c['resident_slots']} resident slots, {c['mode']}), {c['us_per_iter']:.2f} us/iter", flush=True)
 
     learned = not args.no_learned
     train_info = None
@@ -223,6 +231,11 @@ def main():
         "pcg_phase_us_per_iter": pcg_phase_us,
         "phase_ms_per_step": {k: v / args.steps for k, v in phase_ms.items()},
     }
+    if tune is not None:
+        config["pcg_autotune"] = {
+            "chosen": tune["chosen"]["variant"],
+            "table": [{k: r[k] for k in ("variant", "chunks_per_wave", "resident_slots", "mode", "workgroups",
+                                         "us_per_iter", "cg_iters")} for r in tune["table"]]}
     if learned:
         config["learned_guess_used_frac"] = float(st[:, :, 14].mean())
         config["learned_guess_residual_ratio_median"] = float(np.median(st[:, :, 15]))

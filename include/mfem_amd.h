@@ -95,7 +95,7 @@ typedef struct mfem_solver {
   float gravity[3]; /* model.gravity */
   int32_t max_width; /* widest chunk, in slots (max row length of the pattern) */
   int32_t pcg_flags; /* MFEM_PCG_* bits */
-  int32_t pad1;
+  int32_t pcg_variant; /* -1: chosen from pcg_flags; k >= 0: variant k (mfem_pcg_variant_info), set by an autotuner */
   /* model / topology (device) */
   const int32_t* tets;      /* [n_tets][4] */
   const float* dminv;       /* [n_tets][9] tet_poses, row-major */
@@ -354,6 +354,13 @@ int mfem_refine_count(const mfem_refine_args* args, void* workspace, size_t work
 int mfem_refine_apply(const mfem_refine_args* args, void* workspace, size_t workspace_bytes,
                       const mfem_refine_out* out, void* stream);
 const char* mfem_refine_last_error(void);
+
+/* Persistent-PCG variants for an autotuner, no counterpart in the reference:
+   info[0..4] = chunks per wave, resident row slots (0 = streamed rows),
+   mode (0 standard, 1 pipelined), workgroups for S, applicable to S (0/1).
+   Returns MFEM_EINVAL when k >= mfem_pcg_variant_count(). */
+int mfem_pcg_variant_count(void);
+int mfem_pcg_variant_info(const mfem_solver* S, int k, int32_t* info);
 
 /* y = H x with the assembled SELL matrix (warp.sparse.bsr_mv). */
 int mfem_spmv(const mfem_solver* S, const float* x, float* y, void* stream);

@@ -1666,7 +1666,7 @@ int launched() {
 #define FIELD_TABLE(X)                                                                                   \
   X(n_verts) X(n_tets) X(n_tris) X(n_shapes) X(n_chunks) X(n_slots) X(energy) X(iterations) X(cg_maxiter) \
   X(precond) X(line_search) X(ls_max) X(tri_contact) X(friction_on) X(cg_tol) X(ls_alpha0) X(ls_tau)      \
-  X(ls_c) X(ls_thr) X(sing_thr) X(attach_k) X(d0) X(d1) X(kappa) X(eps_v) X(qb) X(gravity) X(max_width) X(pcg_flags) X(pad1) X(tets) \
+  X(ls_c) X(ls_thr) X(sing_thr) X(attach_k) X(d0) X(d1) X(kappa) X(eps_v) X(qb) X(gravity) X(max_width) X(pcg_flags) X(pcg_variant) X(tets) \
   X(dminv) X(materials) X(vol) X(mass) X(inv_mass) X(pin_pos) X(tris) X(shape_type) X(shape_xf)            \
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
@@ -1738,7 +1738,14 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
   if (int rc = coop_info(&ci)) return rc;
   int sel = -1;
   unsigned nblocks = 0;
-  for (int k = 0; k < kNumPcgVariants; ++k) {
+  if (S->pcg_variant >= 0) {  // forced (autotuner)
+    int32_t info[5];
+    if (int rc = mfem_pcg_variant_info(S, S->pcg_variant, info)) return rc;
+    if (!info[4]) return fail(MFEM_EINVAL, "pcg_variant does not apply to this system");
+    sel = S->pcg_variant;
+    nblocks = (unsigned)info[3];
+  }
+  for (int k = 0; k < kNumPcgVariants && sel < 0; ++k) {
     const PcgVariant& v = kPcgVariants[k];
     const int want = (S->pcg_flags & MFEM_PCG_PIPELINED) ? 1 : 0;
     if (v.mode != want) continue;
@@ -1808,6 +1815,23 @@ int mfem_step(const mfem_solver* S, float dt, void* stream) {
     if (int rc = mfem_line_search(S, dt, it, stream)) return rc;
   }
   return mfem_end_step(S, dt, stream);
+}
+
+int mfem_pcg_variant_count(void) { return kNumPcgVariants; }
+
+int mfem_pcg_variant_info(const mfem_solver* S, int k, int32_t* info) {
+  if (!S || !info || k < 0 || k >= kNumPcgVariants) return fail(MFEM_EINVAL, "bad PCG variant query");
+  CoopInfo* ci = nullptr;
+  if (int rc = coop_info(&ci)) return rc;
+  const PcgVariant& v = kPcgVariants[k];
+  const long waves = (S->n_chunks + v.cpw - 1) / v.cpw;
+  const long nb = (waves + kWaves - 1) / kWaves;
+  info[0] = v.cpw;
+  info[1] = v.rw;
+  info[2] = v.mode;
+  info[3] = (int32_t)nb;
+  info[4] = (nb <= ci->max_blocks_pcg[k] && nb <= MFEM_RED_MAX_BLOCKS && (v.rw == 0 || S->max_width <= v.rw)) ? 1 : 0;
+  return MFEM_OK;
 }
 
 int mfem_ws_features(const mfem_solver* S, float dt, int newton_iter, void* stream) {

@@ -21,7 +21,7 @@ FIELDS = [
     ("line_search", c_i32), ("ls_max", c_i32), ("tri_contact", c_i32), ("friction_on", c_i32),
     ("cg_tol", c_f32), ("ls_alpha0", c_f32), ("ls_tau", c_f32), ("ls_c", c_f32), ("ls_thr", c_f32),
     ("sing_thr", c_f32), ("attach_k", c_f32), ("d0", c_f32), ("d1", c_f32), ("kappa", c_f32), ("eps_v", c_f32),
-    ("qb", c_f32 * 3), ("gravity", c_f32 * 3), ("max_width", c_i32), ("pcg_flags", c_i32), ("pad1", c_i32),
+    ("qb", c_f32 * 3), ("gravity", c_f32 * 3), ("max_width", c_i32), ("pcg_flags", c_i32), ("pcg_variant", c_i32),
 ] + [(n, c_ptr) for n in (
     "tets", "dminv", "materials", "vol", "mass", "inv_mass", "pin_pos", "tris", "shape_type", "shape_xf",
     "shape_scale", "shape_mu", "chunk_ptr", "sell_col", "sell_val", "diag_lane", "contrib_ptr", "contrib_src",
@@ -53,7 +53,7 @@ ENTRY_POINTS = [
     "mfem_scan_i32", "mfem_scan_workspace_bytes", "mfem_broadphase_pt", "mfem_broadphase_ee",
     "mfem_broadphase_workspace_bytes", "mfem_soft_rigid_distance", "mfem_pair_barrier", "mfem_collide_last_error",
     "mfem_refine_workspace_bytes", "mfem_refine_select", "mfem_refine_count", "mfem_refine_apply",
-    "mfem_refine_last_error",
+    "mfem_refine_last_error", "mfem_pcg_variant_count", "mfem_pcg_variant_info",
 ]
 
 
@@ -115,6 +115,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mfem_refine_count.argtypes = [vp, vp, sz, vp, vp]
     lib.mfem_refine_apply.argtypes = [vp, vp, sz, vp, vp]
     lib.mfem_refine_last_error.restype = ctypes.c_char_p
+    lib.mfem_pcg_variant_count.argtypes = []
+    lib.mfem_pcg_variant_info.argtypes = [P, i, vp]
     _lib = lib
     return lib
 

@@ -216,6 +216,7 @@ class RefinementSolver:
         # solver-specific knobs (not reference kwargs): pipelined PCG, resident rows
         S.pcg_flags = (_ffi.PCG_PIPELINED if kwargs.get("pcg_pipelined", False) else 0) | \
                       (0 if kwargs.get("pcg_resident_rows", True) else _ffi.PCG_STREAM_ROWS)
+        S.pcg_variant = -1  # chosen from pcg_flags until autotune() picks one
         S.energy = _ENERGIES[energy]
         S.iterations = self.iterations
         S.cg_maxiter = int(kwargs.get("cg_max_iterations", 1000))
@@ -286,6 +287,7 @@ class RefinementSolver:
         S.n_chunks, S.n_slots = pat.n_chunks, pat.n_slots
         S.max_width = int(np.diff(pat.chunk_ptr).max())
         S.tri_contact = int(self._tri_contact_kw and n_tris > 0)
+        S.pcg_variant = -1  # block counts change with the topology: back to the automatic choice
         for name in ("chunk_ptr", "sell_col", "sell_val", "diag_lane", "contrib_ptr", "contrib_src", "vinc_ptr",
                      "vinc_src", "pz", "px", "tcon_ptr", "tcon_src", "vtri_ptr", "vtri_src"):
             setattr(S, name, b[name].data_ptr())
@@ -532,6 +534,69 @@ class RefinementSolver:
         self._call("mfem_merit", ctypes.c_float(dt), ctypes.c_float(alpha), ctypes.c_void_p(out.data_ptr()),
                    self._stream())
         return float(out.item())
+
+    # autotuning and graph capture ----------------------------------------------
+    def pcg_variants(self) -> list[dict]:
+        """The persistent-PCG variants and whether each applies to the
+        current system (chunks per wave, resident row slots, mode,
+        workgroups)."""
+        n = self.lib.mfem_pcg_variant_count()
+        out = []
+        info = (ctypes.c_int32 * 5)()
+        for k in range(n):
+            _ffi.call("mfem_pcg_variant_info", ctypes.byref(self._S), ctypes.c_int(k), info)
+            out.append(dict(variant=k, chunks_per_wave=info[0], resident_slots=info[1],
+                            mode="pipelined" if info[2] else "standard", workgroups=info[3], applicable=bool(info[4])))
+        return out
+
+    def autotune(self, state: "State", dt: float, repeats: int = 3) -> dict:
+        """Times every applicable PCG variant on the system assembled at
+        ``state`` (one Newton iteration's matrix) and keeps the fastest per
+        solve.  Leaves the state unchanged; the choice holds until the
+        topology changes (refinement resets it)."""
+        b = self.buffers
+        saved = {k: b[k].clone() for k in ("dx", "stats", "x_start", "x_tilde", "lag_n", "lag_fn", "lag_sid")}
+        self.bind(state)
+        self.begin_step(dt)
+        self.update_system(dt)
+        self.assemble(dt)
+        table = []
+        for v in self.pcg_variants():
+            if not v["applicable"]:
+                continue
+            self._S.pcg_variant = v["variant"]
+            self.global_solve(0, learned=False)  # warm-up (code objects, caches)
+            ms = []
+            for _ in range(repeats):
+                a = torch.cuda.Event(enable_timing=True)
+                e = torch.cuda.Event(enable_timing=True)
+                a.record()
+                self.global_solve(0, learned=False)
+                e.record()
+                torch.cuda.synchronize(self.device)
+                ms.append(a.elapsed_time(e))
+            st = self.last_stats()[0]
+            table.append(dict(v, ms_per_solve=float(np.median(ms)), cg_iters=int(st[0]),
+                              us_per_iter=float(np.median(ms)) * 1e3 / max(int(st[0]), 1), relres=float(st[1])))
+        best = min(table, key=lambda r: r["ms_per_solve"])
+        self._S.pcg_variant = best["variant"]
+        for k, t in saved.items():
+            b[k].copy_(t)
+        return dict(chosen=best, table=table)
+
+    def capture_step(self, state_in: "State", state_out: "State", dt: float):
+        """A HIP graph of ``step(state_in, state_out, ...)`` on these exact
+        tensors (refinement must be off: a pass synchronises with the host).
+        Replay with ``graph.replay()``; the caller alternates two graphs
+        (a->b, b->a) to advance."""
+        if self._refine_enabled:
+            raise RuntimeError("capture_step needs enable_refinement=False")
+        self.step(state_in, state_out, None, None, dt)  # warm-up outside the capture
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.step(state_in, state_out, None, None, dt)
+        return g
 
     # diagnostics ------------------------------------------------------------
     def last_stats(self) -> np.ndarray:
