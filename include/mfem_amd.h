@@ -151,7 +151,7 @@ typedef struct mfem_solver {
   /* persistent-kernel scratch */
   float* pz;    /* [2][n_chunks*64][3] published PCG vector (double-buffered) */
   double* px;   /* [2][n_chunks*64][3] published FP64 PCG vector (residual checks, pipelined variant) */
-  double* red;  /* [2][MFEM_RED_MAX_BLOCKS][4] grid reduction partials */
+  double* red;  /* [2][MFEM_RED_MAX_BLOCKS][8] 64-bit words: epoch-tagged halves of the grid reduction partials */
   uint32_t* flags; /* [MFEM_RED_MAX_BLOCKS] grid barrier arrival epochs */
   int32_t* status; /* [8] see MFEM_STATUS_* */
   double* stats;   /* [MFEM_MAX_STATS_ITERS][MFEM_STATS_PER_ITER] per Newton iteration */

@@ -531,18 +531,19 @@ __global__ void __launch_bounds__(kBlock) k_rhs(mfem_solver S, float dt) {
 // ---------------------------------------------------------------------------
 // grid-wide reduction + barrier for the persistent kernels
 //
-// Cross-workgroup data (published vectors, partial sums, arrival epochs) is
-// written and read with agent-scope coherent accesses (sc1: write-through to
-// memory on store, no stale L1/L2 hit on load).  Arrival protocol: every wave
-// waits for its coherent stores to complete (s_waitcnt vmcnt(0)), the
-// workgroup synchronises, thread 0 stores its FP64 partials, waits again and
-// stores its arrival epoch.  Every workgroup polls all epochs, then sums the
-// partials in one fixed order, so every workgroup obtains the bit-identical
-// total and takes the same branch.  This avoids agent-scope release/acquire
-// fences, whose L2 write-back / invalidate cost ~13 us per barrier at ~180
-// workgroups on MI355X (vs ~4 us for this protocol).  Partials ping-pong
-// between two slots by epoch parity.  Waits are bounded (kBarrierTimeout): on
-// timeout the error bit is set and every workgroup leaves the kernel.
+// Cross-workgroup data (published vectors, partial sums) is written and read
+// with agent-scope coherent accesses (sc1: write-through to memory on store,
+// no stale L1/L2 hit on load).  Arrival protocol: every wave waits for its
+// coherent stores to complete (s_waitcnt vmcnt(0)), the workgroup
+// synchronises, and 2 * NV threads store the workgroup's FP64 partials as
+// epoch-tagged 64-bit halves.  Every workgroup polls all tagged partials
+// (arrival and value in one load), then sums them in one fixed order, so
+// every workgroup obtains the bit-identical total and takes the same branch.
+// This avoids agent-scope release/acquire fences, whose L2 write-back /
+// invalidate cost ~13 us per barrier at ~180 workgroups on MI355X.  Partials
+// ping-pong between two slots by epoch parity.  Waits are bounded
+// (kBarrierTimeout): on timeout the error bit is set and every workgroup
+// leaves the kernel.
 // ---------------------------------------------------------------------------
 struct GridSync {
   double* red;
@@ -589,35 +590,57 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
   const uint32_t ep = gs.epoch;
   const int slot = ep & 1u;
   const int nb = gridDim.x;
-  if (threadIdx.x == 0) {
+  // Partials are published as 64-bit words (epoch << 32 | one half of the
+  // FP64 value): the value arrives together with the arrival signal, so
+  // readers poll the partials themselves (no separate flag store, no
+  // second round trip to read them).  A word is single-copy atomic, so a
+  // matching tag means a matching half.  A slot is rewritten only two
+  // epochs later, after every workgroup has passed the next barrier, so the
+  // tag of the slot being read is either ep - 2 (not yet written) or ep.
+  uint64_t* words = reinterpret_cast<uint64_t*>(gs.red);
+  if (threadIdx.x < 2 * NV) {
+    const int i = threadIdx.x >> 1;
+    double acc = sm[0][i];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      double acc = sm[0][i];
-#pragma unroll
-      for (int w = 1; w < kWaves; ++w) acc += sm[w][i];
-      pub_store(&gs.red[((size_t)slot * MFEM_RED_MAX_BLOCKS + blockIdx.x) * 4 + i], acc);
-    }
-    wait_vmem();
-    pub_store(&gs.flags[blockIdx.x], ep);
+    for (int w = 1; w < kWaves; ++w) acc += sm[w][i];
+    const uint64_t bits = (uint64_t)__double_as_longlong(acc);
+    const uint32_t half = (threadIdx.x & 1) ? (uint32_t)(bits >> 32) : (uint32_t)bits;
+    pub_store(&words[((size_t)slot * MFEM_RED_MAX_BLOCKS + blockIdx.x) * 8 + threadIdx.x],
+              ((uint64_t)ep << 32) | half);
   }
-  // wait until every workgroup has arrived.  A workgroup that has passed this
-  // barrier may already have stored ep + 1 at the next one before this wave
-  // polls again, so the test is "flag >= ep" (wrap-safe), never "flag == ep".
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) acc[i] = 0.0;
   bool to = false;
   for (int b = threadIdx.x; b < nb && !to; b += kBlock) {
-    if ((int32_t)(pub_load(&gs.flags[b]) - ep) >= 0) continue;
-    const uint64_t t0 = wall_clock64();
-    while ((int32_t)(pub_load(&gs.flags[b]) - ep) < 0) {
-      if (wall_clock64() - t0 > kBarrierTimeout || pub_load(&gs.status[MFEM_STATUS_ERRBITS]) != 0) {
-        to = true;
-        break;
+    const uint64_t* src = words + ((size_t)slot * MFEM_RED_MAX_BLOCKS + b) * 8;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      uint64_t lo = pub_load(&src[2 * i]), hi = pub_load(&src[2 * i + 1]);
+      if ((uint32_t)(lo >> 32) != ep || (uint32_t)(hi >> 32) != ep) {
+        const uint64_t t0 = wall_clock64();
+        while ((uint32_t)(lo >> 32) != ep || (uint32_t)(hi >> 32) != ep) {
+          if (wall_clock64() - t0 > kBarrierTimeout || pub_load(&gs.status[MFEM_STATUS_ERRBITS]) != 0) {
+            to = true;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          lo = pub_load(&src[2 * i]);
+          hi = pub_load(&src[2 * i + 1]);
+        }
+        if (to) break;
       }
-      __builtin_amdgcn_s_sleep(1);
+      acc[i] += __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
     }
   }
   if (to) {
     atomicOr(&gs.status[MFEM_STATUS_ERRBITS], 1);
     abort_flag = 1;
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const double w = wave_sum(acc[i]);
+    if (lane == 0) tot[wave][i] = w;
   }
   __syncthreads();
   if (abort_flag) {
@@ -626,18 +649,10 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
   }
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) acc += pub_load(&gs.red[((size_t)slot * MFEM_RED_MAX_BLOCKS + b) * 4 + i]);
-    acc = wave_sum(acc);
-    if (lane == 0) tot[wave][i] = acc;
-  }
-  __syncthreads();
+    double a = tot[0][i];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double acc = tot[0][i];
-#pragma unroll
-    for (int w = 1; w < kWaves; ++w) acc += tot[w][i];
-    v[i] = acc;
+    for (int w = 1; w < kWaves; ++w) a += tot[w][i];
+    v[i] = a;
   }
   __syncthreads();
   return true;

@@ -185,7 +185,7 @@ class RefinementSolver:
             b[n] = zf(Vc)
         b["dx"] = torch.zeros(Vc, 3, dtype=torch.float64, device=dev)
         b["ds"] = zf(Tc, 6)
-        b["red"] = torch.zeros(2 * _ffi.MFEM_RED_MAX_BLOCKS * 4, dtype=torch.float64, device=dev)
+        b["red"] = torch.zeros(2 * _ffi.MFEM_RED_MAX_BLOCKS * 8, dtype=torch.float64, device=dev)
         b["flags"] = torch.zeros(_ffi.MFEM_RED_MAX_BLOCKS, dtype=torch.int32, device=dev)
         b["status"] = torch.zeros(8, dtype=torch.int32, device=dev)
         b["stats"] = torch.zeros(_ffi.MFEM_MAX_STATS_ITERS * _ffi.MFEM_STATS_PER_ITER, dtype=torch.float64, device=dev)
@@ -614,10 +614,12 @@ class RefinementSolver:
         return int(self.buffers["status"][0].item())
 
     def reset_status(self) -> None:
-        """Clears the error bits, the barrier epoch and the arrival flags."""
+        """Clears the error bits, the barrier epoch and the epoch-tagged
+        reduction partials (their tags restart with the epoch)."""
         torch.cuda.synchronize(self.device)
         self.buffers["status"].zero_()
         self.buffers["flags"].zero_()
+        self.buffers["red"].zero_()
         torch.cuda.synchronize(self.device)
 
     def system_csr(self):
