@@ -29,9 +29,10 @@ def test_autotune_keeps_results(gpu):
     solver, state = _make(m, gpu)
     res = solver.autotune(state, 0.01)
     print("autotune:", [(r["variant"], r["chunks_per_wave"], r["resident_slots"], r["mode"],
-                         round(r["us_per_iter"], 2), r["cg_iters"]) for r in res["table"]], "chosen", res["chosen"]["variant"])
+                         round(r["ms_per_step"], 3), r["cg_iters"]) for r in res["table"]], "chosen", res["chosen"]["variant"])
     assert len(res["table"]) >= 2
-    assert all(r["relres"] <= 1e-6 for r in res["table"])
+    # the state is untouched by tuning
+    np.testing.assert_array_equal(state.particle_q.cpu().numpy(), m.particle_q.astype(np.float32))
     o = Oracle(m, 64)
     out = state.clone()
     for k in range(4):

@@ -1616,7 +1616,7 @@ struct PcgVariant {
 // stagnates near rel-res 1e-3 on the contact scenes and is not instantiated.
 const PcgVariant kPcgVariants[] = {
     PCG_V(1, 16, 0), PCG_V(1, 0, 0), PCG_V(2, 0, 0), PCG_V(4, 0, 0), PCG_V(8, 0, 0),
-    PCG_V(1, 0, 1),  PCG_V(2, 0, 1), PCG_V(4, 0, 1), PCG_V(8, 0, 1),
+    PCG_V(1, 16, 1), PCG_V(1, 0, 1), PCG_V(2, 0, 1), PCG_V(4, 0, 1), PCG_V(8, 0, 1),
 };
 #undef PCG_V
 constexpr int kNumPcgVariants = sizeof(kPcgVariants) / sizeof(kPcgVariants[0]);

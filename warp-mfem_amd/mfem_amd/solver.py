@@ -549,39 +549,50 @@ class RefinementSolver:
                             mode="pipelined" if info[2] else "standard", workgroups=info[3], applicable=bool(info[4])))
         return out
 
-    def autotune(self, state: "State", dt: float, repeats: int = 3) -> dict:
-        """Times every applicable PCG variant on the system assembled at
-        ``state`` (one Newton iteration's matrix) and keeps the fastest per
-        solve.  Leaves the state unchanged; the choice holds until the
-        topology changes (refinement resets it)."""
+    def autotune(self, state: "State", dt: float, repeats: int = 2) -> dict:
+        """Times every applicable PCG variant on whole steps from ``state``
+        (all Newton iterations, with the warm start if one is attached: the
+        variants differ in iteration counts as well as in cost per
+        iteration) and keeps the fastest.  The state and the solver's
+        per-tet / per-vertex state are restored afterwards; refinement passes
+        are not run while tuning.  The choice holds until the topology
+        changes."""
         b = self.buffers
-        saved = {k: b[k].clone() for k in ("dx", "stats", "x_start", "x_tilde", "lag_n", "lag_fn", "lag_sid")}
-        self.bind(state)
-        self.begin_step(dt)
-        self.update_system(dt)
-        self.assemble(dt)
+        keys = ("s", "lam", "dx", "stats", "x_start", "x_tilde", "lag_n", "lag_fn", "lag_sid")
+        saved = {k: b[k].clone() for k in keys}
+        scratch = state.clone()
+        refine, self._refine_enabled = self._refine_enabled, False
+        step, self._step = self._step, self._step
         table = []
-        for v in self.pcg_variants():
-            if not v["applicable"]:
-                continue
-            self._S.pcg_variant = v["variant"]
-            self.global_solve(0, learned=False)  # warm-up (code objects, caches)
-            ms = []
-            for _ in range(repeats):
-                a = torch.cuda.Event(enable_timing=True)
-                e = torch.cuda.Event(enable_timing=True)
-                a.record()
-                self.global_solve(0, learned=False)
-                e.record()
-                torch.cuda.synchronize(self.device)
-                ms.append(a.elapsed_time(e))
-            st = self.last_stats()[0]
-            table.append(dict(v, ms_per_solve=float(np.median(ms)), cg_iters=int(st[0]),
-                              us_per_iter=float(np.median(ms)) * 1e3 / max(int(st[0]), 1), relres=float(st[1])))
-        best = min(table, key=lambda r: r["ms_per_solve"])
+        try:
+            for v in self.pcg_variants():
+                if not v["applicable"]:
+                    continue
+                self._S.pcg_variant = v["variant"]
+                ms, iters = [], []
+                for r in range(repeats + 1):  # the first run warms up
+                    for k in keys:
+                        b[k].copy_(saved[k])
+                    a = torch.cuda.Event(enable_timing=True)
+                    e = torch.cuda.Event(enable_timing=True)
+                    a.record()
+                    self.step(state, scratch, None, None, dt)
+                    e.record()
+                    torch.cuda.synchronize(self.device)
+                    if self.status() != 0:
+                        raise RuntimeError("device error while autotuning")
+                    if r > 0:
+                        ms.append(a.elapsed_time(e))
+                        iters.append(int(self.last_stats()[:, 0].sum()))
+                table.append(dict(v, ms_per_step=float(np.median(ms)), cg_iters=int(np.median(iters)),
+                                  us_per_iter=float(np.median(ms)) * 1e3 / max(int(np.median(iters)), 1)))
+        finally:
+            self._refine_enabled = refine
+            self._step = step
+            for k in keys:
+                b[k].copy_(saved[k])
+        best = min(table, key=lambda r: r["ms_per_step"])
         self._S.pcg_variant = best["variant"]
-        for k, t in saved.items():
-            b[k].copy_(t)
         return dict(chosen=best, table=table)
 
     def capture_step(self, state_in: "State", state_out: "State", dt: float):
