@@ -151,7 +151,8 @@ typedef struct mfem_solver {
   /* persistent-kernel scratch */
   float* pz;    /* [2][n_chunks*64][3] published PCG vector (double-buffered) */
   double* px;   /* [2][n_chunks*64][3] published FP64 PCG vector (residual checks, pipelined variant) */
-  double* red;  /* [2][MFEM_RED_MAX_BLOCKS][8] 64-bit words: epoch-tagged halves of the grid reduction partials */
+  double* red;  /* [2 kernels][2][MFEM_RED_MAX_BLOCKS][8] 64-bit words: epoch-tagged halves of the grid
+                   reduction partials (PCG region, line-search region) */
   uint32_t* flags; /* [MFEM_RED_MAX_BLOCKS] grid barrier arrival epochs */
   int32_t* status; /* [8] see MFEM_STATUS_* */
   double* stats;   /* [MFEM_MAX_STATS_ITERS][MFEM_STATS_PER_ITER] per Newton iteration */
@@ -171,6 +172,21 @@ typedef struct mfem_solver {
   float* tri_bary;         /* [n_tris][3] barycentric weights of the closest point */
   float* tri_nrm;          /* [n_tris][3] contact normal (capsule -> triangle) */
   int32_t* tri_sid;        /* [n_tris] shape id, -1 when no capsule */
+  /* batches of independent bodies (n_bodies > 1): body b owns vertices
+     [body_vert[b], body_vert[b+1]) starting at a multiple of 256 (whole PCG
+     workgroups of 4 chunks), tets [body_tet[b], ...) and triangles
+     [body_tri[b], ...); each body has its own CG scalars, convergence,
+     line search and stats ([n_bodies][MFEM_MAX_STATS_ITERS][MFEM_STATS_PER_ITER]).
+     No counterpart in the reference (one body per solver there). */
+  int32_t n_bodies;          /* 0 or 1: one body */
+  int32_t ls_blocks;         /* line-search workgroups when n_bodies > 1 */
+  const int32_t* body_wg;    /* [n_bodies + 1] PCG workgroup ranges (chunk ranges / 4) */
+  const int32_t* body_vert;  /* [n_bodies + 1] */
+  const int32_t* body_tet;   /* [n_bodies + 1] */
+  const int32_t* body_tri;   /* [n_bodies + 1] */
+  const int32_t* body_ls_wg; /* [n_bodies + 1] line-search workgroup ranges */
+  int32_t* body_epoch;       /* [n_bodies] grid-barrier epochs */
+  const int32_t* tet_energy; /* [n_tets] per-tet energy (0 ARAP, 1 Neo-Hookean) or null: `energy` for all */
 } mfem_solver;
 
 /* ABI self-description (checked against the Python mirror by the tests) */
@@ -360,6 +376,9 @@ const char* mfem_refine_last_error(void);
    mode (0 standard, 1 pipelined), workgroups for S, applicable to S (0/1).
    Returns MFEM_EINVAL when k >= mfem_pcg_variant_count(). */
 int mfem_pcg_variant_count(void);
+/* Co-resident workgroups of the persistent line-search kernel (the bound
+   for ls_blocks of a batch). */
+int mfem_line_search_max_blocks(void);
 int mfem_pcg_variant_info(const mfem_solver* S, int k, int32_t* info);
 
 /* y = H x with the assembled SELL matrix (warp.sparse.bsr_mv). */

@@ -372,7 +372,7 @@ __global__ void __launch_bounds__(kBlock) k_element(mfem_solver S) {
 
   // elastic energy, gradient, (PSD) Hessian and inverse in stretch space
   float gs[6], H[36], Hi[36], psi;
-  if (S.energy == 0) {
+  if ((S.tet_energy ? S.tet_energy[e] : S.energy) == 0) {
     psi = arap_energy(s, mu, vol);
     arap_grad_hess(s, mu, vol, gs, H, Hi);
   } else {
@@ -551,7 +551,21 @@ struct GridSync {
   int32_t* status;
   uint32_t epoch;
   bool abort;
+  int wg0, wg1;   // the workgroups that take part (one body's)
+  size_t region;  // word offset of this kernel's partials in red
 };
+
+// largest b in [0, n) with ptr[b] <= x (ptr ascending, ptr[0] == 0)
+__device__ __forceinline__ int body_of(const int32_t* ptr, int n, int x) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ptr[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+constexpr size_t kRedRegionWords = (size_t)2 * MFEM_RED_MAX_BLOCKS * 8;
 
 // s_waitcnt vmcnt(0): all of this wave's memory operations have completed
 __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
@@ -589,7 +603,6 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
   gs.epoch += 1u;
   const uint32_t ep = gs.epoch;
   const int slot = ep & 1u;
-  const int nb = gridDim.x;
   // Partials are published as 64-bit words (epoch << 32 | one half of the
   // FP64 value): the value arrives together with the arrival signal, so
   // readers poll the partials themselves (no separate flag store, no
@@ -597,7 +610,7 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
   // matching tag means a matching half.  A slot is rewritten only two
   // epochs later, after every workgroup has passed the next barrier, so the
   // tag of the slot being read is either ep - 2 (not yet written) or ep.
-  uint64_t* words = reinterpret_cast<uint64_t*>(gs.red);
+  uint64_t* words = reinterpret_cast<uint64_t*>(gs.red) + gs.region;
   if (threadIdx.x < 2 * NV) {
     const int i = threadIdx.x >> 1;
     double acc = sm[0][i];
@@ -612,7 +625,7 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
 #pragma unroll
   for (int i = 0; i < NV; ++i) acc[i] = 0.0;
   bool to = false;
-  for (int b = threadIdx.x; b < nb && !to; b += kBlock) {
+  for (int b = gs.wg0 + threadIdx.x; b < gs.wg1 && !to; b += kBlock) {
     const uint64_t* src = words + ((size_t)slot * MFEM_RED_MAX_BLOCKS + b) * 8;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
@@ -790,7 +803,13 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
   // a barrier timed out earlier: flags and epoch are inconsistent until the
   // host clears status and flags, so do nothing
   if (S.status[MFEM_STATUS_ERRBITS] != 0) return;
-  GridSync gs{S.red, S.flags, S.status, (uint32_t)S.status[MFEM_STATUS_EPOCH], false};
+  // one body: all workgroups; a batch: the body's workgroups (chunks of a
+  // body are whole workgroups, one chunk per wave), its own epoch and stats
+  const int body = S.n_bodies > 1 ? body_of(S.body_wg, S.n_bodies, blockIdx.x) : 0;
+  const int wg0 = S.n_bodies > 1 ? S.body_wg[body] : 0;
+  const int wg1 = S.n_bodies > 1 ? S.body_wg[body + 1] : (int)gridDim.x;
+  int32_t* ep_ptr = S.n_bodies > 1 ? S.body_epoch + body : S.status + MFEM_STATUS_EPOCH;
+  GridSync gs{S.red, S.flags, S.status, (uint32_t)*ep_ptr, false, wg0, wg1, 0};
   const int lane = threadIdx.x & 63;
   const int gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
   const int nw = gridDim.x * kWaves;
@@ -861,7 +880,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
     }
   }
   // device-clock profile of the solve phases (wave 0 of workgroup 0 only)
-  const bool prof = blockIdx.x == 0 && threadIdx.x < 64;
+  const bool prof = blockIdx.x == wg0 && threadIdx.x < 64;
   uint64_t tk[6] = {0, 0, 0, 0, 0, 0};
   uint64_t tprev = wall_clock64();
 #define PCG_LAP(k)                        \
@@ -1206,9 +1225,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
 #pragma unroll
     for (int a = 0; a < 3; ++a) S.dx[3 * row + a] = bnorm2 > 0.0 ? x[k][a] : 0.0;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == wg0 && threadIdx.x == 0) {
     if (newton_iter >= 0 && newton_iter < MFEM_MAX_STATS_ITERS) {
-      double* st = S.stats + newton_iter * MFEM_STATS_PER_ITER;
+      double* st = S.stats + ((size_t)body * MFEM_MAX_STATS_ITERS + newton_iter) * MFEM_STATS_PER_ITER;
       st[0] = (double)it;
       st[1] = bnorm2 > 0.0 ? sqrt(rt2 / bnorm2) : 0.0;
       st[7] = (double)restarts;
@@ -1217,7 +1236,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
       st[14] = used_learned ? 1.0 : 0.0;
       st[15] = guess_ratio;
     }
-    S.status[MFEM_STATUS_EPOCH] = (int32_t)gs.epoch;
+    *ep_ptr = (int32_t)gs.epoch;
   }
 #undef PCG_LAP
 }
@@ -1309,8 +1328,9 @@ __global__ void __launch_bounds__(kBlock) k_ws_features(mfem_solver S, float dt,
   f[13] = (float)valence / 16.f;
   f[14] = 1.f;
   // step length the line search accepted in the previous Newton iteration
+  const int body = S.n_bodies > 1 ? body_of(S.body_vert, S.n_bodies, i) : 0;
   f[15] = (newton_iter > 0 && newton_iter <= MFEM_MAX_STATS_ITERS)
-              ? (float)S.stats[(newton_iter - 1) * MFEM_STATS_PER_ITER + 4]
+              ? (float)S.stats[((size_t)body * MFEM_MAX_STATS_ITERS + newton_iter - 1) * MFEM_STATS_PER_ITER + 4]
               : 0.f;
 #pragma unroll
   for (int k = 0; k < 16; ++k) S.ws_x[16 * i + k] = f2bf16(f[k]);
@@ -1429,7 +1449,7 @@ __device__ double tet_merit(const mfem_solver& S, int e, float a) {
 #pragma unroll
   for (int k = 0; k < 9; ++k) D[k] = S.dminv[9 * e + k];
   const float mu = S.materials[3 * e], la = S.materials[3 * e + 1], vol = S.vol[e];
-  float psi = S.energy == 0 ? arap_energy(s, mu, vol) : nh_energy(s, mu, la, vol);
+  float psi = (S.tet_energy ? S.tet_energy[e] : S.energy) == 0 ? arap_energy(s, mu, vol) : nh_energy(s, mu, la, vol);
   float F[9], U[9], sg[3], V[9], S6[6];
   deformation_gradient(xv[0], xv[1], xv[2], xv[3], D, F);
   svd3(F, U, sg, V);
@@ -1490,20 +1510,30 @@ __device__ double tet_slope(const mfem_solver& S, int e) {
 
 __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, int newton_iter) {
   if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // see k_pcg
-  GridSync gs{S.red, S.flags, S.status, (uint32_t)S.status[MFEM_STATUS_EPOCH], false};
-  const int tid = blockIdx.x * kBlock + threadIdx.x;
-  const int nth = gridDim.x * kBlock;
+  // a batch: the body's workgroups search over the body's vertices, tets
+  // and triangles only, with the body's own merit, step and stats
+  const bool batch = S.n_bodies > 1;
+  const int body = batch ? body_of(S.body_ls_wg, S.n_bodies, blockIdx.x) : 0;
+  const int wg0 = batch ? S.body_ls_wg[body] : 0;
+  const int wg1 = batch ? S.body_ls_wg[body + 1] : (int)gridDim.x;
+  const int v0 = batch ? S.body_vert[body] : 0, v1 = batch ? S.body_vert[body + 1] : S.n_verts;
+  const int e0 = batch ? S.body_tet[body] : 0, e1 = batch ? S.body_tet[body + 1] : S.n_tets;
+  const int f0t = batch ? S.body_tri[body] : 0, f1t = batch ? S.body_tri[body + 1] : S.n_tris;
+  int32_t* ep_ptr = batch ? S.body_epoch + body : S.status + MFEM_STATUS_EPOCH;
+  GridSync gs{S.red, S.flags, S.status, (uint32_t)*ep_ptr, false, wg0, wg1, kRedRegionWords};
+  const int tid = (blockIdx.x - wg0) * kBlock + threadIdx.x;
+  const int nth = (wg1 - wg0) * kBlock;
   double f = 0.0, sl = 0.0;
-  for (int i = tid; i < S.n_verts; i += nth) {
+  for (int i = v0 + tid; i < v1; i += nth) {
     f += vertex_merit(S, i, 0.f, dt);
     sl += vertex_slope(S, i);
   }
-  for (int e = tid; e < S.n_tets; e += nth) {
+  for (int e = e0 + tid; e < e1; e += nth) {
     f += tet_merit(S, e, 0.f);
     sl += tet_slope(S, e);
   }
   if (tri_contact_on(S))
-    for (int t = tid; t < S.n_tris; t += nth) f += tri_merit(S, t, 0.f);
+    for (int t = f0t + tid; t < f1t; t += nth) f += tri_merit(S, t, 0.f);
   double r0[2] = {f, sl};
   if (!grid_reduce(gs, r0)) return;
   const double f0 = r0[0];
@@ -1514,10 +1544,10 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
   do {
     const float a = (float)alpha;
     f = 0.0;
-    for (int i = tid; i < S.n_verts; i += nth) f += vertex_merit(S, i, a, dt);
-    for (int e = tid; e < S.n_tets; e += nth) f += tet_merit(S, e, a);
+    for (int i = v0 + tid; i < v1; i += nth) f += vertex_merit(S, i, a, dt);
+    for (int e = e0 + tid; e < e1; e += nth) f += tet_merit(S, e, a);
     if (tri_contact_on(S))
-      for (int t = tid; t < S.n_tris; t += nth) f += tri_merit(S, t, a);
+      for (int t = f0t + tid; t < f1t; t += nth) f += tri_merit(S, t, a);
     double r1[1] = {f};
     if (!grid_reduce(gs, r1)) return;
     f1 = r1[0];
@@ -1529,23 +1559,23 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
   const bool accepted = it < S.ls_max;
   if (accepted) {
     const float a = (float)used;
-    for (int i = tid; i < S.n_verts; i += nth)
+    for (int i = v0 + tid; i < v1; i += nth)
 #pragma unroll
       for (int k = 0; k < 3; ++k) S.x[3 * i + k] = S.x[3 * i + k] + a * (float)S.dx[3 * i + k];
-    for (int e = tid; e < S.n_tets; e += nth)
+    for (int e = e0 + tid; e < e1; e += nth)
 #pragma unroll
       for (int k = 0; k < 6; ++k) S.s[6 * e + k] = S.s[6 * e + k] + a * S.ds[6 * e + k];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == wg0 && threadIdx.x == 0) {
     if (newton_iter >= 0 && newton_iter < MFEM_MAX_STATS_ITERS) {
-      double* st = S.stats + newton_iter * MFEM_STATS_PER_ITER;
+      double* st = S.stats + ((size_t)body * MFEM_MAX_STATS_ITERS + newton_iter) * MFEM_STATS_PER_ITER;
       st[2] = (double)it;
       st[3] = accepted ? 1.0 : 0.0;
       st[4] = used;
       st[5] = f0;
       st[6] = f1;
     }
-    S.status[MFEM_STATUS_EPOCH] = (int32_t)gs.epoch;
+    *ep_ptr = (int32_t)gs.epoch;
   }
 }
 
@@ -1557,8 +1587,8 @@ __global__ void __launch_bounds__(kBlock) k_plain_update(mfem_solver S, int newt
   if (i < S.n_tets)
 #pragma unroll
     for (int k = 0; k < 6; ++k) S.s[6 * i + k] += S.ds[6 * i + k];
-  if (i == 0 && newton_iter >= 0 && newton_iter < MFEM_MAX_STATS_ITERS) {
-    double* st = S.stats + newton_iter * MFEM_STATS_PER_ITER;
+  if (i < (S.n_bodies > 1 ? S.n_bodies : 1) && newton_iter >= 0 && newton_iter < MFEM_MAX_STATS_ITERS) {
+    double* st = S.stats + ((size_t)i * MFEM_MAX_STATS_ITERS + newton_iter) * MFEM_STATS_PER_ITER;
     st[2] = 0.0;
     st[3] = 1.0;
     st[4] = 1.0;
@@ -1661,6 +1691,13 @@ int check_solver(const mfem_solver* S) {
   if (S->iterations > MFEM_MAX_STATS_ITERS) return fail(MFEM_EINVAL, "iterations > MFEM_MAX_STATS_ITERS");
   if (!S->x || !S->v || !S->s || !S->lam || !S->tets || !S->sell_val || !S->status || !S->stats)
     return fail(MFEM_EINVAL, "null buffer");
+  if (S->n_bodies > 1) {
+    if (!S->body_wg || !S->body_vert || !S->body_tet || !S->body_tri || !S->body_ls_wg || !S->body_epoch)
+      return fail(MFEM_EINVAL, "a batch needs its body arrays");
+    if (S->n_chunks % kWaves != 0) return fail(MFEM_EINVAL, "a batch needs whole workgroups of chunks per body");
+    if (S->ls_blocks < S->n_bodies || S->ls_blocks > MFEM_RED_MAX_BLOCKS)
+      return fail(MFEM_EINVAL, "ls_blocks must be in [n_bodies, MFEM_RED_MAX_BLOCKS]");
+  }
   if (S->tri_contact && S->n_tris > 0 &&
       (!S->tris || !S->tcon_ptr || !S->tcon_src || !S->vtri_ptr || !S->vtri_src || !S->tri_dist || !S->tri_bary ||
        !S->tri_nrm || !S->tri_sid))
@@ -1686,7 +1723,7 @@ int launched() {
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
   X(lag_sid) X(G) X(Hl) X(gl) X(Hsinv) X(gs) X(psi) X(Kb) X(fe) X(gx) X(gB) X(gD) X(Hv) X(dist) X(eB)      \
-  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid)
+  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy)
 
 extern "C" {
 
@@ -1728,6 +1765,13 @@ int mfem_update_system(const mfem_solver* S, float dt, void* stream) {
   if (int rc = check_solver(S)) return rc;
   if (!(dt > 0.f)) return fail(MFEM_EINVAL, "dt must be > 0");
   hipStream_t st = (hipStream_t)stream;
+  if (S->n_bodies > 1) {
+    if (!S->body_wg || !S->body_vert || !S->body_tet || !S->body_tri || !S->body_ls_wg || !S->body_epoch)
+      return fail(MFEM_EINVAL, "a batch needs its body arrays");
+    if (S->n_chunks % kWaves != 0) return fail(MFEM_EINVAL, "a batch needs whole workgroups of chunks per body");
+    if (S->ls_blocks < S->n_bodies || S->ls_blocks > MFEM_RED_MAX_BLOCKS)
+      return fail(MFEM_EINVAL, "ls_blocks must be in [n_bodies, MFEM_RED_MAX_BLOCKS]");
+  }
   if (S->tri_contact && S->n_tris > 0 && S->n_shapes > 0) {
     hipLaunchKernelGGL(k_tri_contact, dim3(blocks_for(S->n_tris)), dim3(kBlock), 0, st, *S);
     if (int rc = launched()) return rc;
@@ -1764,6 +1808,7 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
     const PcgVariant& v = kPcgVariants[k];
     const int want = (S->pcg_flags & MFEM_PCG_PIPELINED) ? 1 : 0;
     if (v.mode != want) continue;
+    if (S->n_bodies > 1 && v.cpw != 1) continue;  // a batch: one chunk per wave, body-aligned workgroups
     if (v.rw > 0 && ((S->pcg_flags & MFEM_PCG_STREAM_ROWS) || S->max_width > v.rw)) continue;
     long waves = (S->n_chunks + v.cpw - 1) / v.cpw;
     long nb = (waves + kWaves - 1) / kWaves;
@@ -1800,6 +1845,10 @@ int mfem_line_search(const mfem_solver* S, float dt, int newton_iter, void* stre
   long nb = blocks_for(n);
   if (nb > ci->max_blocks_ls) nb = ci->max_blocks_ls;
   if (nb > MFEM_RED_MAX_BLOCKS) nb = MFEM_RED_MAX_BLOCKS;
+  if (S->n_bodies > 1) {
+    if (S->ls_blocks > ci->max_blocks_ls) return fail(MFEM_ECAPACITY, "ls_blocks exceeds the resident workgroups");
+    nb = S->ls_blocks;
+  }
   if (nb < 1) return fail(MFEM_EHIP, "line-search kernel cannot be resident");
   mfem_solver Sc = *S;
   void* args[] = {(void*)&Sc, (void*)&dt, (void*)&newton_iter};
@@ -1834,6 +1883,12 @@ int mfem_step(const mfem_solver* S, float dt, void* stream) {
 
 int mfem_pcg_variant_count(void) { return kNumPcgVariants; }
 
+int mfem_line_search_max_blocks(void) {
+  CoopInfo* ci = nullptr;
+  if (coop_info(&ci)) return 0;
+  return ci->max_blocks_ls < MFEM_RED_MAX_BLOCKS ? ci->max_blocks_ls : MFEM_RED_MAX_BLOCKS;
+}
+
 int mfem_pcg_variant_info(const mfem_solver* S, int k, int32_t* info) {
   if (!S || !info || k < 0 || k >= kNumPcgVariants) return fail(MFEM_EINVAL, "bad PCG variant query");
   CoopInfo* ci = nullptr;
@@ -1845,7 +1900,10 @@ int mfem_pcg_variant_info(const mfem_solver* S, int k, int32_t* info) {
   info[1] = v.rw;
   info[2] = v.mode;
   info[3] = (int32_t)nb;
-  info[4] = (nb <= ci->max_blocks_pcg[k] && nb <= MFEM_RED_MAX_BLOCKS && (v.rw == 0 || S->max_width <= v.rw)) ? 1 : 0;
+  info[4] = (nb <= ci->max_blocks_pcg[k] && nb <= MFEM_RED_MAX_BLOCKS && (v.rw == 0 || S->max_width <= v.rw) &&
+             (S->n_bodies <= 1 || v.cpw == 1))
+                ? 1
+                : 0;
   return MFEM_OK;
 }
 

@@ -29,6 +29,8 @@ FIELDS = [
     "G", "Hl", "gl", "Hsinv", "gs", "psi", "Kb", "fe", "gx", "gB", "gD", "Hv", "dist", "eB", "eD", "g",
     "dx", "ds", "pz", "px", "red", "flags", "status", "stats", "mlp_w", "mlp_wbf", "ws_x", "ws_aux", "ws_y", "guess",
     "tcon_ptr", "tcon_src", "vtri_ptr", "vtri_src", "tri_dist", "tri_bary", "tri_nrm", "tri_sid",
+)] + [("n_bodies", c_i32), ("ls_blocks", c_i32)] + [(n, c_ptr) for n in (
+    "body_wg", "body_vert", "body_tet", "body_tri", "body_ls_wg", "body_epoch", "tet_energy",
 )]
 
 
@@ -53,7 +55,7 @@ ENTRY_POINTS = [
     "mfem_scan_i32", "mfem_scan_workspace_bytes", "mfem_broadphase_pt", "mfem_broadphase_ee",
     "mfem_broadphase_workspace_bytes", "mfem_soft_rigid_distance", "mfem_pair_barrier", "mfem_collide_last_error",
     "mfem_refine_workspace_bytes", "mfem_refine_select", "mfem_refine_count", "mfem_refine_apply",
-    "mfem_refine_last_error", "mfem_pcg_variant_count", "mfem_pcg_variant_info",
+    "mfem_refine_last_error", "mfem_pcg_variant_count", "mfem_pcg_variant_info", "mfem_line_search_max_blocks",
 ]
 
 
@@ -116,6 +118,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mfem_refine_apply.argtypes = [vp, vp, sz, vp, vp]
     lib.mfem_refine_last_error.restype = ctypes.c_char_p
     lib.mfem_pcg_variant_count.argtypes = []
+    lib.mfem_line_search_max_blocks.argtypes = []
     lib.mfem_pcg_variant_info.argtypes = [P, i, vp]
     _lib = lib
     return lib

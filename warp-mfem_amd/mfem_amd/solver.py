@@ -185,7 +185,7 @@ class RefinementSolver:
             b[n] = zf(Vc)
         b["dx"] = torch.zeros(Vc, 3, dtype=torch.float64, device=dev)
         b["ds"] = zf(Tc, 6)
-        b["red"] = torch.zeros(2 * _ffi.MFEM_RED_MAX_BLOCKS * 8, dtype=torch.float64, device=dev)
+        b["red"] = torch.zeros(2 * 2 * _ffi.MFEM_RED_MAX_BLOCKS * 8, dtype=torch.float64, device=dev)
         b["flags"] = torch.zeros(_ffi.MFEM_RED_MAX_BLOCKS, dtype=torch.int32, device=dev)
         b["status"] = torch.zeros(8, dtype=torch.int32, device=dev)
         b["stats"] = torch.zeros(_ffi.MFEM_MAX_STATS_ITERS * _ffi.MFEM_STATS_PER_ITER, dtype=torch.float64, device=dev)
@@ -291,6 +291,48 @@ class RefinementSolver:
         for name in ("chunk_ptr", "sell_col", "sell_val", "diag_lane", "contrib_ptr", "contrib_src", "vinc_ptr",
                      "vinc_src", "pz", "px", "tcon_ptr", "tcon_src", "vtri_ptr", "vtri_src"):
             setattr(S, name, b[name].data_ptr())
+
+    def _set_bodies(self, body_vert, body_tet, body_tri, tet_energy=None) -> None:
+        """Declares the mesh a batch of independent bodies (vertex ranges
+        starting at multiples of 256, contiguous tets / triangles): one PCG
+        and one line-search launch per Newton iteration cover all bodies,
+        each with its own CG scalars, convergence, line search and stats."""
+        dev, b, S = self.device, self.buffers, self._S
+        bv = np.asarray(body_vert, np.int64)
+        n = len(bv) - 1
+        if n < 1 or np.any(bv[:-1] % 256) or bv[-1] != S.n_verts or S.n_verts % 256:
+            raise ValueError("body vertex ranges must start at multiples of 256 and end at n_verts (a multiple of 256)")
+        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=dev)
+        b["body_vert"] = i32(bv)
+        b["body_tet"] = i32(body_tet)
+        b["body_tri"] = i32(body_tri)
+        b["body_wg"] = i32(bv // 256)
+        # line-search workgroups in proportion to each body's work, at least one
+        work = np.diff(bv) + np.diff(np.asarray(body_tet, np.int64))
+        cap = int(self.lib.mfem_line_search_max_blocks())
+        if cap < n:
+            raise ValueError(f"{n} bodies need {n} line-search workgroups, {cap} can be resident")
+        share = np.maximum(1, np.floor(work / work.sum() * (cap - n)).astype(np.int64))
+        while share.sum() > cap:
+            share[np.argmax(share)] -= 1
+        b["body_ls_wg"] = i32(np.concatenate([[0], np.cumsum(share)]))
+        b["body_epoch"] = torch.zeros(n, dtype=torch.int32, device=dev)
+        b["stats"] = torch.zeros(n * _ffi.MFEM_MAX_STATS_ITERS * _ffi.MFEM_STATS_PER_ITER, dtype=torch.float64,
+                                 device=dev)
+        if tet_energy is not None:
+            b["tet_energy"] = i32(np.concatenate([np.asarray(tet_energy, np.int32),
+                                                  np.zeros(self.max_tets - len(tet_energy), np.int32)]))
+            S.tet_energy = b["tet_energy"].data_ptr()
+        S.n_bodies, S.ls_blocks = n, int(share.sum())
+        for name in ("body_vert", "body_tet", "body_tri", "body_wg", "body_ls_wg", "body_epoch", "stats"):
+            setattr(S, name, b[name].data_ptr())
+        self.n_bodies = n
+
+    def body_stats(self, body: int) -> np.ndarray:
+        """last_stats() of one body of a batch."""
+        torch.cuda.synchronize(self.device)
+        st = self.buffers["stats"].view(-1, _ffi.MFEM_MAX_STATS_ITERS, _ffi.MFEM_STATS_PER_ITER)[body]
+        return st[: self.iterations].cpu().numpy().copy()
 
     @property
     def active_particle_count(self) -> int:
