@@ -8,12 +8,14 @@ plane with IPC barrier (dhat = 1e-3) and friction (mu = 0.3), dt = 1/100,
 (FP64 residual), line search on.  Synthetic mesh, generated in-process.
 
 Order: W warmup steps (plain PCG); ``--train-steps`` plain steps whose
-solves are logged (features, candidates, right-hand side, matrix); training
-of the warm-start MLP on the GPU on the solver's own objective, the initial
-CG residual (BF16 MFMA forward/backward, SpMV, Adam); then K timed steps with
-plain PCG, K with the learned warm start and, as an ablation, K with the
-network's correction switched off (previous Newton step as the guess), all
-from the same state.  One "step" = RefinementSolver.step on the whole body.
+solves are logged (features, candidates, right-hand side, matrix, PCG
+solution); training of the warm-start MLP on the GPU on the energy-norm error
+of its guess (``--objective``; BF16 MFMA forward/backward, SpMV, Adam) and
+model selection by CG iterations over two validation steps (the trained
+correction or none); then K timed steps with plain PCG, K with the learned
+warm start and, as an ablation, K with the network's correction switched off
+(previous Newton step as the guess), all from the same state and solver
+state.  One "step" = RefinementSolver.step on the whole body.
 value = tets * K / wall time of the learned leg (inputs resident in HBM);
 the plain leg's rate is reported beside it (config.plain_value).
 
@@ -42,6 +44,8 @@ for _p in (ROOT, os.path.join(ROOT, "warp-mfem_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+_OBJECTIVES = {"energy": "log e^T H e per solve, e = dx0 - dx* (energy-norm error of the guess)",
+               "residual": "log |g - H dx0|^2 per solve (initial residual)"}
 BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA
 
 
@@ -50,12 +54,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="octopus", choices=["octopus", "beam"])
+    ap.add_argument("--workload", default="octopus", choices=["octopus", "beam", "batch"],
+                    help="octopus: the ~2e5-tet body (default); beam: the 1,920-tet cantilever; batch: 64 seeded "
+                         "soft bodies (~9.4e5 tets) in one batched solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--pcg", default="standard", choices=["standard", "pipelined"])
     ap.add_argument("--stream-rows", action="store_true", help="stream matrix rows instead of keeping them in VGPRs")
     ap.add_argument("--train-steps", type=int, default=6, help="plain steps logged as training solves")
     ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--objective", default="energy", choices=sorted(_OBJECTIVES),
+                    help="training objective of the warm-start network")
     ap.add_argument("--no-learned", action="store_true", help="time plain PCG only")
     ap.add_argument("--dump-iters", default=None,
                     help="write the CG iteration count of every PCG launch, in launch order, to this JSON file "
@@ -81,6 +89,9 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": src.numel() * 4, "copy_probe_bytes_written": dst.numel() * 4,
                           "copies": 5}), flush=True)
         return
+
+    if args.workload == "batch":
+        return run_batch(args)
 
     from mfem_amd import scene
     from mfem_amd.solver import RefinementSolver, model_state
@@ -121,17 +132,20 @@ def main():
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         net = WarmStartNet("cuda:0")
-        hist = net.fit_residual(solves, solver, epochs=args.epochs, lr=1e-4, seed=0)
+        hist = net.fit_solves(solves, solver, args.objective, epochs=args.epochs, lr=1e-4, seed=0)
+        del solves
+        val = net.validate_cg(solver, state, dt, steps=2)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        train_info = {"solves": len(solves), "pairs": len(solves) * m.particle_count, "train_steps": args.train_steps,
-                      "epochs": args.epochs, "objective": "log |g - H dx0|^2 per solve", "collect_s": t1 - t0,
+        train_info = {"solves": args.train_steps * solver.iterations,
+                      "pairs": args.train_steps * solver.iterations * m.particle_count, "train_steps": args.train_steps,
+                      "epochs": args.epochs, "objective": _OBJECTIVES[args.objective], "collect_s": t1 - t0,
                       "train_s": t2 - t1, "loss_prev_newton_step": hist["base"],
-                      "loss_per_epoch": [round(v, 4) for v in hist["loss"]], "best_epoch": hist["best_epoch"]}
-        print(f"[bench] trained on {len(solves)} solves in {t2 - t1:.2f} s, objective {hist['loss'][0]:.4g} -> "
-              f"{min(hist['loss']):.4g} (kept epoch {hist['best_epoch']}; previous Newton step alone: "
-              f"{hist['base']:.4g})", flush=True)
-        del solves
+                      "loss_per_epoch": [round(v, 4) for v in hist["loss"]], "best_epoch": hist["best_epoch"],
+                      "validation": val}
+        print(f"[bench] trained in {t2 - t1:.2f} s, objective {hist['loss'][0]:.4g} -> {min(hist['loss']):.4g} "
+              f"(kept epoch {hist['best_epoch']}; previous Newton step alone: {hist['base']:.4g}); validation CG "
+              f"iterations {val['cg_iters']}, kept {val['kept']}", flush=True)
 
     khz = solver.lib.mfem_wall_clock_khz()
 
@@ -148,6 +162,7 @@ def main():
     def timed_leg(s0, use_net):
         st = s0.clone()
         o = st.clone()
+        solver.restore(snap0)
         solver.set_warm_start(use_net)
         timers: dict = {}
         hist = []
@@ -167,6 +182,7 @@ def main():
         return wall, stats, timers, st
 
     s0 = state.clone()
+    snap0 = solver.snapshot()  # every leg starts from the same solver state
     wall_p, st_p, timers_p, end_p = timed_leg(s0, None)
     if args.dump_iters:
         launch_iters += st_p[:, :, 0].astype(int).reshape(-1).tolist()
@@ -241,7 +257,8 @@ def main():
             "table": [{k: r[k] for k in ("variant", "chunks_per_wave", "resident_slots", "mode", "workgroups",
                                          "ms_per_step", "us_per_iter", "cg_iters")} for r in tune["table"]]}
     if learned:
-        config["learned_guess_used_frac"] = float(st[:, :, 14].mean())
+        config["pcg_start_frac"] = {name: float((st[:, :, 14] == k).mean()) for k, name in
+                                    enumerate(["x_tilde_minus_x", "learned", "prev_newton_step"])}
         config["learned_guess_residual_ratio_median"] = float(np.median(st[:, :, 15]))
         config["max_abs_position_diff_vs_plain"] = pos_diff
         config["training"] = train_info
@@ -282,6 +299,239 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(m, dt)
     print(json.dumps(result), flush=True)
+
+
+def run_batch(args):
+    """The batch workload: 64 seeded soft bodies (boxes / ellipsoids, 2e3 to
+    5e4 tets each, ARAP / Neo-Hookean, E log-uniform in [1e4, 1e6] Pa) over
+    the plane z = 0, dt = 1/100, 12 Newton iterations, every body's PCG to
+    rel-res 1e-6 on its own, in one persistent PCG launch and one
+    line-search launch per Newton iteration.  Legs as for one body (plain,
+    learned after training on logged batch solves, autotuned), plus the
+    learned leg replayed as HIP graphs of a whole step (value = the faster
+    of the eager and graph learned legs).  Roofline: k_pcg algorithmic bytes
+    per launch = sum over bodies of that body's CG iterations x (40 B per
+    nonzero block + 24 B per vertex), over the launch's event-timed
+    duration."""
+    import numpy as np
+    import torch
+
+    from mfem_amd import _ffi, scene
+    from mfem_amd.batch import BatchSolver
+    from mfem_amd.learned import WarmStartNet, collect_solves
+
+    dt = 0.01
+    t0 = time.perf_counter()
+    models = scene.soft_body_batch()
+    params = {k: v for k, v in models[0].params.items() if k not in ("iterations", "energy")}
+    bs = BatchSolver(models, models[0].params["iterations"], device="cuda:0", **params)
+    solver = bs.solver
+    nb = bs.n_bodies
+    T = bs.tet_count
+    Vr = sum(m.particle_count for m in models)
+    wl = f"soft_body_batch_{nb}_bodies_{T // 1000}k_tets"
+    print(f"[bench] {wl}: {T} tets, {Vr} verts ({solver.active_particle_count} rows, body alignment "
+          f"{256 * bs.body_wg_align} rows), built in {time.perf_counter() - t0:.1f} s, warmup {args.warmup}", flush=True)
+    state, out = bs.make_state(), bs.make_state()
+    for k in range(args.warmup):
+        bs.step(state, out, dt)
+        state, out = out, state
+    torch.cuda.synchronize()
+    if bs.status() != 0:
+        raise SystemExit("device error during warmup")
+
+    def body_stats():
+        return solver.buffers["stats"].view(nb, _ffi.MFEM_MAX_STATS_ITERS, _ffi.MFEM_STATS_PER_ITER)[:, : solver.iterations]
+
+    learned = not args.no_learned
+    net, train_info = None, None
+    if learned:
+        t0 = time.perf_counter()
+        solves = collect_solves(solver, state, dt, steps=args.train_steps)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        net = WarmStartNet("cuda:0")
+        hist = net.fit_solves(solves, solver, args.objective, epochs=args.epochs, lr=1e-4, seed=0)
+        del solves
+        val = net.validate_cg(solver, state, dt, steps=2)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        train_info = {"solves": args.train_steps * solver.iterations,
+                      "pairs": args.train_steps * solver.iterations * Vr, "train_steps": args.train_steps,
+                      "epochs": args.epochs, "objective": _OBJECTIVES[args.objective], "collect_s": t1 - t0,
+                      "train_s": t2 - t1, "loss_prev_newton_step": hist["base"],
+                      "loss_per_epoch": [round(v, 4) for v in hist["loss"]], "best_epoch": hist["best_epoch"],
+                      "validation": val}
+        print(f"[bench] trained in {t2 - t1:.2f} s (kept epoch {hist['best_epoch']}); validation CG iterations "
+              f"{val['cg_iters']}, kept {val['kept']}", flush=True)
+    tune = None
+    if not args.no_autotune:
+        solver.set_warm_start(net)
+        tune = solver.autotune(state, dt)
+        c = tune["chosen"]
+        print(f"[bench] autotuned PCG: variant {c['variant']} ({c['chunks_per_wave']} chunk(s)/wave, "
+              f"{c['resident_slots']} resident slots, {c['mode']}), {c['ms_per_step']:.2f} ms/step", flush=True)
+
+    def timed_leg(s0, use_net):
+        st, o = s0.clone(), s0.clone()
+        solver.restore(snap0)
+        solver.set_warm_start(use_net)
+        timers: dict = {}
+        hist = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            solver.step_instrumented(st, o, dt, timers)
+            hist.append(body_stats().clone())
+            st, o = o, st
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        if bs.status() != 0:
+            raise SystemExit("device error during the timed steps")
+        stats = torch.stack(hist).cpu().numpy()  # (K, bodies, iters, stats)
+        assert np.all(stats[..., 1] <= 1e-6), "a body's PCG solve missed rel-res 1e-6"
+        return wall, stats, timers, st
+
+    def graph_leg(s0, use_net):
+        solver.set_warm_start(use_net)
+        a, b = s0.clone(), s0.clone()
+        solver.restore(snap0)
+        g_ab = solver.capture_step(a, b, dt)
+        g_ba = solver.capture_step(b, a, dt)
+        a.assign(s0)
+        solver.restore(snap0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            (g_ab if k % 2 == 0 else g_ba).replay()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        if bs.status() != 0:
+            raise SystemExit("device error during the graph replays")
+        assert np.all(body_stats().cpu().numpy()[..., 1] <= 1e-6)
+        return wall
+
+    s0 = state.clone()
+    snap0 = solver.snapshot()  # every leg starts from the same solver state
+    wall_p, st_p, _, end_p = timed_leg(s0, None)
+    print(f"[bench] plain: {args.steps} steps in {wall_p:.3f} s", flush=True)
+    if learned:
+        wall_e, st, timers, end_l = timed_leg(s0, net)
+        print(f"[bench] learned: {args.steps} steps in {wall_e:.3f} s", flush=True)
+    else:
+        wall_e, st, timers = wall_p, st_p, _
+    wall_g = graph_leg(s0, net)
+    print(f"[bench] learned, HIP graph replays: {args.steps} steps in {wall_g:.3f} s", flush=True)
+    wall = min(wall_e, wall_g)
+
+    # per-body algorithmic bytes per CG iteration
+    rp = solver.pattern.rowptr
+    bv = bs.body_vert
+    bytes_body = np.array([40.0 * (rp[bv[i] + models[i].particle_count] - rp[bv[i]]) + 24.0 * models[i].particle_count
+                           for i in range(nb)])
+    pcg_ms = np.array([x.elapsed_time(y) for x, y in timers["global_solve"]])  # (K * iters)
+    it_body = st[..., 0]  # (K, bodies, iters)
+    launch_bytes = np.einsum("kbi,b->ki", it_body, bytes_body).reshape(-1)
+    launch_iters = it_body.max(axis=1).reshape(-1)
+    active = launch_iters > 0
+    achieved = float(launch_bytes[active].sum() / (pcg_ms[active].sum() * 1e-3) / 1e9) if active.any() else 0.0
+    per_step_body = it_body.sum(axis=2)  # (K, bodies)
+    value = T * args.steps / wall
+    value_plain = T * args.steps / wall_p
+    config = {
+        "workload": wl,
+        "bodies": nb,
+        "tets": T,
+        "verts": Vr,
+        "rows": int(solver.active_particle_count),
+        "body_alignment_rows": 256 * bs.body_wg_align,
+        "tets_per_body_min_max": [int(min(m.tet_count for m in models)), int(max(m.tet_count for m in models))],
+        "neohookean_bodies": int(sum(m.params["energy"] == "neohookean" for m in models)),
+        "newton_iterations": solver.iterations,
+        "dt": dt,
+        "preconditioner": "block_jacobi",
+        "learned_warm_start": learned,
+        "reductions": "f64, per body",
+        "launches_per_newton_iteration": {"k_pcg": 1, "k_linesearch": 1},
+        "eager_ms_per_step": wall_e * 1e3 / args.steps,
+        "graph_ms_per_step": wall_g * 1e3 / args.steps,
+        "plain_value": value_plain,
+        "plain_ms_per_step": wall_p * 1e3 / args.steps,
+        "learned_speedup": value / value_plain,
+        "cg_iters_per_step_launch": float(st[..., 0].max(axis=1).sum(axis=1).mean()),
+        "plain_cg_iters_per_step_launch": float(st_p[..., 0].max(axis=1).sum(axis=1).mean()),
+        "body_cg_iters_per_step_min_median_max": [float(per_step_body.mean(axis=0).min()),
+                                                  float(np.median(per_step_body.mean(axis=0))),
+                                                  float(per_step_body.mean(axis=0).max())],
+        "cg_relres_max": float(st[..., 1].max()),
+        "hardest_bodies": [{"tets": models[i].tet_count, "energy": models[i].params["energy"],
+                            "E": float(_young(models[i])), "cg_iters_per_step": float(per_step_body[:, i].mean())}
+                           for i in np.argsort(-per_step_body.mean(axis=0))[:4]],
+    }
+    if tune is not None:
+        config["pcg_autotune"] = {
+            "chosen": tune["chosen"]["variant"],
+            "table": [{k: r[k] for k in ("variant", "chunks_per_wave", "resident_slots", "mode", "workgroups",
+                                         "ms_per_step", "us_per_iter", "cg_iters")} for r in tune["table"]]}
+    if learned:
+        config["training"] = train_info
+        config["pcg_start_frac"] = {name: float((st[..., 14] == k).mean()) for k, name in
+                                    enumerate(["x_tilde_minus_x", "learned", "prev_newton_step"])}
+    result = {
+        "metric": "tet-steps/s at CG rel-res 1e-6, learned-precond vs plain PCG, 1 MI355X",
+        "value": value,
+        "unit": "tet-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": config,
+        "roofline": {
+            "kernel": "k_pcg",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "us_per_launch_iter": float(pcg_ms[active].sum() * 1e3 / max(1.0, launch_iters[active].sum())),
+        },
+    }
+    if learned:
+        result["mlp"] = mlp_mfma(solver, net, solver.active_particle_count)
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline_bodies(models, dt)
+    print(json.dumps(result), flush=True)
+
+
+def _young(m):
+    mu, lam = m.tet_materials[0, 0], m.tet_materials[0, 1]
+    return mu * (3 * lam + 2 * mu) / (lam + mu)
+
+
+def cpu_baseline_bodies(models, dt, budget_tets: int = 150_000):
+    """FP64 C oracle, one step of the first bodies of the batch (up to
+    ``budget_tets`` tets) from their initial states, one thread."""
+    from oracle import Oracle
+
+    tets, sec, n = 0, 0.0, 0
+    for m in models:
+        if n and tets + m.tet_count > budget_tets:
+            break
+        o = Oracle(m, 64)
+        t0 = time.perf_counter()
+        o.step(dt)
+        sec += time.perf_counter() - t0
+        tets += m.tet_count
+        n += 1
+    return {"value": tets / sec, "unit": "tet-steps/s", "cores": 1, "kind": "port",
+            "sample": f"1 step (12 Newton iterations) of the first {n} bodies ({tets} tets) from their initial "
+                      f"states, one body at a time, FP64 C oracle, 1 thread, {sec:.1f} s"}
 
 
 def mlp_mfma(solver, net, V):

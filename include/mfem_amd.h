@@ -42,8 +42,10 @@ extern "C" {
 /* stats[iter][k]: 0 cg_iters, 1 cg_relres (FP64, true residual), 2 ls_trials,
    3 ls_accepted, 4 alpha, 5 merit f0, 6 merit f1, 7 cg_restarts,
    8..13 PCG phase device-clock ticks (SpMV, reduce 1, update, reduce 2,
-   residual checks, prologue), 14 learned guess used (0/1),
-   15 |g - H x0_learned| / |g - H x0_plain| (learned warm start only) */
+   residual checks, prologue), 14 start used with the learned warm start
+   (0 x~ - x, 1 learned guess, 2 previous Newton step: the lowest
+   quadratic-model energy), 15 |g - H x0_learned| / |g - H x0_plain|
+   (learned warm start only) */
 #define MFEM_STATS_PER_ITER 16
 #define MFEM_MAX_STATS_ITERS 64
 #define MFEM_RED_MAX_BLOCKS 2048
@@ -52,7 +54,7 @@ extern "C" {
 /* pcg_flags bits */
 #define MFEM_PCG_PIPELINED 1   /* one grid reduction per CG iteration (pipelined PCG) */
 #define MFEM_PCG_STREAM_ROWS 2 /* stream matrix rows from HBM instead of keeping them in VGPRs */
-#define MFEM_PCG_LEARNED_GUESS 4 /* start from the learned warm start when it has the smaller residual */
+#define MFEM_PCG_LEARNED_GUESS 4 /* start from the lowest-energy of x~ - x, the learned guess, the previous step */
 
 /* status[] words */
 #define MFEM_STATUS_ERRBITS 0 /* bit 0: grid barrier timed out; the persistent kernels (PCG, line search)
@@ -173,20 +175,24 @@ typedef struct mfem_solver {
   float* tri_nrm;          /* [n_tris][3] contact normal (capsule -> triangle) */
   int32_t* tri_sid;        /* [n_tris] shape id, -1 when no capsule */
   /* batches of independent bodies (n_bodies > 1): body b owns vertices
-     [body_vert[b], body_vert[b+1]) starting at a multiple of 256 (whole PCG
-     workgroups of 4 chunks), tets [body_tet[b], ...) and triangles
+     [body_vert[b], body_vert[b+1]) starting at a multiple of
+     256 * body_wg_align (whole PCG workgroups of 4 waves x chunks per wave
+     for every variant whose chunks per wave divide body_wg_align), tets
+     [body_tet[b], ...) and triangles
      [body_tri[b], ...); each body has its own CG scalars, convergence,
      line search and stats ([n_bodies][MFEM_MAX_STATS_ITERS][MFEM_STATS_PER_ITER]).
      No counterpart in the reference (one body per solver there). */
   int32_t n_bodies;          /* 0 or 1: one body */
   int32_t ls_blocks;         /* line-search workgroups when n_bodies > 1 */
-  const int32_t* body_wg;    /* [n_bodies + 1] PCG workgroup ranges (chunk ranges / 4) */
+  const int32_t* body_wg;    /* [n_bodies + 1] chunk ranges / 4 (256-row blocks) */
   const int32_t* body_vert;  /* [n_bodies + 1] */
   const int32_t* body_tet;   /* [n_bodies + 1] */
   const int32_t* body_tri;   /* [n_bodies + 1] */
   const int32_t* body_ls_wg; /* [n_bodies + 1] line-search workgroup ranges */
   int32_t* body_epoch;       /* [n_bodies] grid-barrier epochs */
   const int32_t* tet_energy; /* [n_tets] per-tet energy (0 ARAP, 1 Neo-Hookean) or null: `energy` for all */
+  int32_t body_wg_align;     /* a batch: 1, 2, 4 or 8, dividing every body_wg[b] */
+  int32_t pad2;
 } mfem_solver;
 
 /* ABI self-description (checked against the Python mirror by the tests) */
@@ -373,7 +379,12 @@ const char* mfem_refine_last_error(void);
 
 /* Persistent-PCG variants for an autotuner, no counterpart in the reference:
    info[0..4] = chunks per wave, resident row slots (0 = streamed rows),
-   mode (0 standard, 1 pipelined), workgroups for S, applicable to S (0/1).
+   mode (0 standard, 1 pipelined, 2 standard re-reading the preconditioner
+   and right-hand side to run four waves per SIMD), workgroups for S,
+   applicable to S (0/1:
+   co-resident, rows fit the slots, a batch's bodies aligned to its
+   workgroups).  Reads only n_chunks, max_width, n_bodies and body_wg_align
+   of S (a host-only struct may ask before any buffer exists).
    Returns MFEM_EINVAL when k >= mfem_pcg_variant_count(). */
 int mfem_pcg_variant_count(void);
 /* Co-resident workgroups of the persistent line-search kernel (the bound

@@ -9,7 +9,9 @@ kernels of mfem_kernels.hip), including the BF16 rounding points, and is
 pinned by tests/test_oracle_mlp.py against what mathematics fixes: central
 finite differences of an unrounded FP64 copy of the same network, the
 erf form of GELU, Adam's closed-form step under a constant gradient, zero
-loss at an exact blend, and hand-computed features on tiny systems.
+loss at an exact blend, the energy-norm objective's identity
+e^T H e = 2 Phi(dx0) + g . dx* (Phi the quadratic model), and hand-computed
+features on tiny systems.
 
 Parameter layout (FP32, 6288 values): W1 [16][64], b1 [64], W2 [64][64],
 b2 [64], W3 [64][16], b3 [16]; only the first 3 output columns are used.
@@ -139,6 +141,21 @@ def residual_loss_and_dy(y, cand, g, H):
     rr = float(np.sum(r * r))
     gd = (-2.0 / rr) * (H @ r.reshape(-1)).reshape(V, 3)
     return float(np.log(rr)), np.einsum("nkd,nd->nk", C, gd)
+
+
+def energy_loss_and_dy(y, cand, dx_star, H):
+    """Log of the squared energy-norm error of the corrected guess,
+    L = log (e^T H e) with e = dx0 - dx_star (dx_star: the system's
+    solution), dx0 = c3 + y0 c1 + y1 c2 + y2 c3, and dL/dy (V, 3) =
+    c_k . (2 H e / e^T H e).  CG's error bound is in this norm."""
+    V = dx_star.shape[0]
+    C = np.asarray(cand, np.float64).reshape(V, 3, 3)
+    y = np.asarray(y, np.float64)
+    dx0 = C[:, 2] + np.einsum("nk,nkd->nd", y, C)
+    e = dx0 - np.asarray(dx_star, np.float64)
+    He = (H @ e.reshape(-1)).reshape(V, 3)
+    eHe = float(np.sum(e * He))
+    return float(np.log(eHe)), np.einsum("nkd,nd->nk", C, (2.0 / eHe) * He)
 
 
 def adam(w, m, v, g, lr, t, beta1=0.9, beta2=0.999, eps=1e-8):

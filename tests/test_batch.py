@@ -87,3 +87,43 @@ def test_batch_padding_rows_stay_put(gpu):
         lo = bs.body_slice(i).stop
         hi = int(bs.body_vert[i + 1])
         np.testing.assert_array_equal(x[lo:hi], pad0[lo:hi])
+
+
+def test_batch_wide_workgroups_match_oracle(gpu):
+    """Bodies aligned to 2048 rows: every applicable PCG variant with 2, 4 or
+    8 chunks per wave (workgroups covering consecutive 256-row blocks of one
+    body) gives each body the FP64 oracle's positions within 1e-4."""
+    from mfem_amd.batch import BatchSolver
+
+    models = scene.soft_body_batch(n_bodies=5, seed=11, tets_min=300, tets_max=3000)
+    params = {k: v for k, v in models[0].params.items() if k not in ("iterations", "energy")}
+    it = 4
+    bs = BatchSolver(models, it, device=gpu, body_wg_align=8, **params)
+    assert np.all(bs.body_vert % 2048 == 0)
+    dt, steps = 0.01, 3
+    ref = []
+    for m in models:
+        o = Oracle(m, 64, params=dict(params, energy=m.params["energy"], iterations=it))
+        for k in range(steps):
+            o.step(dt)
+        ref.append(o.x.copy())
+    wide = [v for v in bs.solver.pcg_variants() if v["applicable"] and v["chunks_per_wave"] > 1]
+    assert {v["chunks_per_wave"] for v in wide} == {2, 4, 8}
+    keys = ("s", "lam", "dx", "x_start", "x_tilde", "lag_n", "lag_fn", "lag_sid")
+    init = {k: bs.solver.buffers[k].clone() for k in keys}
+    for v in wide:
+        bs.solver._S.pcg_variant = v["variant"]
+        for k in keys:
+            bs.solver.buffers[k].copy_(init[k])
+        a, b = bs.make_state(), bs.make_state()
+        for k in range(steps):
+            bs.step(a, b, dt)
+            a, b = b, a
+            assert bs.status() == 0
+            for i in range(bs.n_bodies):
+                assert np.all(bs.body_stats(i)[:, 1] <= 1e-6)
+        for i in range(bs.n_bodies):
+            xb = a.particle_q[bs.body_slice(i)].double().cpu().numpy()
+            e = np.abs(xb - ref[i]).max() / np.abs(ref[i]).max()
+            print("variant", v["variant"], v["chunks_per_wave"], v["mode"], "body", i, "rel err", e)
+            assert e <= 1e-4

@@ -206,6 +206,31 @@ def test_pcg_variants_match_oracle(gpu, pipelined, resident):
     _close(state.particle_q.double().cpu().numpy(), o.x, 1e-4, f"positions pipelined={pipelined} resident={resident}")
 
 
+def test_every_pcg_variant_matches_oracle(gpu):
+    # each variant of the table forced in turn (chunks per wave 1-8, resident /
+    # streamed / low-register rows, standard / pipelined), from the same start
+    m, dt = _perturbed_scene("drop_box")
+    solver, state0 = _make(m, gpu)
+    o = Oracle(m, 64)
+    for k in range(3):
+        o.step(dt)
+    snap = solver.snapshot()
+    variants = [v for v in solver.pcg_variants() if v["applicable"]]
+    assert {v["mode"] for v in variants} == {"standard", "pipelined", "standard_lowreg"}
+    for v in variants:
+        solver._S.pcg_variant = v["variant"]
+        solver.restore(snap)
+        state, out = state0.clone(), state0.clone()
+        for k in range(3):
+            solver.step(state, out, None, None, dt)
+            state, out = out, state
+            st = solver.last_stats()
+            assert np.all(st[:, 1] <= 1e-6), (v, st)
+            assert solver.status() == 0
+        print("variant", v, "cg", int(st[:, 0].sum()))
+        _close(state.particle_q.double().cpu().numpy(), o.x, 1e-4, f"positions, variant {v}")
+
+
 def test_graph_free_repeatability(gpu):
     # deterministic reductions and gathers: two runs give bit-identical states
     m = scene.drop_box_scene()

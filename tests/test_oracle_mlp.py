@@ -9,6 +9,7 @@ constant gradient, zero loss at an exact blend, hand-computed features.
 import math
 
 import numpy as np
+import pytest
 import scipy.sparse as sp
 
 from oracle import mlp
@@ -224,6 +225,30 @@ def test_residual_objective_gradient_matches_finite_differences():
     c[:, 2] = dx
     L0, _ = mlp.residual_loss_and_dy(np.zeros((V, 3)), c.reshape(V, 9), g, H)
     assert L0 < np.log(1e-20)
+
+
+def test_energy_objective_gradient_matches_finite_differences():
+    rng = np.random.default_rng(8)
+    V = 12
+    A = rng.normal(0, 1, (3 * V, 3 * V))
+    H = A @ A.T + 3 * V * np.eye(3 * V)
+    cand = rng.normal(0, 1, (V, 9))
+    g = rng.normal(0, 1, (V, 3))
+    dx = np.linalg.solve(H, g.reshape(-1)).reshape(V, 3)
+    y = rng.normal(0, 0.3, (V, 3))
+    L, dy = mlp.energy_loss_and_dy(y, cand, dx, H)
+    h = 1e-6
+    for n, k in [(0, 0), (4, 1), (9, 2), (11, 1)]:
+        yp, ym = y.copy(), y.copy()
+        yp[n, k] += h
+        ym[n, k] -= h
+        fd = (mlp.energy_loss_and_dy(yp, cand, dx, H)[0] - mlp.energy_loss_and_dy(ym, cand, dx, H)[0]) / (2 * h)
+        assert abs(fd - dy[n, k]) <= 1e-6 * max(1.0, abs(fd)), (n, k, fd, dy[n, k])
+    # e^T H e = 2 Phi(dx0) + g . dx_star, Phi the quadratic model energy
+    C = cand.reshape(V, 3, 3)
+    dx0 = (C[:, 2] + np.einsum("nk,nkd->nd", y, C)).reshape(-1)
+    phi = 0.5 * dx0 @ H @ dx0 - g.reshape(-1) @ dx0
+    assert np.exp(L) == pytest.approx(2 * phi + g.reshape(-1) @ dx.reshape(-1), rel=1e-9)
 
 
 def test_backward_from_output_gradient_matches_chain_rule():

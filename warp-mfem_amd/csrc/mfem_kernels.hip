@@ -758,7 +758,7 @@ struct RowBlocks {
 template <>
 struct RowBlocks<0> {};
 
-template <int CPW, int RW>
+template <int CPW, int RW, int GRP = 8>
 __device__ __forceinline__ void spmv_rows(const mfem_solver& S, const int (&chunk)[CPW], const bool (&own)[CPW],
                                           int lane, const RowBlocks<RW>& rb, const float* vec, float (&y)[CPW][3]) {
   if constexpr (RW > 0) {
@@ -768,7 +768,7 @@ __device__ __forceinline__ void spmv_rows(const mfem_solver& S, const int (&chun
 #pragma unroll
     for (int k = 0; k < CPW; ++k) {
       y[k][0] = y[k][1] = y[k][2] = 0.f;
-      if (own[k]) spmv_row_stream<float, float>(S, chunk[k], lane, vec, y[k], true);
+      if (own[k]) spmv_row_stream<float, float, GRP>(S, chunk[k], lane, vec, y[k], true);
     }
   }
 }
@@ -796,19 +796,43 @@ __device__ __forceinline__ void apply_M(const float (&M)[CPW][9], int k, const d
   for (int a = 0; a < 3; ++a) z[a] = (T)(M[k][3 * a] * r[0] + M[k][3 * a + 1] * r[1] + M[k][3 * a + 2] * r[2]);
 }
 
-template <int CPW, int RW, int MODE>
-__global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) {
+// block-Jacobi inverse of a row's diagonal block (identity when the block is
+// singular or preconditioning is off)
+// (row < 0: identity)
+__device__ __forceinline__ void block_jacobi(const mfem_solver& S, int row, float (&M)[9]) {
+#pragma unroll
+  for (int a = 0; a < 9; ++a) M[a] = (a % 4 == 0) ? 1.f : 0.f;
+  if (S.precond && row >= 0) {
+    float Dg[9];
+    const int dl = S.diag_lane[row];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) Dg[a] = S.sell_val[val_index(dl, a)];
+    if (fabsf(det3(Dg)) >= S.sing_thr) inv3(Dg, M);
+  }
+}
+
+// LOWREG (standard mode, streamed rows, one chunk per wave): the
+// preconditioner blocks and the right-hand side are re-read from memory where
+// used instead of held in registers and the phase clock is off, so that four
+// waves fit per SIMD (twice the co-resident rows of the register-held form,
+// for batches of many bodies).
+template <int CPW, int RW, int MODE, bool LOWREG>
+__device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) {
+  static_assert(!LOWREG || (CPW == 1 && RW == 0 && MODE == 0), "LOWREG: standard, streamed, one chunk per wave");
   constexpr bool PIPE = MODE != 0;
   constexpr bool PUB32 = MODE == 2;  // pipelined: publish the SpMV input in FP32
   // a barrier timed out earlier: flags and epoch are inconsistent until the
   // host clears status and flags, so do nothing
   if (S.status[MFEM_STATUS_ERRBITS] != 0) return;
-  // one body: all workgroups; a batch: the body's workgroups (chunks of a
-  // body are whole workgroups, one chunk per wave), its own epoch and stats
-  const int body = S.n_bodies > 1 ? body_of(S.body_wg, S.n_bodies, blockIdx.x) : 0;
-  const int wg0 = S.n_bodies > 1 ? S.body_wg[body] : 0;
-  const int wg1 = S.n_bodies > 1 ? S.body_wg[body + 1] : (int)gridDim.x;
-  int32_t* ep_ptr = S.n_bodies > 1 ? S.body_epoch + body : S.status + MFEM_STATUS_EPOCH;
+  // one body: all workgroups, chunks strided over the grid; a batch: the
+  // body's workgroups, its own epoch and stats (workgroup w covers the
+  // 256-row blocks [w*CPW, (w+1)*CPW); bodies start at multiples of CPW
+  // blocks, so every workgroup belongs to one body)
+  const bool batch = S.n_bodies > 1;
+  const int body = batch ? body_of(S.body_wg, S.n_bodies, (int)blockIdx.x * CPW) : 0;
+  const int wg0 = batch ? S.body_wg[body] / CPW : 0;
+  const int wg1 = batch ? S.body_wg[body + 1] / CPW : (int)gridDim.x;
+  int32_t* ep_ptr = batch ? S.body_epoch + body : S.status + MFEM_STATUS_EPOCH;
   GridSync gs{S.red, S.flags, S.status, (uint32_t)*ep_ptr, false, wg0, wg1, 0};
   const int lane = threadIdx.x & 63;
   const int gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
@@ -831,7 +855,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
   double* px0 = pxbuf(pubx);
 #pragma unroll
   for (int k = 0; k < CPW; ++k) {
-    chunk[k] = gw + k * nw;
+    chunk[k] = batch ? ((int)blockIdx.x * CPW + k) * kWaves + (int)(threadIdx.x >> 6) : gw + k * nw;
     own[k] = chunk[k] < S.n_chunks;
     const int row = chunk[k] * 64 + lane;
     live[k] = own[k] && row < S.n_verts;
@@ -850,13 +874,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
         b[k][a] = S.g[3 * row + a];
         x[k][a] = (double)S.x_tilde[3 * row + a] - (double)S.x[3 * row + a];
       }
-      if (S.precond) {
-        float Dg[9];
-        const int dl = S.diag_lane[row];
-#pragma unroll
-        for (int a = 0; a < 9; ++a) Dg[a] = S.sell_val[val_index(dl, a)];
-        if (fabsf(det3(Dg)) >= S.sing_thr) inv3(Dg, M[k]);
-      }
+      if constexpr (!LOWREG) block_jacobi(S, row, M[k]);
       bb += (double)b[k][0] * b[k][0] + (double)b[k][1] * b[k][1] + (double)b[k][2] * b[k][2];
     }
     if (own[k]) {
@@ -880,7 +898,19 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
     }
   }
   // device-clock profile of the solve phases (wave 0 of workgroup 0 only)
-  const bool prof = blockIdx.x == wg0 && threadIdx.x < 64;
+  const bool prof = !LOWREG && blockIdx.x == wg0 && threadIdx.x < 64;
+  // LOWREG: right-hand side and preconditioner re-read where used
+  auto reload_b = [&]() {
+    if constexpr (LOWREG) {
+#pragma unroll
+      for (int k = 0; k < CPW; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) b[k][a] = live[k] ? S.g[3 * (chunk[k] * 64 + lane) + a] : 0.f;
+    }
+  };
+  auto reload_M = [&](int k) {
+    if constexpr (LOWREG) block_jacobi(S, live[k] ? chunk[k] * 64 + lane : -1, M[k]);
+  };
   uint64_t tk[6] = {0, 0, 0, 0, 0, 0};
   uint64_t tprev = wall_clock64();
 #define PCG_LAP(k)                        \
@@ -951,6 +981,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
       for (int k = 0; k < CPW; ++k) {
         if (!own[k]) continue;
         const int row = chunk[k] * 64 + lane;
+        reload_M(k);
         apply_M<CPW, float>(M, k, r[k], z[k]);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -968,61 +999,99 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
     return true;
   };
 
-  bool used_learned = false;
+  int guess_used = 0;  // 0 x~ - x, 1 learned, 2 previous Newton step
   double guess_ratio = 0.0;
   if (bnorm2 > 0.0) {
     double rz = 0.0, rr = 0.0;
     {
       double y[CPW][3];
       spmv_rows64<CPW, RW, false>(S, chunk, own, lane, rb, px0, y);
+      reload_b();
 #pragma unroll
       for (int k = 0; k < CPW; ++k)
 #pragma unroll
         for (int a = 0; a < 3; ++a) r[k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
     }
     if ((S.pcg_flags & MFEM_PCG_LEARNED_GUESS) && S.guess) {
-      // learned warm start: keep it only if its residual is smaller than the
-      // plain guess's (block-Jacobi CG from x~ - x is the fallback)
+      // learned warm start: three candidate starts, x~ - x (the plain guess),
+      // the learned guess and the previous Newton step, each with its
+      // residual r = g - H x0 and quadratic-model energy
+      // Phi(x0) = x0.H x0 / 2 - g.x0 = -x0.(g + r) / 2, which orders the
+      // starts by the energy norm of their error (the norm CG minimises);
+      // the lowest-energy start is kept.
+      double phi0 = 0.0, r0 = 0.0;
+#pragma unroll
+      for (int k = 0; k < CPW; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          phi0 -= 0.5 * x[k][a] * ((double)b[k][a] + r[k][a]);
+          r0 += r[k][a] * r[k][a];
+        }
+      // candidate c of a row: 1 learned guess, 2 previous Newton step
+      auto cand = [&](int c, int row, int a) -> double {
+        return c == 1 ? (double)S.guess[3 * row + a] : S.dx[3 * row + a];
+      };
       double* gb = pxbuf(pubx);
-      double r0 = 0.0;
 #pragma unroll
       for (int k = 0; k < CPW; ++k) {
         if (!own[k]) continue;
         const int row = chunk[k] * 64 + lane;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          pub_store(&gb[3 * row + a], live[k] ? (double)S.guess[3 * row + a] : 0.0);
-          r0 += r[k][a] * r[k][a];
-        }
+        for (int a = 0; a < 3; ++a) pub_store(&gb[3 * row + a], live[k] ? cand(1, row, a) : 0.0);
       }
       ++pubx;
-      double redg[1] = {r0};
-      if (!grid_reduce(gs, redg)) return;
-      r0 = redg[0];
-      double y[CPW][3];
-      spmv_rows64<CPW, RW, false>(S, chunk, own, lane, rb, gb, y);
-      double r1 = 0.0;
-      double rl[CPW][3];
+      double red0[2] = {phi0, r0};
+      if (!grid_reduce(gs, red0)) return;
+      phi0 = red0[0];
+      r0 = red0[1];
+      // (the plain guess's SpMV input is no longer read by any workgroup:
+      // its buffer takes the previous Newton step)
+      double rc[2][CPW][3];
+      double phic[2], rrc[2];
 #pragma unroll
-      for (int k = 0; k < CPW; ++k)
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          rl[k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
-          r1 += rl[k][a] * rl[k][a];
-        }
-      double redl[1] = {r1};
-      if (!grid_reduce(gs, redl)) return;
-      r1 = redl[0];
-      guess_ratio = r0 > 0.0 ? sqrt(r1 / r0) : 0.0;
-      if (r1 < r0) {
-        used_learned = true;
+      for (int c = 0; c < 2; ++c) {
+        double* cb = pxbuf(pubx - 1);
+        double y[CPW][3];
+        spmv_rows64<CPW, RW, false>(S, chunk, own, lane, rb, cb, y);
+        reload_b();
+        double ph = 0.0, rq = 0.0;
 #pragma unroll
         for (int k = 0; k < CPW; ++k) {
           const int row = chunk[k] * 64 + lane;
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
-            r[k][a] = rl[k][a];
-            x[k][a] = live[k] ? (double)S.guess[3 * row + a] : 0.0;
+            rc[c][k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
+            ph -= live[k] ? 0.5 * cand(c + 1, row, a) * ((double)b[k][a] + rc[c][k][a]) : 0.0;
+            rq += rc[c][k][a] * rc[c][k][a];
+          }
+        }
+        if (c == 0) {
+          double* nb = pxbuf(pubx);
+#pragma unroll
+          for (int k = 0; k < CPW; ++k) {
+            if (!own[k]) continue;
+            const int row = chunk[k] * 64 + lane;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) pub_store(&nb[3 * row + a], live[k] ? cand(2, row, a) : 0.0);
+          }
+          ++pubx;
+        }
+        double redc[2] = {ph, rq};
+        if (!grid_reduce(gs, redc)) return;
+        phic[c] = redc[0];
+        rrc[c] = redc[1];
+      }
+      guess_ratio = r0 > 0.0 ? sqrt(rrc[0] / r0) : 0.0;
+      const int best = (phic[0] <= phic[1]) ? (phic[0] < phi0 ? 1 : 0) : (phic[1] < phi0 ? 2 : 0);
+      if (best != 0) {
+        guess_used = best;
+#pragma unroll
+        for (int k = 0; k < CPW; ++k) {
+          const int row = chunk[k] * 64 + lane;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            r[k][a] = rc[best - 1][k][a];
+            x[k][a] = live[k] ? cand(best, row, a) : 0.0;
           }
         }
       }
@@ -1081,6 +1150,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
         {
           double y[CPW][3];
           spmv_rows64<CPW, RW, false>(S, chunk, own, lane, rb, xb, y);
+          reload_b();
 #pragma unroll
           for (int k = 0; k < CPW; ++k)
 #pragma unroll
@@ -1162,7 +1232,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
         const float fb = (float)beta;
         {
           float y[CPW][3];
-          spmv_rows<CPW, RW>(S, chunk, own, lane, rb, pzbuf(pub - 1), y);
+          spmv_rows<CPW, RW, LOWREG ? 4 : 8>(S, chunk, own, lane, rb, pzbuf(pub - 1), y);
 #pragma unroll
           for (int k = 0; k < CPW; ++k) {
             if (!own[k]) continue;
@@ -1196,6 +1266,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
             x[k][a] += alpha * (double)p[k][a];
             r[k][a] -= alpha * (double)q[k][a];
           }
+          reload_M(k);
           apply_M<CPW, float>(M, k, r[k], z[k]);
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
@@ -1233,12 +1304,22 @@ __global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) 
       st[7] = (double)restarts;
 #pragma unroll
       for (int k = 0; k < 6; ++k) st[8 + k] = (double)tk[k];
-      st[14] = used_learned ? 1.0 : 0.0;
+      st[14] = (double)guess_used;
       st[15] = guess_ratio;
     }
     *ep_ptr = (int32_t)gs.epoch;
   }
 #undef PCG_LAP
+}
+
+template <int CPW, int RW, int MODE>
+__global__ void __launch_bounds__(kBlock) k_pcg(mfem_solver S, int newton_iter) {
+  pcg_body<CPW, RW, MODE, false>(S, newton_iter);
+}
+
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_pcg_lowreg(mfem_solver S, int newton_iter) {
+  pcg_body<1, 0, 0, true>(S, newton_iter);
 }
 
 // ---------------------------------------------------------------------------
@@ -1635,17 +1716,20 @@ __global__ void __launch_bounds__(kBlock) k_merit(mfem_solver S, float dt, float
 namespace {
 
 struct PcgVariant {
-  int cpw, rw, mode;
+  int cpw, rw, mode;  // mode 0 standard, 1 pipelined, 2 standard with four waves per SIMD
   const void* fn;
 };
 #define PCG_V(C, R, P) {C, R, P, reinterpret_cast<const void*>(&k_pcg<C, R, P>)}
 // resident-row variants first (one chunk per wave, rows in VGPRs), then the
-// streaming ones with more chunks per wave for larger systems.  mode 0 =
-// standard, 1 = pipelined with an FP64 SpMV input (its FP64 recurrences leave
-// no room for resident rows).  Pipelining with an FP32 SpMV input (mode 2)
-// stagnates near rel-res 1e-3 on the contact scenes and is not instantiated.
+// streaming ones (the low-register one holds twice the rows of the
+// register-held one-chunk form in one grid) with more chunks per wave for
+// larger systems.  Pipelined variants use an FP64 SpMV input (their FP64
+// recurrences leave no room for resident rows); pipelining with an FP32 SpMV
+// input stagnates near rel-res 1e-3 on the contact scenes and is not
+// instantiated.
 const PcgVariant kPcgVariants[] = {
-    PCG_V(1, 16, 0), PCG_V(1, 0, 0), PCG_V(2, 0, 0), PCG_V(4, 0, 0), PCG_V(8, 0, 0),
+    PCG_V(1, 16, 0), PCG_V(1, 0, 0), {1, 0, 2, reinterpret_cast<const void*>(&k_pcg_lowreg)},
+    PCG_V(2, 0, 0), PCG_V(4, 0, 0), PCG_V(8, 0, 0),
     PCG_V(1, 16, 1), PCG_V(1, 0, 1), PCG_V(2, 0, 1), PCG_V(4, 0, 1), PCG_V(8, 0, 1),
 };
 #undef PCG_V
@@ -1695,6 +1779,10 @@ int check_solver(const mfem_solver* S) {
     if (!S->body_wg || !S->body_vert || !S->body_tet || !S->body_tri || !S->body_ls_wg || !S->body_epoch)
       return fail(MFEM_EINVAL, "a batch needs its body arrays");
     if (S->n_chunks % kWaves != 0) return fail(MFEM_EINVAL, "a batch needs whole workgroups of chunks per body");
+    if (S->body_wg_align != 1 && S->body_wg_align != 2 && S->body_wg_align != 4 && S->body_wg_align != 8)
+      return fail(MFEM_EINVAL, "body_wg_align must be 1, 2, 4 or 8");
+    if ((S->n_chunks / kWaves) % S->body_wg_align != 0)
+      return fail(MFEM_EINVAL, "a batch's blocks must be a multiple of body_wg_align");
     if (S->ls_blocks < S->n_bodies || S->ls_blocks > MFEM_RED_MAX_BLOCKS)
       return fail(MFEM_EINVAL, "ls_blocks must be in [n_bodies, MFEM_RED_MAX_BLOCKS]");
   }
@@ -1723,7 +1811,7 @@ int launched() {
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
   X(lag_sid) X(G) X(Hl) X(gl) X(Hsinv) X(gs) X(psi) X(Kb) X(fe) X(gx) X(gB) X(gD) X(Hv) X(dist) X(eB)      \
-  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy)
+  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(pad2)
 
 extern "C" {
 
@@ -1765,13 +1853,6 @@ int mfem_update_system(const mfem_solver* S, float dt, void* stream) {
   if (int rc = check_solver(S)) return rc;
   if (!(dt > 0.f)) return fail(MFEM_EINVAL, "dt must be > 0");
   hipStream_t st = (hipStream_t)stream;
-  if (S->n_bodies > 1) {
-    if (!S->body_wg || !S->body_vert || !S->body_tet || !S->body_tri || !S->body_ls_wg || !S->body_epoch)
-      return fail(MFEM_EINVAL, "a batch needs its body arrays");
-    if (S->n_chunks % kWaves != 0) return fail(MFEM_EINVAL, "a batch needs whole workgroups of chunks per body");
-    if (S->ls_blocks < S->n_bodies || S->ls_blocks > MFEM_RED_MAX_BLOCKS)
-      return fail(MFEM_EINVAL, "ls_blocks must be in [n_bodies, MFEM_RED_MAX_BLOCKS]");
-  }
   if (S->tri_contact && S->n_tris > 0 && S->n_shapes > 0) {
     hipLaunchKernelGGL(k_tri_contact, dim3(blocks_for(S->n_tris)), dim3(kBlock), 0, st, *S);
     if (int rc = launched()) return rc;
@@ -1806,9 +1887,9 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
   }
   for (int k = 0; k < kNumPcgVariants && sel < 0; ++k) {
     const PcgVariant& v = kPcgVariants[k];
-    const int want = (S->pcg_flags & MFEM_PCG_PIPELINED) ? 1 : 0;
-    if (v.mode != want) continue;
-    if (S->n_bodies > 1 && v.cpw != 1) continue;  // a batch: one chunk per wave, body-aligned workgroups
+    const bool want = (S->pcg_flags & MFEM_PCG_PIPELINED) != 0;
+    if ((v.mode == 1) != want) continue;
+    if (S->n_bodies > 1 && S->body_wg_align % v.cpw != 0) continue;  // a batch: body-aligned workgroups
     if (v.rw > 0 && ((S->pcg_flags & MFEM_PCG_STREAM_ROWS) || S->max_width > v.rw)) continue;
     long waves = (S->n_chunks + v.cpw - 1) / v.cpw;
     long nb = (waves + kWaves - 1) / kWaves;
@@ -1901,7 +1982,7 @@ int mfem_pcg_variant_info(const mfem_solver* S, int k, int32_t* info) {
   info[2] = v.mode;
   info[3] = (int32_t)nb;
   info[4] = (nb <= ci->max_blocks_pcg[k] && nb <= MFEM_RED_MAX_BLOCKS && (v.rw == 0 || S->max_width <= v.rw) &&
-             (S->n_bodies <= 1 || v.cpw == 1))
+             (S->n_bodies <= 1 || S->body_wg_align % v.cpw == 0))
                 ? 1
                 : 0;
   return MFEM_OK;

@@ -1,9 +1,10 @@
 """Many independent soft bodies solved together on one GPU.
 
 The bodies are concatenated into one mesh whose vertex ranges start at
-multiples of 256 (one persistent-PCG workgroup = 4 chunks of 64 rows, so
-every workgroup belongs to one body; unused rows are inert pinned padding
-vertices).  Every per-element kernel runs once over all bodies; the
+multiples of 256 * a rows (a persistent-PCG workgroup holds 4 waves of 1, 2,
+4 or 8 chunks of 64 rows; with a = the chunks per wave every workgroup
+belongs to one body; unused rows are inert pinned padding vertices).  The
+smallest a whose rows fit one co-resident grid is chosen.  Every per-element kernel runs once over all bodies; the
 persistent PCG and line-search kernels run one launch per Newton iteration
 for the whole batch, each body with its own grid reductions (its own
 workgroups), CG scalars, convergence test, line-search step and stats, so a
@@ -16,23 +17,23 @@ material vary per body.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
 from .scene import Model
 from .solver import RefinementSolver, State
 
-_ALIGN = 256
-
-
-def union_model(models: list, far: float = 1.0e3):
-    """(union Model, body_vert, body_tet, body_tri, tet_energy)."""
+def union_model(models: list, far: float = 1.0e3, align: int = 256):
+    """(union Model, body_vert, body_tet, body_tri, tet_energy); every body's
+    vertex range starts at a multiple of ``align``."""
     qs, qds, ms, ims, ts, ps, mats, trs, en = [], [], [], [], [], [], [], [], []
     bv, bt, bf = [0], [0], [0]
     nv = 0
     for k, m in enumerate(models):
         V = m.particle_count
-        pad = (-V) % _ALIGN
+        pad = (-V) % align
         q = np.asarray(m.particle_q, np.float64)
         qs.append(q)
         qds.append(np.asarray(m.particle_qd, np.float64))
@@ -62,14 +63,39 @@ def union_model(models: list, far: float = 1.0e3):
     return u, np.array(bv), np.array(bt), np.array(bf), np.concatenate(en)
 
 
+def pick_alignment(vert_counts, device="cuda") -> int:
+    """Smallest a in (1, 2, 4, 8) for which the bodies, padded to multiples
+    of 256 * a rows, fit some streamed-row persistent-PCG variant with a's
+    chunks per wave (or fewer) in one co-resident grid."""
+    from . import _ffi
+
+    lib = _ffi.load()
+    torch.cuda.set_device(torch.device(device))
+    info = (ctypes.c_int32 * 5)()
+    for a in (1, 2, 4, 8):
+        rows = sum(v + (-v) % (256 * a) for v in vert_counts)
+        S = _ffi.MfemSolverStruct()
+        S.n_chunks, S.max_width, S.n_bodies, S.body_wg_align = rows // 64, 1 << 30, len(vert_counts), a
+        for k in range(int(lib.mfem_pcg_variant_count())):
+            if lib.mfem_pcg_variant_info(ctypes.byref(S), k, info) != 0:
+                raise _ffi.MfemError(lib.mfem_last_error().decode())
+            if info[4]:
+                return a
+    raise ValueError(f"{sum(vert_counts)} vertices in {len(vert_counts)} bodies do not fit one persistent grid")
+
+
 class BatchSolver:
     """RefinementSolver-style stepping of a list of models as one batch
     (refinement off).  ``state`` tensors hold the union's vertices;
     ``body_slice(b)`` gives body b's real rows."""
 
-    def __init__(self, models: list, iterations: int, device="cuda", **kwargs):
+    def __init__(self, models: list, iterations: int, device="cuda", body_wg_align: int | None = None, **kwargs):
         self.models = list(models)
-        self.union, self.body_vert, self.body_tet, self.body_tri, tet_energy = union_model(self.models)
+        if body_wg_align is None:
+            body_wg_align = pick_alignment([m.particle_count for m in self.models], device)
+        self.body_wg_align = int(body_wg_align)
+        self.union, self.body_vert, self.body_tet, self.body_tri, tet_energy = union_model(
+            self.models, align=256 * self.body_wg_align)
         kw = dict(kwargs)
         kw["enable_refinement"] = False
         self.solver = RefinementSolver(self.union, iterations, max_tets=self.union.tet_count, device=device, **kw)

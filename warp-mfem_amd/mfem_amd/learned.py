@@ -13,13 +13,21 @@ step,
 
 from 16 rotation-invariant features (candidate norms and angles, diagonal /
 off-diagonal stiffness ratios, mass term, pinning, Newton iteration).  The
-PCG keeps the learned guess only when its residual |g - H dx0| is below the
-plain guess's, so block-Jacobi CG from x~ - x stays the fallback and the
-judge of convergence (the FP64 relative residual).
+PCG starts from whichever of x~ - x (the plain guess), the learned guess and
+the previous Newton step has the lowest quadratic-model energy
+Phi(x0) = x0.H x0 / 2 - g.x0 (the order of their energy-norm errors, which
+bound CG's convergence), so block-Jacobi CG from x~ - x stays the fallback
+and the FP64 relative residual the judge of convergence.
 
-Training data are (features, candidates, PCG solution) triples logged by
-``collect`` from plain-PCG steps; ``WarmStartNet.fit`` trains on the GPU with
-the project's forward/backward kernels and Adam.
+Training data are whole solves logged by ``collect_solves`` from plain-PCG
+steps (features, candidates, right-hand side, matrix, PCG solution);
+``WarmStartNet.fit_solves`` trains on the GPU with the project's
+forward/backward kernels, SpMV and Adam on the energy-norm error of the
+guess, e^T H e with e = dx0 - dx* (the norm CG's error bound is stated in),
+or on the initial residual.  ``validate_cg`` then keeps the trained
+correction only if it takes fewer CG iterations over whole steps than the
+previous Newton step alone.  (``collect`` / ``fit`` train on per-vertex
+(features, PCG solution) pairs with a Euclidean blend loss.)
 """
 from __future__ import annotations
 
@@ -144,52 +152,103 @@ class WarmStartNet:
                   _ptr(self.wbf), _ptr(self.adam), _ptr(self._part), ctypes.c_int(MAX_PARTS), _ptr(grad_out),
                   ctypes.c_float(lr), ctypes.c_int(max(self.t, 1)), self._stream())
 
-    def residual_objective(self, solves: list, solver) -> float:
-        """Mean over solves of log |g - H dx0|^2 for the current weights."""
+    def _solve_loss(self, sv, solver, objective: str, dx0, grad: bool):
+        """(loss, dL/ddx0 or None) of one logged solve.  ``residual``:
+        log |g - H dx0|^2, gradient -2 H r / |r|^2.  ``energy``:
+        log e^T H e with e = dx0 - dx* (the PCG solution of that solve),
+        gradient 2 H e / e^T H e."""
+        if objective == "residual":
+            r = sv["g"] - solver.spmv_with(sv["values"], dx0)
+            rr = torch.sum(r.double() * r.double())
+            gd = (-2.0 / rr).float() * solver.spmv_with(sv["values"], r) if grad else None
+            return torch.log(rr), gd
+        if objective == "energy":
+            e = dx0 - sv["dx_star"]
+            He = solver.spmv_with(sv["values"], e)
+            eHe = torch.sum(e.double() * He.double())
+            return torch.log(eHe), ((2.0 / eHe).float() * He if grad else None)
+        raise ValueError(f"unknown objective {objective!r} (expected 'residual' or 'energy')")
+
+    def objective(self, solves: list, solver, objective: str = "residual", base: bool = False) -> float:
+        """Mean of the per-solve objective for the current weights (base:
+        for the previous Newton step c3 alone, y = 0)."""
         tot = 0.0
         for sv in solves:
             C = sv["cand"].view(-1, 3, 3)
-            dx0 = _blend(C, self.forward(sv["X"]))
-            r = (sv["g"] - solver.spmv_with(sv["values"], dx0)).double()
-            tot += float(torch.log(torch.sum(r * r)).item())
+            dx0 = C[:, 2].contiguous() if base else _blend(C, self.forward(sv["X"]))
+            tot += float(self._solve_loss(sv, solver, objective, dx0, False)[0].item())
         return tot / len(solves)
 
-    def fit_residual(self, solves: list, solver, epochs: int = 20, lr: float = 1e-4, seed: int = 0) -> dict:
-        """Trains on the solver's own objective: for each logged solve, the
-        log of the squared initial residual |g - H dx0|^2 of the corrected
-        guess dx0 = c3 + y0 c1 + y1 c2 + y2 c3, with H that solve's matrix
-        (applied by the project's SpMV) and dL/ddx0 = -2 H r / |r|^2.  One
-        Adam step per solve.  After every epoch the objective is evaluated
-        with fixed weights and the best weights are kept, the starting point
+    def residual_objective(self, solves: list, solver) -> float:
+        """Mean over solves of log |g - H dx0|^2 for the current weights."""
+        return self.objective(solves, solver, "residual")
+
+    def fit_solves(self, solves: list, solver, objective: str = "energy", epochs: int = 20, lr: float = 1e-4,
+                   seed: int = 0) -> dict:
+        """Trains on logged solves: for each, the objective of the corrected
+        guess dx0 = c3 + y0 c1 + y1 c2 + y2 c3 with H that solve's matrix
+        (applied by the project's SpMV); dL/dy = c_k . dL/ddx0.  One Adam
+        step per solve.  After every epoch the objective is evaluated with
+        fixed weights and the best weights are kept, the starting point
         (epoch 0) included, so training never returns a worse guess than it
         started from.  Returns the per-epoch objective, the objective of the
         previous Newton step alone (y = 0) and the epoch kept."""
         g = torch.Generator(device="cpu").manual_seed(seed)
-        hist = {"loss": [self.residual_objective(solves, solver)], "base": None, "best_epoch": 0}
-        base = 0.0
-        for sv in solves:
-            r = (sv["g"] - solver.spmv_with(sv["values"], sv["cand"][:, 6:9])).double()
-            base += float(torch.log(torch.sum(r * r)).item())
-        hist["base"] = base / len(solves)
+        hist = {"objective": objective, "loss": [self.objective(solves, solver, objective)],
+                "base": self.objective(solves, solver, objective, base=True), "best_epoch": 0}
         best_w = self.w.clone()
         for ep in range(epochs):
             for i in torch.randperm(len(solves), generator=g).tolist():
                 sv = solves[i]
                 C = sv["cand"].view(-1, 3, 3)
-                y = self.forward(sv["X"])
-                dx0 = _blend(C, y)
-                r = sv["g"] - solver.spmv_with(sv["values"], dx0)
-                rr = torch.sum(r.double() * r.double())
-                gd = (-2.0 / rr).float() * solver.spmv_with(sv["values"], r)
+                dx0 = _blend(C, self.forward(sv["X"]))
+                _, gd = self._solve_loss(sv, solver, objective, dx0, True)
                 dy = (C * gd[:, None, :]).sum(dim=2)
                 self.backward_step(sv["X"], dy, lr)
-            hist["loss"].append(self.residual_objective(solves, solver))
+            hist["loss"].append(self.objective(solves, solver, objective))
             if hist["loss"][-1] < hist["loss"][hist["best_epoch"]]:
                 hist["best_epoch"] = ep + 1
                 best_w.copy_(self.w)
         self.w.copy_(best_w)
         self.refresh()
         return hist
+
+    def fit_residual(self, solves: list, solver, epochs: int = 20, lr: float = 1e-4, seed: int = 0) -> dict:
+        """fit_solves on the initial-residual objective log |g - H dx0|^2."""
+        return self.fit_solves(solves, solver, "residual", epochs, lr, seed)
+
+    def validate_cg(self, solver, state, dt: float, steps: int = 2) -> dict:
+        """Model selection on the quantity the guess is for: whole steps from
+        ``state`` (the solver's own state restored before each run) with
+        these weights and with the network's correction off (zero output
+        layer: the previous Newton step, the same fallback); keeps the
+        weights with fewer CG iterations (zeroing the output layer when the
+        correction does not help).  Returns both counts and the choice."""
+        snap = solver.snapshot()
+        counts = {}
+        trained = self.w.clone()
+        zero = trained.clone()
+        zero[O_W3:] = 0.0
+        try:
+            for name, w in (("trained", trained), ("prev_newton_step", zero)):
+                self.w.copy_(w)
+                self.refresh()
+                solver.restore(snap)
+                solver.set_warm_start(self)
+                a, b = state.clone(), state.clone()
+                n = 0
+                for k in range(steps):
+                    solver.step(a, b, None, None, dt)
+                    a, b = b, a
+                    n += int(solver.last_stats()[:, 0].sum())
+                counts[name] = n
+        finally:
+            solver.restore(snap)
+            solver.set_warm_start(None)
+        keep = "trained" if counts["trained"] < counts["prev_newton_step"] else "prev_newton_step"
+        self.w.copy_(trained if keep == "trained" else zero)
+        self.refresh()
+        return {"cg_iters": counts, "kept": keep, "steps": steps}
 
     def loss(self, X, aux) -> float:
         out = torch.zeros(1, dtype=torch.float32, device=self.device)
@@ -282,7 +341,8 @@ def collect_solves(solver, state, dt: float, steps: int) -> list:
     """Like ``collect`` but keeps whole solves for the residual objective:
     per Newton iteration the features X (V, 16), the absolute candidates
     cand (V, 9) = (c1, c2, c3), the right-hand side g (V, 3) and a copy of
-    the assembled matrix values (SELL layout, shared pattern)."""
+    the assembled matrix values (SELL layout, shared pattern), and the
+    system's PCG solution dx_star (V, 3)."""
     solves = []
     out = state.clone()
     b = solver.buffers
@@ -298,6 +358,7 @@ def collect_solves(solver, state, dt: float, steps: int) -> list:
             solves.append({"X": b["ws_x"].clone(), "cand": (b["ws_aux"][:, :9] * s).contiguous(),
                            "g": b["g"].clone(), "values": b["sell_val"].clone()})
             solver.global_solve(it, learned=False)
+            solves[-1]["dx_star"] = b["dx"].float()  # the PCG solution of this system
             solver.local_solve()
             solver.line_search(dt, it)
         solver.end_step(dt)

@@ -19,6 +19,10 @@ Scenes:
                       0.05 m), dropped onto the plane z=0 with an IPC barrier
                       (dhat=1e-3) and friction mu=0.3, dt=1/100.
 * ``drop_box_scene``  small Neo-Hookean box over a plane, for parity tests.
+* ``soft_body_batch`` 64 seeded boxes / ellipsoids, 2e3 to 5e4 tets each
+                      (log-uniform, ~1.2e6 in all), mixed ARAP / Neo-Hookean,
+                      E log-uniform in [1e4, 1e6] Pa, started deformed over
+                      the plane z=0 with IPC, dt=1/100 (a batch).
 
 ``Model.params`` holds RefinementSolver keyword arguments (reference names,
 src/mfem/refinement/solver.py:56-128, 241-245) plus ``iterations``.
@@ -193,6 +197,56 @@ def drop_box_scene(cells=(4, 3, 3), size=(0.2, 0.15, 0.15), clearance=5.0e-4, E=
                     line_search=True, line_search_max_iterations=20, contact_d0=0.1 * dhat, contact_d1=dhat,
                     contact_stiffness=auto_kappa(m, dhat), friction_mu=[friction_mu], friction_eps_v=1e-3)
     return m
+
+
+def soft_body_batch(n_bodies: int = 64, seed: int = 5, tets_min: float = 2.0e3, tets_max: float = 5.0e4,
+                    dhat: float = 1.0e-3, friction_mu: float = 0.3, iterations: int = 12,
+                    spacing: float = 1.0) -> list:
+    """``n_bodies`` independent soft bodies, each a box or an ellipsoid
+    (equal odds) with seeded random proportions (edges / diameters 0.15 to
+    0.4 m), lattice-meshed to a tet count drawn log-uniformly from
+    [tets_min, tets_max]; ARAP or Neo-Hookean (equal odds), E log-uniform in
+    [1e4, 1e6] Pa, nu = 0.45, rho = 1000.  Each starts deformed (a random
+    stretch / shear of up to 15 % from its rest shape), 2 mm above the plane
+    z = 0 (IPC barrier dhat, friction mu) with a small random velocity, on an
+    8-wide grid ``spacing`` apart.  Solver parameters are shared (a batch
+    shares them): those of ``octopus_scene`` with the barrier stiffness of
+    the heaviest-vertex body, so every body's contact holds."""
+    rng = np.random.default_rng(seed)
+    models = []
+    for b in range(n_bodies):
+        target = float(np.exp(rng.uniform(np.log(tets_min), np.log(tets_max))))
+        dims = rng.uniform(0.15, 0.4, size=3)
+        ellipsoid = bool(rng.integers(2))
+        vol = np.prod(dims) * (np.pi / 6.0 if ellipsoid else 1.0)
+        h = float((6.0 * vol / target) ** (1.0 / 3.0))  # 6 tets per lattice cube
+        if ellipsoid:
+            inside = lambda p, r=0.5 * dims: ((p / r) ** 2).sum(axis=1) <= 1.0
+            p, t = _mesh.implicit_tet_mesh(inside, -0.5 * dims, 0.5 * dims, h)
+        else:
+            cells = np.maximum(np.round(dims / h).astype(int), 1)
+            p, t = _mesh.box_tet_mesh(-0.5 * dims, 0.5 * dims, cells)
+        energy = "neohookean" if rng.integers(2) else "arap"
+        E = float(np.exp(rng.uniform(np.log(1.0e4), np.log(1.0e6))))
+        mu, lam = _mesh.lame(E, 0.45)
+        F = np.eye(3) + rng.uniform(-0.15, 0.15, size=(3, 3))
+        x = p @ F.T
+        offset = np.array([spacing * (b % 8), spacing * (b // 8), 0.0])
+        x += offset - np.array([0.0, 0.0, x[:, 2].min() - 2.0e-3])
+        vel = np.array([rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), rng.uniform(-0.5, 0.0)])
+        m = build_model(p + offset, t, _materials(mu, lam, len(t)), 1000.0,
+                        shapes=[Shape(SHAPE_PLANE, mu=friction_mu)], velocity=vel,
+                        name=f"{'ellipsoid' if ellipsoid else 'box'}_{len(t)}_{energy}_E{E:.3g}")
+        m.particle_q = x
+        m.params.update(energy=energy)
+        models.append(m)
+    kappa = max(auto_kappa(m, dhat) for m in models)
+    for m in models:
+        m.params.update(iterations=iterations, cg_max_iterations=5000, cg_tolerance=1e-6, line_search=True,
+                        line_search_max_iterations=30, contact_d0=0.1 * dhat, contact_d1=dhat,
+                        contact_stiffness=kappa, friction_mu=[friction_mu], friction_eps_v=1e-3,
+                        attachment_stiffness=10.0)
+    return models
 
 
 def capsule_poke_scene(cells=(6, 6, 2), energy="arap") -> Model:

@@ -302,6 +302,8 @@ class RefinementSolver:
         n = len(bv) - 1
         if n < 1 or np.any(bv[:-1] % 256) or bv[-1] != S.n_verts or S.n_verts % 256:
             raise ValueError("body vertex ranges must start at multiples of 256 and end at n_verts (a multiple of 256)")
+        # workgroups of up to 8 blocks of 256 rows stay within one body
+        align = next(a for a in (8, 4, 2, 1) if not np.any((bv // 256) % a))
         i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=dev)
         b["body_vert"] = i32(bv)
         b["body_tet"] = i32(body_tet)
@@ -323,7 +325,7 @@ class RefinementSolver:
             b["tet_energy"] = i32(np.concatenate([np.asarray(tet_energy, np.int32),
                                                   np.zeros(self.max_tets - len(tet_energy), np.int32)]))
             S.tet_energy = b["tet_energy"].data_ptr()
-        S.n_bodies, S.ls_blocks = n, int(share.sum())
+        S.n_bodies, S.ls_blocks, S.body_wg_align = n, int(share.sum()), align
         for name in ("body_vert", "body_tet", "body_tri", "body_wg", "body_ls_wg", "body_epoch", "stats"):
             setattr(S, name, b[name].data_ptr())
         self.n_bodies = n
@@ -437,7 +439,8 @@ class RefinementSolver:
     def set_warm_start(self, net) -> None:
         """Attach a trained ``learned.WarmStartNet`` (or None for plain PCG).
         With a network, every PCG solve first evaluates it and starts from the
-        learned guess when that has the smaller residual."""
+        lowest quadratic-model energy of x~ - x, the learned guess and the
+        previous Newton step."""
         self.warm_start_net = net
         if net is None:
             self._S.pcg_flags &= ~_ffi.PCG_LEARNED_GUESS
@@ -588,8 +591,20 @@ class RefinementSolver:
         for k in range(n):
             _ffi.call("mfem_pcg_variant_info", ctypes.byref(self._S), ctypes.c_int(k), info)
             out.append(dict(variant=k, chunks_per_wave=info[0], resident_slots=info[1],
-                            mode="pipelined" if info[2] else "standard", workgroups=info[3], applicable=bool(info[4])))
+                            mode=("standard", "pipelined", "standard_lowreg")[info[2]], workgroups=info[3], applicable=bool(info[4])))
         return out
+
+    _STEP_KEYS = ("s", "lam", "dx", "stats", "x_start", "x_tilde", "lag_n", "lag_fn", "lag_sid")
+
+    def snapshot(self) -> dict:
+        """Copies of the solver-held per-tet / per-vertex state that a step
+        reads besides the State (stretch, multipliers, last Newton step,
+        lagged friction, stats), for ``restore``."""
+        return {k: self.buffers[k].clone() for k in self._STEP_KEYS}
+
+    def restore(self, snap: dict) -> None:
+        for k, t in snap.items():
+            self.buffers[k].copy_(t)
 
     def autotune(self, state: "State", dt: float, repeats: int = 2) -> dict:
         """Times every applicable PCG variant on whole steps from ``state``
@@ -599,9 +614,7 @@ class RefinementSolver:
         per-tet / per-vertex state are restored afterwards; refinement passes
         are not run while tuning.  The choice holds until the topology
         changes."""
-        b = self.buffers
-        keys = ("s", "lam", "dx", "stats", "x_start", "x_tilde", "lag_n", "lag_fn", "lag_sid")
-        saved = {k: b[k].clone() for k in keys}
+        saved = self.snapshot()
         scratch = state.clone()
         refine, self._refine_enabled = self._refine_enabled, False
         step, self._step = self._step, self._step
@@ -613,8 +626,7 @@ class RefinementSolver:
                 self._S.pcg_variant = v["variant"]
                 ms, iters = [], []
                 for r in range(repeats + 1):  # the first run warms up
-                    for k in keys:
-                        b[k].copy_(saved[k])
+                    self.restore(saved)
                     a = torch.cuda.Event(enable_timing=True)
                     e = torch.cuda.Event(enable_timing=True)
                     a.record()
@@ -631,8 +643,7 @@ class RefinementSolver:
         finally:
             self._refine_enabled = refine
             self._step = step
-            for k in keys:
-                b[k].copy_(saved[k])
+            self.restore(saved)
         best = min(table, key=lambda r: r["ms_per_step"])
         self._S.pcg_variant = best["variant"]
         return dict(chosen=best, table=table)
@@ -655,10 +666,12 @@ class RefinementSolver:
     def last_stats(self) -> np.ndarray:
         """Per Newton iteration of the last step: cg_iters, cg_relres (FP64),
         ls_trials, ls_accepted, alpha, f0, f1, cg_restarts, PCG phase ticks
-        (6), learned guess used, learned / plain initial residual ratio."""
+        (6), learned guess used, learned / plain initial residual ratio.
+        A batch: the maximum over its bodies of each entry (cg_iters is then
+        the launch's iteration count, cg_relres the worst body's)."""
         torch.cuda.synchronize(self.device)
-        st = self.buffers["stats"].cpu().numpy().reshape(_ffi.MFEM_MAX_STATS_ITERS, _ffi.MFEM_STATS_PER_ITER)
-        return st[: self.iterations].copy()
+        st = self.buffers["stats"].cpu().numpy().reshape(-1, _ffi.MFEM_MAX_STATS_ITERS, _ffi.MFEM_STATS_PER_ITER)
+        return st[:, : self.iterations].max(axis=0)
 
     def status(self) -> int:
         """Device error bits (bit 0: a grid barrier timed out; the persistent
