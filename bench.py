@@ -464,6 +464,7 @@ def run_batch(args):
                                                   float(np.median(per_step_body.mean(axis=0))),
                                                   float(per_step_body.mean(axis=0).max())],
         "cg_relres_max": float(st[..., 1].max()),
+        "phase_ms_per_step": {k: sum(a.elapsed_time(b) for a, b in v) / args.steps for k, v in timers.items()},
         "hardest_bodies": [{"tets": models[i].tet_count, "energy": models[i].params["energy"],
                             "E": float(_young(models[i])), "cg_iters_per_step": float(per_step_body[:, i].mean())}
                            for i in np.argsort(-per_step_body.mean(axis=0))[:4]],
