@@ -64,6 +64,10 @@ def main():
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--objective", default="energy", choices=sorted(_OBJECTIVES),
                     help="training objective of the warm-start network")
+    ap.add_argument("--local-sweeps", type=int, default=None,
+                    help="workgroup-local block-Jacobi sweeps added to the preconditioner of the accelerated legs "
+                         "(even; 0 = block Jacobi as the reference; default: the autotuner picks 0 or 2)")
+    ap.add_argument("--local-omega", type=float, default=0.5)
     ap.add_argument("--no-learned", action="store_true", help="time plain PCG only")
     ap.add_argument("--dump-iters", default=None,
                     help="write the CG iteration count of every PCG launch, in launch order, to this JSON file "
@@ -151,19 +155,24 @@ def main():
 
     tune = None
     if not args.no_autotune:
-        # whole steps with the warm start the timed leg uses
+        # whole steps with the warm start and preconditioner the timed leg uses
         solver.set_warm_start(net)
-        tune = solver.autotune(state, dt)
+        solver.set_local_preconditioner(0, args.local_omega)
+        tune = solver.autotune(state, dt, local_sweeps=_sweep_options(args))
         c = tune["chosen"]
+        args.local_sweeps = c["local_sweeps"]
         print(f"[bench] autotuned PCG: variant {c['variant']} ({c['chunks_per_wave']} chunk(s)/wave, "
-              f"{c['resident_slots']} resident slots, {c['mode']}), {c['ms_per_step']:.2f} ms/step, "
-              f"{c['us_per_iter']:.2f} us/iter", flush=True)
+              f"{c['resident_slots']} resident slots, {c['mode']}), {c['local_sweeps']} local sweeps, "
+              f"{c['ms_per_step']:.2f} ms/step, {c['us_per_iter']:.2f} us/iter", flush=True)
+    if args.local_sweeps is None:
+        args.local_sweeps = 0
 
-    def timed_leg(s0, use_net):
+    def timed_leg(s0, use_net, sweeps=0):
         st = s0.clone()
         o = st.clone()
         solver.restore(snap0)
         solver.set_warm_start(use_net)
+        solver.set_local_preconditioner(sweeps, args.local_omega)
         timers: dict = {}
         hist = []
         torch.cuda.synchronize()
@@ -191,14 +200,17 @@ def main():
                        "verts": m.particle_count, "nnz_blocks": int(len(solver.pattern.cols))}, f)
     print(f"[bench] plain: {args.steps} steps in {wall_p:.3f} s", flush=True)
     if learned:
-        wall, st, timers, end_l = timed_leg(s0, net)
-        print(f"[bench] learned: {args.steps} steps in {wall:.3f} s", flush=True)
+        wall, st, timers, end_l = timed_leg(s0, net, args.local_sweeps)
+        print(f"[bench] learned + local preconditioner: {args.steps} steps in {wall:.3f} s", flush=True)
         pos_diff = float((end_l.particle_q - end_p.particle_q).abs().max().item())
-        # ablation: the same warm start with the network's correction off
-        # (zero output layer: the previous Newton step, same fallback)
+        # ablations: the same with the network's correction off (zero output
+        # layer: the previous Newton step, same fallback), and the learned
+        # start with the reference's block-Jacobi preconditioner
         base_net = WarmStartNet("cuda:0", weights=np.zeros(net.w.numel(), np.float32))
-        wall_b, st_b, _, _ = timed_leg(s0, base_net)
+        wall_b, st_b, _, _ = timed_leg(s0, base_net, args.local_sweeps)
         print(f"[bench] previous Newton step only: {args.steps} steps in {wall_b:.3f} s", flush=True)
+        wall_j, st_j, _, _ = timed_leg(s0, net, 0)
+        print(f"[bench] learned, block Jacobi: {args.steps} steps in {wall_j:.3f} s", flush=True)
     else:
         wall, st, timers = wall_p, st_p, timers_p
 
@@ -234,7 +246,9 @@ def main():
         "verts": V,
         "newton_iterations": solver.iterations,
         "dt": dt,
-        "preconditioner": "block_jacobi",
+        "preconditioner": {"plain": "block_jacobi",
+                           "value": f"block_jacobi + {args.local_sweeps} workgroup-local sweeps (omega "
+                                    f"{args.local_omega})" if args.local_sweeps else "block_jacobi"},
         "pcg": args.pcg,
         "learned_warm_start": learned,
         "reductions": "f64",
@@ -255,7 +269,8 @@ def main():
         config["pcg_autotune"] = {
             "chosen": tune["chosen"]["variant"],
             "table": [{k: r[k] for k in ("variant", "chunks_per_wave", "resident_slots", "mode", "workgroups",
-                                         "ms_per_step", "us_per_iter", "cg_iters")} for r in tune["table"]]}
+                                         "local_sweeps", "ms_per_step", "us_per_iter", "cg_iters")}
+                      for r in tune["table"]]}
     if learned:
         config["pcg_start_frac"] = {name: float((st[:, :, 14] == k).mean()) for k, name in
                                     enumerate(["x_tilde_minus_x", "learned", "prev_newton_step"])}
@@ -265,6 +280,9 @@ def main():
         config["ablation_prev_newton_step_only"] = {
             "value": T * args.steps / wall_b, "ms_per_step": wall_b * 1e3 / args.steps,
             "cg_iters_per_step": float(st_b[:, :, 0].sum(axis=1).mean())}
+        config["ablation_block_jacobi"] = {
+            "value": T * args.steps / wall_j, "ms_per_step": wall_j * 1e3 / args.steps,
+            "cg_iters_per_step": float(st_j[:, :, 0].sum(axis=1).mean())}
     result = {
         "metric": "tet-steps/s at CG rel-res 1e-6, learned-precond vs plain PCG, 1 MI355X",
         "value": value,
@@ -367,15 +385,21 @@ def run_batch(args):
     tune = None
     if not args.no_autotune:
         solver.set_warm_start(net)
-        tune = solver.autotune(state, dt)
+        solver.set_local_preconditioner(0, args.local_omega)
+        tune = solver.autotune(state, dt, local_sweeps=_sweep_options(args))
         c = tune["chosen"]
+        args.local_sweeps = c["local_sweeps"]
         print(f"[bench] autotuned PCG: variant {c['variant']} ({c['chunks_per_wave']} chunk(s)/wave, "
-              f"{c['resident_slots']} resident slots, {c['mode']}), {c['ms_per_step']:.2f} ms/step", flush=True)
+              f"{c['resident_slots']} resident slots, {c['mode']}), {c['local_sweeps']} local sweeps, "
+              f"{c['ms_per_step']:.2f} ms/step", flush=True)
+    if args.local_sweeps is None:
+        args.local_sweeps = 0
 
-    def timed_leg(s0, use_net):
+    def timed_leg(s0, use_net, sweeps=0):
         st, o = s0.clone(), s0.clone()
         solver.restore(snap0)
         solver.set_warm_start(use_net)
+        solver.set_local_preconditioner(sweeps, args.local_omega)
         timers: dict = {}
         hist = []
         torch.cuda.synchronize()
@@ -392,8 +416,9 @@ def run_batch(args):
         assert np.all(stats[..., 1] <= 1e-6), "a body's PCG solve missed rel-res 1e-6"
         return wall, stats, timers, st
 
-    def graph_leg(s0, use_net):
+    def graph_leg(s0, use_net, sweeps=0):
         solver.set_warm_start(use_net)
+        solver.set_local_preconditioner(sweeps, args.local_omega)
         a, b = s0.clone(), s0.clone()
         solver.restore(snap0)
         g_ab = solver.capture_step(a, b, dt)
@@ -416,11 +441,13 @@ def run_batch(args):
     wall_p, st_p, _, end_p = timed_leg(s0, None)
     print(f"[bench] plain: {args.steps} steps in {wall_p:.3f} s", flush=True)
     if learned:
-        wall_e, st, timers, end_l = timed_leg(s0, net)
-        print(f"[bench] learned: {args.steps} steps in {wall_e:.3f} s", flush=True)
+        wall_e, st, timers, end_l = timed_leg(s0, net, args.local_sweeps)
+        print(f"[bench] learned + local preconditioner: {args.steps} steps in {wall_e:.3f} s", flush=True)
+        wall_j, st_j, _, _ = timed_leg(s0, net, 0)
+        print(f"[bench] learned, block Jacobi: {args.steps} steps in {wall_j:.3f} s", flush=True)
     else:
         wall_e, st, timers = wall_p, st_p, _
-    wall_g = graph_leg(s0, net)
+    wall_g = graph_leg(s0, net, args.local_sweeps)
     print(f"[bench] learned, HIP graph replays: {args.steps} steps in {wall_g:.3f} s", flush=True)
     wall = min(wall_e, wall_g)
 
@@ -449,7 +476,9 @@ def run_batch(args):
         "neohookean_bodies": int(sum(m.params["energy"] == "neohookean" for m in models)),
         "newton_iterations": solver.iterations,
         "dt": dt,
-        "preconditioner": "block_jacobi",
+        "preconditioner": {"plain": "block_jacobi",
+                           "value": f"block_jacobi + {args.local_sweeps} workgroup-local sweeps (omega "
+                                    f"{args.local_omega})" if args.local_sweeps else "block_jacobi"},
         "learned_warm_start": learned,
         "reductions": "f64, per body",
         "launches_per_newton_iteration": {"k_pcg": 1, "k_linesearch": 1},
@@ -473,11 +502,15 @@ def run_batch(args):
         config["pcg_autotune"] = {
             "chosen": tune["chosen"]["variant"],
             "table": [{k: r[k] for k in ("variant", "chunks_per_wave", "resident_slots", "mode", "workgroups",
-                                         "ms_per_step", "us_per_iter", "cg_iters")} for r in tune["table"]]}
+                                         "local_sweeps", "ms_per_step", "us_per_iter", "cg_iters")}
+                      for r in tune["table"]]}
     if learned:
         config["training"] = train_info
         config["pcg_start_frac"] = {name: float((st[..., 14] == k).mean()) for k, name in
                                     enumerate(["x_tilde_minus_x", "learned", "prev_newton_step"])}
+        config["ablation_block_jacobi"] = {
+            "value": T * args.steps / wall_j, "ms_per_step": wall_j * 1e3 / args.steps,
+            "cg_iters_per_step_launch": float(st_j[..., 0].max(axis=1).sum(axis=1).mean())}
     result = {
         "metric": "tet-steps/s at CG rel-res 1e-6, learned-precond vs plain PCG, 1 MI355X",
         "value": value,
@@ -533,6 +566,10 @@ def cpu_baseline_bodies(models, dt, budget_tets: int = 150_000):
     return {"value": tets / sec, "unit": "tet-steps/s", "cores": 1, "kind": "port",
             "sample": f"1 step (12 Newton iterations) of the first {n} bodies ({tets} tets) from their initial "
                       f"states, one body at a time, FP64 C oracle, 1 thread, {sec:.1f} s"}
+
+
+def _sweep_options(args):
+    return (0, 2) if args.local_sweeps is None else (args.local_sweeps,)
 
 
 def mlp_mfma(solver, net, V):

@@ -192,7 +192,15 @@ typedef struct mfem_solver {
   int32_t* body_epoch;       /* [n_bodies] grid-barrier epochs */
   const int32_t* tet_energy; /* [n_tets] per-tet energy (0 ARAP, 1 Neo-Hookean) or null: `energy` for all */
   int32_t body_wg_align;     /* a batch: 1, 2, 4 or 8, dividing every body_wg[b] */
-  int32_t pad2;
+  /* workgroup-local preconditioner (no counterpart in the reference, whose
+     CG uses block Jacobi): 0 = block Jacobi z = D^-1 r; an even count s > 0
+     = s damped block-Jacobi sweeps on the rows of the PCG workgroup,
+     z <- z + omega D^-1 (r - A_loc z) from z = D^-1 r, with A_loc the matrix
+     restricted to the workgroup's rows: a polynomial in D^-1 A_loc times
+     D^-1, symmetric positive definite for every even s and omega > 0.
+     Standard (non-pipelined) PCG variants only. */
+  int32_t local_sweeps;
+  float local_omega;
 } mfem_solver;
 
 /* ABI self-description (checked against the Python mirror by the tests) */

@@ -27,10 +27,13 @@ def _make(m, gpu, **extra):
 def test_autotune_keeps_results(gpu):
     m = scene.drop_box_scene()
     solver, state = _make(m, gpu)
-    res = solver.autotune(state, 0.01)
-    print("autotune:", [(r["variant"], r["chunks_per_wave"], r["resident_slots"], r["mode"],
+    res = solver.autotune(state, 0.01, local_sweeps=(0, 2))
+    print("autotune:", [(r["variant"], r["chunks_per_wave"], r["resident_slots"], r["mode"], r["local_sweeps"],
                          round(r["ms_per_step"], 3), r["cg_iters"]) for r in res["table"]], "chosen", res["chosen"]["variant"])
     assert len(res["table"]) >= 2
+    assert {r["local_sweeps"] for r in res["table"]} == {0, 2}
+    assert int(solver._S.local_sweeps) == res["chosen"]["local_sweeps"]
+    assert int(solver._S.pcg_variant) == res["chosen"]["variant"]
     # the state is untouched by tuning
     np.testing.assert_array_equal(state.particle_q.cpu().numpy(), m.particle_q.astype(np.float32))
     o = Oracle(m, 64)

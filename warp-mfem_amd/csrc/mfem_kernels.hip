@@ -929,6 +929,91 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
   int it = 0, restarts = 0;
   double rt2 = 0.0;
 
+  // Standard mode: z = M r for the wave's chunks.  Block Jacobi, then (when
+  // local_sweeps > 0) damped block-Jacobi sweeps on the workgroup's own rows,
+  // z <- z + omega D^-1 (r - A_loc z), with the workgroup's z exchanged in
+  // LDS: A_loc keeps only the columns whose rows this workgroup owns.
+  __shared__ float4 zl[CPW * kWaves * 64];
+  const int wave = threadIdx.x >> 6;
+  // LDS slot of column vertex c, or -1 when another workgroup owns it (one
+  // chunk per wave: workgroup w owns rows [256 w, 256 w + 256) in both
+  // layouts)
+  auto local_pos = [&](int c) -> int {
+    if constexpr (CPW == 1) {
+      return (c >> 8) == (int)blockIdx.x ? (c & 255) : -1;
+    } else {
+      const int cc = c >> 6;
+      int li;
+      if (batch) {
+        li = cc - (int)blockIdx.x * kWaves * CPW;
+        if (li < 0 || li >= kWaves * CPW) return -1;
+      } else {
+        const int g = cc % nw;
+        if (g / kWaves != (int)blockIdx.x) return -1;
+        li = (cc / nw) * kWaves + g % kWaves;
+      }
+      return li * 64 + (c & 63);
+    }
+  };
+  auto precondition = [&]() {
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+      if (!own[k]) continue;
+      reload_M(k);
+      apply_M<CPW, float>(M, k, r[k], z[k]);
+    }
+    const int ns = S.local_sweeps;
+    const float om = S.local_omega;
+    for (int sw = 0; sw < ns; ++sw) {
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) {
+        if (!own[k]) continue;
+        zl[(k * kWaves + wave) * 64 + lane] = make_float4(z[k][0], z[k][1], z[k][2], 0.f);
+      }
+      __syncthreads();
+      float t[CPW][3];
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) {
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+        auto add = [&](int c, const float* v) {
+          const int pos = local_pos(c);
+          if (pos < 0) return;
+          const float4 zj = zl[pos];
+          const float z0 = zj.x, z1 = zj.y, z2 = zj.z;
+          a0 += v[0] * z0 + v[1] * z1 + v[2] * z2;
+          a1 += v[3] * z0 + v[4] * z1 + v[5] * z2;
+          a2 += v[6] * z0 + v[7] * z1 + v[8] * z2;
+        };
+        if (own[k]) {
+          if constexpr (RW > 0) {
+#pragma unroll
+            for (int kk = 0; kk < RW; ++kk) add(rb.col[kk], rb.val[kk]);
+          } else {
+            for (int sl = S.chunk_ptr[chunk[k]]; sl < S.chunk_ptr[chunk[k] + 1]; ++sl) {
+              float v[9];
+              const float* vb = S.sell_val + (size_t)sl * 576 + lane;
+#pragma unroll
+              for (int q = 0; q < 9; ++q) v[q] = vb[64 * q];
+              add(S.sell_col[sl * 64 + lane], v);
+            }
+          }
+        }
+        t[k][0] = (float)r[k][0] - a0;
+        t[k][1] = (float)r[k][1] - a1;
+        t[k][2] = (float)r[k][2] - a2;
+      }
+#pragma unroll
+      for (int k = 0; k < CPW; ++k) {
+        if (!own[k]) continue;
+        reload_M(k);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          z[k][a] += om * (M[k][3 * a] * t[k][0] + M[k][3 * a + 1] * t[k][1] + M[k][3 * a + 2] * t[k][2]);
+      }
+      __syncthreads();
+    }
+  };
+
   // (re)start from the current r.  Standard: z = M r published, (r.z, r.r)
   // reduced.  Pipelined: u = M r published (FP64), then w = H u.
   auto restart = [&](double& rz, double& rr) -> bool {
@@ -977,12 +1062,11 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       spmv_rows64<CPW, RW, true>(S, chunk, own, lane, rb, ub, w);
     } else {
       float* zb = pzbuf(pub);
+      precondition();
 #pragma unroll
       for (int k = 0; k < CPW; ++k) {
         if (!own[k]) continue;
         const int row = chunk[k] * 64 + lane;
-        reload_M(k);
-        apply_M<CPW, float>(M, k, r[k], z[k]);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           pub_store(&zb[3 * row + a], z[k][a]);
@@ -1260,14 +1344,17 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
 #pragma unroll
         for (int k = 0; k < CPW; ++k) {
           if (!own[k]) continue;
-          const int row = chunk[k] * 64 + lane;
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
             x[k][a] += alpha * (double)p[k][a];
             r[k][a] -= alpha * (double)q[k][a];
           }
-          reload_M(k);
-          apply_M<CPW, float>(M, k, r[k], z[k]);
+        }
+        precondition();
+#pragma unroll
+        for (int k = 0; k < CPW; ++k) {
+          if (!own[k]) continue;
+          const int row = chunk[k] * 64 + lane;
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
             pub_store(&zb[3 * row + a], z[k][a]);
@@ -1773,6 +1860,9 @@ int check_solver(const mfem_solver* S) {
   if (S->max_width <= 0) return fail(MFEM_EINVAL, "max_width must be the widest chunk (slots)");
   if (S->n_shapes < 0 || S->n_tris < 0) return fail(MFEM_EINVAL, "negative count");
   if (S->iterations > MFEM_MAX_STATS_ITERS) return fail(MFEM_EINVAL, "iterations > MFEM_MAX_STATS_ITERS");
+  if (S->local_sweeps < 0 || S->local_sweeps > 16 || (S->local_sweeps & 1) ||
+      (S->local_sweeps > 0 && !(S->local_omega > 0.f)))
+    return fail(MFEM_EINVAL, "local_sweeps must be even in [0, 16] and local_omega > 0");
   if (!S->x || !S->v || !S->s || !S->lam || !S->tets || !S->sell_val || !S->status || !S->stats)
     return fail(MFEM_EINVAL, "null buffer");
   if (S->n_bodies > 1) {
@@ -1811,7 +1901,7 @@ int launched() {
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
   X(lag_sid) X(G) X(Hl) X(gl) X(Hsinv) X(gs) X(psi) X(Kb) X(fe) X(gx) X(gB) X(gD) X(Hv) X(dist) X(eB)      \
-  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(pad2)
+  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega)
 
 extern "C" {
 
@@ -1889,6 +1979,7 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
     const PcgVariant& v = kPcgVariants[k];
     const bool want = (S->pcg_flags & MFEM_PCG_PIPELINED) != 0;
     if ((v.mode == 1) != want) continue;
+    if (v.mode == 1 && S->local_sweeps > 0) continue;  // the local preconditioner is standard-mode only
     if (S->n_bodies > 1 && S->body_wg_align % v.cpw != 0) continue;  // a batch: body-aligned workgroups
     if (v.rw > 0 && ((S->pcg_flags & MFEM_PCG_STREAM_ROWS) || S->max_width > v.rw)) continue;
     long waves = (S->n_chunks + v.cpw - 1) / v.cpw;
@@ -1982,7 +2073,7 @@ int mfem_pcg_variant_info(const mfem_solver* S, int k, int32_t* info) {
   info[2] = v.mode;
   info[3] = (int32_t)nb;
   info[4] = (nb <= ci->max_blocks_pcg[k] && nb <= MFEM_RED_MAX_BLOCKS && (v.rw == 0 || S->max_width <= v.rw) &&
-             (S->n_bodies <= 1 || S->body_wg_align % v.cpw == 0))
+             (S->n_bodies <= 1 || S->body_wg_align % v.cpw == 0) && !(v.mode == 1 && S->local_sweeps > 0))
                 ? 1
                 : 0;
   return MFEM_OK;

@@ -217,6 +217,8 @@ class RefinementSolver:
         S.pcg_flags = (_ffi.PCG_PIPELINED if kwargs.get("pcg_pipelined", False) else 0) | \
                       (0 if kwargs.get("pcg_resident_rows", True) else _ffi.PCG_STREAM_ROWS)
         S.pcg_variant = -1  # chosen from pcg_flags until autotune() picks one
+        self.set_local_preconditioner(int(kwargs.get("pcg_local_sweeps", 0)),
+                                      float(kwargs.get("pcg_local_omega", 0.5)), S)
         S.energy = _ENERGIES[energy]
         S.iterations = self.iterations
         S.cg_maxiter = int(kwargs.get("cg_max_iterations", 1000))
@@ -594,6 +596,23 @@ class RefinementSolver:
                             mode=("standard", "pipelined", "standard_lowreg")[info[2]], workgroups=info[3], applicable=bool(info[4])))
         return out
 
+    def set_local_preconditioner(self, sweeps: int, omega: float = 0.5, _S=None) -> None:
+        """Preconditioner of the standard PCG: sweeps = 0 is the reference's
+        block Jacobi; an even sweeps > 0 adds that many damped block-Jacobi
+        sweeps (weight omega) over the rows of each PCG workgroup, a
+        symmetric positive definite polynomial preconditioner with no
+        synchronisation beyond the workgroup.  Pipelined PCG variants are
+        not used while sweeps > 0."""
+        S = self._S if _S is None else _S
+        if sweeps < 0 or sweeps > 16 or sweeps % 2 or (sweeps and not omega > 0.0):
+            raise ValueError("pcg_local_sweeps must be even in [0, 16] and pcg_local_omega > 0")
+        S.local_sweeps, S.local_omega = int(sweeps), float(omega)
+        if _S is None and S.pcg_variant >= 0 and sweeps:
+            info = (ctypes.c_int32 * 5)()
+            _ffi.call("mfem_pcg_variant_info", ctypes.byref(S), ctypes.c_int(S.pcg_variant), info)
+            if not info[4]:
+                S.pcg_variant = -1
+
     _STEP_KEYS = ("s", "lam", "dx", "stats", "x_start", "x_tilde", "lag_n", "lag_fn", "lag_sid")
 
     def snapshot(self) -> dict:
@@ -606,45 +625,54 @@ class RefinementSolver:
         for k, t in snap.items():
             self.buffers[k].copy_(t)
 
-    def autotune(self, state: "State", dt: float, repeats: int = 2) -> dict:
-        """Times every applicable PCG variant on whole steps from ``state``
-        (all Newton iterations, with the warm start if one is attached: the
-        variants differ in iteration counts as well as in cost per
-        iteration) and keeps the fastest.  The state and the solver's
-        per-tet / per-vertex state are restored afterwards; refinement passes
-        are not run while tuning.  The choice holds until the topology
-        changes."""
+    def autotune(self, state: "State", dt: float, repeats: int = 2, local_sweeps=None) -> dict:
+        """Times every applicable PCG variant, under each workgroup-local
+        preconditioner setting in ``local_sweeps`` (default: the current one
+        only), on whole steps from ``state`` (all Newton iterations, with the
+        warm start if one is attached: the candidates differ in iteration
+        counts as well as in cost per iteration) and keeps the fastest pair.
+        The state and the solver's per-tet / per-vertex state are restored
+        afterwards; refinement passes are not run while tuning.  The choice
+        holds until the topology changes."""
         saved = self.snapshot()
         scratch = state.clone()
         refine, self._refine_enabled = self._refine_enabled, False
         step, self._step = self._step, self._step
+        omega = float(self._S.local_omega) if self._S.local_omega > 0 else 0.5
+        sweeps_list = [int(self._S.local_sweeps)] if local_sweeps is None else [int(x) for x in local_sweeps]
         table = []
         try:
-            for v in self.pcg_variants():
-                if not v["applicable"]:
-                    continue
-                self._S.pcg_variant = v["variant"]
-                ms, iters = [], []
-                for r in range(repeats + 1):  # the first run warms up
-                    self.restore(saved)
-                    a = torch.cuda.Event(enable_timing=True)
-                    e = torch.cuda.Event(enable_timing=True)
-                    a.record()
-                    self.step(state, scratch, None, None, dt)
-                    e.record()
-                    torch.cuda.synchronize(self.device)
-                    if self.status() != 0:
-                        raise RuntimeError("device error while autotuning")
-                    if r > 0:
-                        ms.append(a.elapsed_time(e))
-                        iters.append(int(self.last_stats()[:, 0].sum()))
-                table.append(dict(v, ms_per_step=float(np.median(ms)), cg_iters=int(np.median(iters)),
-                                  us_per_iter=float(np.median(ms)) * 1e3 / max(int(np.median(iters)), 1)))
+            for sw in sweeps_list:
+                self._S.pcg_variant = -1
+                self.set_local_preconditioner(sw, omega)
+                for v in self.pcg_variants():
+                    if not v["applicable"]:
+                        continue
+                    self._S.pcg_variant = v["variant"]
+                    ms, iters = [], []
+                    for r in range(repeats + 1):  # the first run warms up
+                        self.restore(saved)
+                        a = torch.cuda.Event(enable_timing=True)
+                        e = torch.cuda.Event(enable_timing=True)
+                        a.record()
+                        self.step(state, scratch, None, None, dt)
+                        e.record()
+                        torch.cuda.synchronize(self.device)
+                        if self.status() != 0:
+                            raise RuntimeError("device error while autotuning")
+                        if r > 0:
+                            ms.append(a.elapsed_time(e))
+                            iters.append(int(self.last_stats()[:, 0].sum()))
+                    table.append(dict(v, local_sweeps=sw, ms_per_step=float(np.median(ms)),
+                                      cg_iters=int(np.median(iters)),
+                                      us_per_iter=float(np.median(ms)) * 1e3 / max(int(np.median(iters)), 1)))
         finally:
             self._refine_enabled = refine
             self._step = step
             self.restore(saved)
         best = min(table, key=lambda r: r["ms_per_step"])
+        self._S.pcg_variant = -1
+        self.set_local_preconditioner(best["local_sweeps"], omega)
         self._S.pcg_variant = best["variant"]
         return dict(chosen=best, table=table)
 
