@@ -933,7 +933,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
   // local_sweeps > 0) damped block-Jacobi sweeps on the workgroup's own rows,
   // z <- z + omega D^-1 (r - A_loc z), with the workgroup's z exchanged in
   // LDS: A_loc keeps only the columns whose rows this workgroup owns.
-  __shared__ float4 zl[CPW * kWaves * 64];
+  __shared__ float4 zl[2][CPW * kWaves * 64];  // alternate buffers: one barrier per sweep
   const int wave = threadIdx.x >> 6;
   // LDS slot of column vertex c, or -1 when another workgroup owns it (one
   // chunk per wave: workgroup w owns rows [256 w, 256 w + 256) in both
@@ -965,10 +965,13 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
     const int ns = S.local_sweeps;
     const float om = S.local_omega;
     for (int sw = 0; sw < ns; ++sw) {
+      // (sweep sw writes buffer sw & 1; a wave can only rewrite it two
+      // sweeps later, after every wave has passed the next barrier)
+      float4* zb4 = zl[sw & 1];
 #pragma unroll
       for (int k = 0; k < CPW; ++k) {
         if (!own[k]) continue;
-        zl[(k * kWaves + wave) * 64 + lane] = make_float4(z[k][0], z[k][1], z[k][2], 0.f);
+        zb4[(k * kWaves + wave) * 64 + lane] = make_float4(z[k][0], z[k][1], z[k][2], 0.f);
       }
       __syncthreads();
       float t[CPW][3];
@@ -978,7 +981,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         auto add = [&](int c, const float* v) {
           const int pos = local_pos(c);
           if (pos < 0) return;
-          const float4 zj = zl[pos];
+          const float4 zj = zb4[pos];
           const float z0 = zj.x, z1 = zj.y, z2 = zj.z;
           a0 += v[0] * z0 + v[1] * z1 + v[2] * z2;
           a1 += v[3] * z0 + v[4] * z1 + v[5] * z2;
@@ -1010,8 +1013,9 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         for (int a = 0; a < 3; ++a)
           z[k][a] += om * (M[k][3 * a] * t[k][0] + M[k][3 * a + 1] * t[k][1] + M[k][3 * a + 2] * t[k][2]);
       }
-      __syncthreads();
     }
+    // (the next call's first writes come after a grid reduction, whose
+    // workgroup barrier ends this call's reads)
   };
 
   // (re)start from the current r.  Standard: z = M r published, (r.z, r.r)
