@@ -12,10 +12,13 @@ solves are logged (features, candidates, right-hand side, matrix, PCG
 solution); training of the warm-start MLP on the GPU on the energy-norm error
 of its guess (``--objective``; BF16 MFMA forward/backward, SpMV, Adam) and
 model selection by CG iterations over two validation steps (the trained
-correction or none); then K timed steps with plain PCG, K with the learned
-warm start and, as an ablation, K with the network's correction switched off
-(previous Newton step as the guess), all from the same state and solver
-state.  One "step" = RefinementSolver.step on the whole body.
+correction or none); the autotuner's choice of PCG variant and workgroup-local
+preconditioner sweeps (0 or 2) on whole steps; then K timed steps with plain
+PCG (the reference's start and block Jacobi), K with the learned warm start
+and the tuned preconditioner (the value), and as ablations K with the
+network's correction switched off (previous Newton step as the guess) and K
+with the learned start under block Jacobi, all from the same state and
+solver state.  One "step" = RefinementSolver.step on the whole body.
 value = tets * K / wall time of the learned leg (inputs resident in HBM);
 the plain leg's rate is reported beside it (config.plain_value).
 
@@ -195,6 +198,7 @@ def main():
     wall_p, st_p, timers_p, end_p = timed_leg(s0, None)
     if args.dump_iters:
         launch_iters += st_p[:, :, 0].astype(int).reshape(-1).tolist()
+        os.makedirs(os.path.dirname(os.path.abspath(args.dump_iters)), exist_ok=True)
         with open(args.dump_iters, "w") as f:
             json.dump({"kernel": "k_pcg", "iters_per_launch": launch_iters, "tets": m.tet_count,
                        "verts": m.particle_count, "nnz_blocks": int(len(solver.pattern.cols))}, f)
