@@ -933,7 +933,11 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
   // local_sweeps > 0) damped block-Jacobi sweeps on the workgroup's own rows,
   // z <- z + omega D^-1 (r - A_loc z), with the workgroup's z exchanged in
   // LDS: A_loc keeps only the columns whose rows this workgroup owns.
-  __shared__ float4 zl[2][CPW * kWaves * 64];  // alternate buffers: one barrier per sweep
+  // alternate buffers (one barrier per sweep); slot kZero of each holds zeros
+  // for the columns of other workgroups
+  constexpr int kZero = CPW * kWaves * 64;
+  __shared__ float4 zl[2][kZero + 1];
+  if (threadIdx.x < 2) zl[threadIdx.x][kZero] = make_float4(0.f, 0.f, 0.f, 0.f);
   const int wave = threadIdx.x >> 6;
   // LDS slot of column vertex c, or -1 when another workgroup owns it (one
   // chunk per wave: workgroup w owns rows [256 w, 256 w + 256) in both
@@ -955,6 +959,15 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       return li * 64 + (c & 63);
     }
   };
+  // resident rows: each slot's LDS slot for the sweeps, found once
+  int lp[RW > 0 ? RW : 1];
+  if constexpr (RW > 0) {
+#pragma unroll
+    for (int kk = 0; kk < RW; ++kk) {
+      const int pos = local_pos(rb.col[kk]);
+      lp[kk] = pos < 0 ? kZero : pos;
+    }
+  }
   auto precondition = [&]() {
 #pragma unroll
     for (int k = 0; k < CPW; ++k) {
@@ -990,7 +1003,13 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         if (own[k]) {
           if constexpr (RW > 0) {
 #pragma unroll
-            for (int kk = 0; kk < RW; ++kk) add(rb.col[kk], rb.val[kk]);
+            for (int kk = 0; kk < RW; ++kk) {
+              const float4 zj = zb4[lp[kk]];
+              const float* v = rb.val[kk];
+              a0 += v[0] * zj.x + v[1] * zj.y + v[2] * zj.z;
+              a1 += v[3] * zj.x + v[4] * zj.y + v[5] * zj.z;
+              a2 += v[6] * zj.x + v[7] * zj.y + v[8] * zj.z;
+            }
           } else {
             for (int sl = S.chunk_ptr[chunk[k]]; sl < S.chunk_ptr[chunk[k] + 1]; ++sl) {
               float v[9];
