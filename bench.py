@@ -13,7 +13,7 @@ solution); training of the warm-start MLP on the GPU on the energy-norm error
 of its guess (``--objective``; BF16 MFMA forward/backward, SpMV, Adam) and
 model selection by CG iterations over two validation steps (the trained
 correction or none); the autotuner's choice of PCG variant and workgroup-local
-preconditioner sweeps (0 or 2) on whole steps; then K timed steps with plain
+preconditioner sweeps (0, 2 or 4) on whole steps; then K timed steps with plain
 PCG (the reference's start and block Jacobi), K with the learned warm start
 and the tuned preconditioner (the value), and as ablations K with the
 network's correction switched off (previous Newton step as the guess) and K
@@ -69,7 +69,7 @@ def main():
                     help="training objective of the warm-start network")
     ap.add_argument("--local-sweeps", type=int, default=None,
                     help="workgroup-local block-Jacobi sweeps added to the preconditioner of the accelerated legs "
-                         "(even; 0 = block Jacobi as the reference; default: the autotuner picks 0 or 2)")
+                         "(even; 0 = block Jacobi as the reference; default: the autotuner picks 0, 2 or 4)")
     ap.add_argument("--local-omega", type=float, default=0.5)
     ap.add_argument("--no-learned", action="store_true", help="time plain PCG only")
     ap.add_argument("--dump-iters", default=None,
@@ -498,6 +498,8 @@ def run_batch(args):
                                                   float(per_step_body.mean(axis=0).max())],
         "cg_relres_max": float(st[..., 1].max()),
         "phase_ms_per_step": {k: sum(a.elapsed_time(b) for a, b in v) / args.steps for k, v in timers.items()},
+        "ls_trials_per_newton_iteration": {"max_over_bodies_mean": float(st[..., 2].max(axis=1).mean()),
+                                           "mean_over_bodies": float(st[..., 2].mean())},
         "hardest_bodies": [{"tets": models[i].tet_count, "energy": models[i].params["energy"],
                             "E": float(_young(models[i])), "cg_iters_per_step": float(per_step_body[:, i].mean())}
                            for i in np.argsort(-per_step_body.mean(axis=0))[:4]],
@@ -573,7 +575,7 @@ def cpu_baseline_bodies(models, dt, budget_tets: int = 150_000):
 
 
 def _sweep_options(args):
-    return (0, 2) if args.local_sweeps is None else (args.local_sweeps,)
+    return (0, 2, 4) if args.local_sweeps is None else (args.local_sweeps,)
 
 
 def mlp_mfma(solver, net, V):

@@ -650,6 +650,7 @@ class RefinementSolver:
                         continue
                     self._S.pcg_variant = v["variant"]
                     ms, iters = [], []
+                    best_ms = min((t["ms_per_step"] for t in table), default=float("inf"))
                     for r in range(repeats + 1):  # the first run warms up
                         self.restore(saved)
                         a = torch.cuda.Event(enable_timing=True)
@@ -660,9 +661,12 @@ class RefinementSolver:
                         torch.cuda.synchronize(self.device)
                         if self.status() != 0:
                             raise RuntimeError("device error while autotuning")
-                        if r > 0:
+                        if r > 0 or a.elapsed_time(e) > 3.0 * best_ms:
+                            # (a warm-up run 3x slower than the best is final)
                             ms.append(a.elapsed_time(e))
                             iters.append(int(self.last_stats()[:, 0].sum()))
+                        if r == 0 and ms:
+                            break
                     table.append(dict(v, local_sweeps=sw, ms_per_step=float(np.median(ms)),
                                       cg_iters=int(np.median(iters)),
                                       us_per_iter=float(np.median(ms)) * 1e3 / max(int(np.median(iters)), 1)))
