@@ -362,6 +362,21 @@ typedef struct mfem_refine_args {
   float w_tet, w_vertex;       /* refine_tet_score_weight, refine_vertex_score_weight */
   float penetrating_score;     /* score of an edge whose midpoint is inside a shape */
   float split_t;               /* refine_split_position */
+  /* scoring 1 = "geometric" (refinement.py:123-317): per-edge maximum over
+     incident tets and surface triangles of
+     (L / h_min)(L / L_max)(w_e s_elastic + w_c s_contact), edges shorter
+     than 2 h_min never split; threshold is then refine_geometric_threshold.
+     s_elastic = max(0, rho / mean(rho) - 1) with rho = max(psi, 0) / (vol mu)
+     (tet_score = psi), s_contact = max(0, (d1 - d) / d1) over endpoints whose
+     nearest shape is a capsule (vertex_dist / vertex_sid), the triangle term
+     w_t (d1 - d_tri) / d1 (1 - bary_k) on the edge opposite corner k
+     (tri_dist / tri_bary, null = no triangle term).  0 = "legacy". */
+  int32_t scoring;
+  float min_edge_length, contact_d1, w_elastic, w_vertex_contact, w_tri_contact;
+  const float* vertex_dist;    /* [n_verts] signed distance to the nearest shape */
+  const int32_t* vertex_sid;   /* [n_verts] that shape, -1 none */
+  const float* tri_dist;       /* [n_tris] triangle - capsule distance (or null) */
+  const float* tri_bary;       /* [n_tris][3] barycentric weights of the closest point */
 } mfem_refine_args;
 
 typedef struct mfem_refine_out {
@@ -399,6 +414,13 @@ int mfem_pcg_variant_count(void);
    for ls_blocks of a batch). */
 int mfem_line_search_max_blocks(void);
 int mfem_pcg_variant_info(const mfem_solver* S, int k, int32_t* info);
+
+/* Contact.compute_distance (src/mfem/refinement/contact.py:98-164, 625-681):
+   per vertex the signed distance to the nearest shape and its id (vdist,
+   vsid: [n_verts]); with triangle contact on, the triangle - capsule
+   distance, closest-point barycentric weights, normal and capsule id into
+   S->tri_dist / tri_bary / tri_nrm / tri_sid.  At the bound state x. */
+int mfem_contact_distance(const mfem_solver* S, float* vdist, int32_t* vsid, void* stream);
 
 /* y = H x with the assembled SELL matrix (warp.sparse.bsr_mv). */
 int mfem_spmv(const mfem_solver* S, const float* x, float* y, void* stream);

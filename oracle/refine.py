@@ -1,8 +1,8 @@
 """Deterministic sequential specification of one refinement pass (test
 infrastructure only).
 
-Restates src/mfem/refinement/refinement.py:914-1322 (refine, "legacy"
-scoring) as a sequential algorithm; the reference's own pass is
+Restates src/mfem/refinement/refinement.py:914-1322 (refine, "legacy" and
+"geometric" scoring) as a sequential algorithm; the reference's own pass is
 race-dependent (non-atomic candidate flags, first-inserter ownership of new
 vertices, float atomic_add scores), so its output is defined here and the
 GPU pass must reproduce this one bit for bit:
@@ -100,6 +100,142 @@ def legacy_scores(tets, x, inv_mass, tet_scores, vertex_scores, w_tet, w_vert, p
     pen_int = contribution_int(F32(pen_score), threshold)
     for k in pen:
         scores[k] = max(scores[k], pen_int)
+    return scores
+
+
+# --- "geometric" scoring (populate_candidates_geometric,
+# populate_tri_candidates_geometric, elastic_density_stats;
+# refinement.py:123-317).  The reference combines contributions with
+# atomic_max (order-free) but sums the densities with float atomic_add
+# (order-dependent); here every quantity is FP64 with each operation rounded
+# to nearest, the density sum in the GPU's fixed tree order, and the
+# per-edge maximum is taken on the integer priorities round(c 2^40 / threshold).
+
+CAPSULE = 1
+
+
+def tet_density(tet_score, mu, dminv9):
+    """max(psi, 0) / (vol mu), vol = |1 / (6 det DmInv)| (0 for degenerate data)."""
+    m = [float(F32(v)) for v in np.asarray(dminv9).reshape(9)]
+    c0 = m[4] * m[8] - m[5] * m[7]
+    c1 = m[3] * m[8] - m[5] * m[6]
+    c2 = m[3] * m[7] - m[4] * m[6]
+    det = (m[0] * c0 - m[1] * c1) + m[2] * c2
+    vol = abs(1.0 / (6.0 * det)) if det != 0.0 else float("inf")
+    mu = float(F32(mu))
+    if mu > 0.0 and vol > 0.0 and np.isfinite(vol):
+        e = float(F32(tet_score)) if F32(tet_score) > 0 else 0.0
+        return e / (vol * mu)
+    return 0.0
+
+
+def density_total(rho, block=256):
+    """Sum of the densities in the GPU's order: a pairwise tree inside each
+    block of 256 tets, then per-thread strided sums of the block sums and a
+    pairwise tree over the 256 threads."""
+    rho = [float(v) for v in rho]
+    n_blk = (len(rho) + block - 1) // block
+    part = []
+    for b in range(n_blk):
+        sh = rho[b * block:(b + 1) * block] + [0.0] * max(0, (b + 1) * block - len(rho))
+        w = block // 2
+        while w > 0:
+            for i in range(w):
+                sh[i] = sh[i] + sh[i + w]
+            w //= 2
+        part.append(sh[0])
+    sh = []
+    for t in range(block):
+        acc = 0.0
+        for i in range(t, n_blk, block):
+            acc = acc + part[i]
+        sh.append(acc)
+    w = block // 2
+    while w > 0:
+        for i in range(w):
+            sh[i] = sh[i] + sh[i + w]
+        w //= 2
+    return sh[0]
+
+
+def _len64(x, a, b):
+    xa = [float(F32(v)) for v in x[a]]
+    xb = [float(F32(v)) for v in x[b]]
+    d0, d1, d2 = xb[0] - xa[0], xb[1] - xa[1], xb[2] - xa[2]
+    return float(np.sqrt((d0 * d0 + d1 * d1) + d2 * d2))
+
+
+def _prox(d, d1):
+    p = (d1 - float(F32(d))) / d1
+    return p if p > 0.0 else 0.0
+
+
+def _priority(c, threshold):
+    v = c * (SCORE_ONE / float(F32(threshold)))
+    f = int(np.rint(v)) if v > 0.0 else 0
+    return min(f, SCORE_CAP)
+
+
+def geometric_scores(tets, x, inv_mass, tet_scores, mu, dminv, vertex_dist, vertex_sid, shape_type, h_min, d1,
+                     w_e, w_c, threshold, tris=None, tri_dist=None, tri_bary=None, w_t=1.0,
+                     elastic_total=None):
+    """{edge key: integer priority} of every registered edge (every edge whose
+    endpoints are not both pinned; 0 when it may not split).  elastic_total
+    = the density sum (None: computed over these tets in the GPU's order;
+    0 means absolute densities)."""
+    h = float(F32(h_min))
+    d1 = float(F32(d1))
+    w_e, w_c, w_t = float(F32(w_e)), float(F32(w_c)), float(F32(w_t))
+    rho = [tet_density(tet_scores[t], mu[t], dminv[t]) for t in range(len(tets))]
+    total = density_total(rho) if elastic_total is None else float(elastic_total)
+    scores: dict[int, int] = {}
+    for t, tet in enumerate(tets):
+        s_el = rho[t]
+        if total > 0.0 and len(tets) > 0:
+            mean = total / float(len(tets))
+            ex = s_el / mean - 1.0
+            s_el = ex if ex > 0.0 else 0.0
+        l_max = 0.0
+        for a, b, _, _ in tet_edges(tet):
+            l_max = max(l_max, _len64(x, a, b))
+        l_max = max(l_max, 1.0e-20)
+        for a, b, _, _ in tet_edges(tet):
+            if inv_mass[a] == 0 and inv_mass[b] == 0:
+                continue
+            k = edge_key(a, b)
+            scores.setdefault(k, 0)
+            L = _len64(x, a, b)
+            if L < 2.0 * h:
+                continue
+            s_c = 0.0
+            for v in (a, b):
+                sid = int(vertex_sid[v])
+                if 0 <= sid < len(shape_type) and int(shape_type[sid]) == CAPSULE:
+                    s_c = max(s_c, _prox(vertex_dist[v], d1))
+            c = ((L / h) * (L / l_max)) * (w_e * s_el + w_c * s_c)
+            scores[k] = max(scores[k], _priority(c, threshold))
+    if tris is not None and tri_dist is not None and tri_bary is not None:
+        for f, tv in enumerate(tris):
+            d = float(F32(tri_dist[f]))
+            if not d < d1:
+                continue
+            prox = _prox(d, d1)
+            l_max = 1.0e-20
+            for k in range(3):
+                l_max = max(l_max, _len64(x, int(tv[k]), int(tv[(k + 1) % 3])))
+            for k in range(3):
+                a, b = int(tv[(k + 1) % 3]), int(tv[(k + 2) % 3])
+                if inv_mass[a] == 0 and inv_mass[b] == 0:
+                    continue
+                L = _len64(x, a, b)
+                if L < 2.0 * h:
+                    continue
+                key = edge_key(a, b)
+                if key not in scores:
+                    continue
+                w = 1.0 - float(F32(tri_bary[f][k]))
+                c = (((L / h) * (L / l_max)) * w_t) * prox * w
+                scores[key] = max(scores[key], _priority(c, threshold))
     return scores
 
 

@@ -33,7 +33,7 @@ def _case(cells, h, seed, pinned_face=True):
 
 
 def _gpu_pass(gpu, p, rest, t, tris, inv_mass, vs, ts, threshold, w_tet, w_vert, max_tets, max_verts, max_tris,
-              plane_z=None, split_t=0.5):
+              plane_z=None, split_t=0.5, geo=None, shape_types=None):
     from mfem_amd.refine import Refiner
 
     V, T, F = len(p), len(t), len(tris)
@@ -57,6 +57,18 @@ def _gpu_pass(gpu, p, rest, t, tris, inv_mass, vs, ts, threshold, w_tet, w_vert,
         shapes = (torch.tensor([0], dtype=torch.int32, device=dev),
                   torch.tensor([[0, 0, plane_z, 0, 0, 0, 1]], dtype=torch.float32, device=dev),
                   torch.tensor([[0, 0, 0]], dtype=torch.float32, device=dev))
+    if shape_types is not None:  # far away: only their types matter (geometric scoring)
+        n = len(shape_types)
+        shapes = (torch.tensor(shape_types, dtype=torch.int32, device=dev),
+                  torch.tensor([[0, 0, -100.0, 0, 0, 0, 1]] * n, dtype=torch.float32, device=dev),
+                  torch.tensor([[0.01, 0.05, 0]] * n, dtype=torch.float32, device=dev))
+    extra = {}
+    if geo is not None:
+        extra = dict(geo)
+        for k in ("vertex_dist", "tri_dist", "tri_bary"):
+            if extra.get(k) is not None:
+                extra[k] = torch.as_tensor(extra[k], dtype=torch.float32, device=dev).contiguous()
+        extra["vertex_sid"] = torch.as_tensor(extra["vertex_sid"], dtype=torch.int32, device=dev)
     out = {"tets": torch.zeros(max_tets, 4, dtype=torch.int32, device=dev),
            "stretch": torch.zeros(max_tets, 6, device=dev), "lam": torch.zeros(max_tets, 6, device=dev),
            "materials": torch.zeros(max_tets, 3, device=dev), "dminv": torch.zeros(max_tets, 9, device=dev),
@@ -66,7 +78,7 @@ def _gpu_pass(gpu, p, rest, t, tris, inv_mass, vs, ts, threshold, w_tet, w_vert,
                   inv_mass=cap(inv_mass, V), tet_score=cap(ts, T), vertex_score=cap(vs, V),
                   stretch=cap(stretch, T), lam=cap(lam, T), materials=cap(mat, T), dminv=cap(dminv, T),
                   n_verts=V, n_tets=T, n_tris=F, shapes=shapes, threshold=threshold, w_tet=w_tet, w_vertex=w_vert,
-                  split_t=split_t, out=out)
+                  split_t=split_t, out=out, **extra)
     host = {k: val.cpu().numpy() for k, val in out.items()}
     return res, host, x.cpu().numpy(), v.cpu().numpy(), rs.cpu().numpy(), stretch, lam, mat, dminv
 
@@ -112,6 +124,44 @@ def test_refine_pass_matches_specification(gpu, cells, h, seed, threshold, w_tet
     err = np.abs(G["dminv"][:nt] - ref_dminv).max() / np.abs(ref_dminv).max()
     print("dminv rel err", err)
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("seed,with_tri", [(0, False), (1, True), (2, True)])
+def test_geometric_refine_pass_matches_specification(gpu, seed, with_tri):
+    """Geometric scoring (edge length, longest-edge ratio, excess energy
+    density, capsule proximity of the endpoints, triangle proximity): the
+    same topology and new vertices as the specification, bit for bit."""
+    p, rest, t, tris, inv_mass, vs, ts = _case((6, 3, 3), 0.1, seed)
+    rng = np.random.default_rng(100 + seed)
+    V, F = len(p), len(tris)
+    shape_types = [1, 0]  # capsule, plane
+    geo = dict(scoring="geometric", min_edge_length=0.04, contact_d1=0.2, w_elastic=1.0, w_vertex_contact=0.8,
+               w_tri_contact=1.5,
+               vertex_dist=rng.uniform(-0.05, 0.3, V).astype(np.float32),
+               vertex_sid=rng.integers(-1, 2, V).astype(np.int32))
+    if with_tri:
+        bary = rng.uniform(0, 1, (F, 3)).astype(np.float32)
+        geo.update(tri_dist=rng.uniform(-0.05, 0.4, F).astype(np.float32),
+                   tri_bary=(bary / bary.sum(axis=1, keepdims=True)).astype(np.float32))
+    threshold = 3.0
+    max_tets, max_verts, max_tris = 4 * len(t), 4 * len(p), 4 * len(tris)
+    res, G, x, v, rs, stretch, lam, mat, dminv = _gpu_pass(gpu, p, rest, t, tris, inv_mass, vs, ts, threshold, 0.0,
+                                                           0.1, max_tets, max_verts, max_tris, geo=geo,
+                                                           shape_types=shape_types)
+    sc = R.geometric_scores(t, p, inv_mass, ts, mat[:, 0], dminv, geo["vertex_dist"], geo["vertex_sid"], shape_types,
+                            0.04, 0.2, 1.0, 0.8, threshold, tris=tris if with_tri else None,
+                            tri_dist=geo.get("tri_dist"), tri_bary=geo.get("tri_bary"), w_t=1.5)
+    sel = R.select_edges(t, sc)
+    O = R.refine_pass(t, tris, p, v[:len(p)], rest, stretch, lam, mat, sel, len(p), max_tets, max_verts)
+    print("geometric: candidates", sum(1 for s_ in sc.values() if s_ > R.SCORE_ONE), "selected", len(sel),
+          "rounds", res["rounds"])
+    assert len(sel) > 0 and res["new_verts"] == len(sel)
+    nt, nv, nf = res["n_tets"], res["n_verts"], res["n_tris"]
+    assert nt == len(O["tets"]) and nv == O["n_verts"] and nf == len(O["tris"])
+    np.testing.assert_array_equal(G["tets"][:nt], O["tets"])
+    np.testing.assert_array_equal(G["tris"][:nf], O["tris"])
+    np.testing.assert_array_equal(x[:nv], O["x"])
+    np.testing.assert_array_equal(rs[:nv], O["rest"])
 
 
 def test_refine_capacity_guard_drops_pass(gpu):
@@ -244,3 +294,38 @@ def test_step_after_refinement_matches_fp64_oracle(gpu):
     err = np.abs(x - o.x).max() / np.abs(o.x).max()
     print("positions after refinement: rel err", err)
     assert err <= 1e-4
+
+
+def test_solver_geometric_refinement_near_the_tool(gpu):
+    """refine_scoring="geometric" in the solver (distances recomputed for the
+    current tool pose, solver.py:449-466): with the elastic term off, only
+    edges near the capsule split (contact proximity), every new vertex lies
+    within the barrier range plus an edge length of the capsule, the rest
+    volume is conserved and every solve converges."""
+    from mfem_amd import scene
+    from mfem_amd.solver import RefinementSolver, model_state
+
+    m = scene.capsule_poke_scene()
+    kw = {k: v for k, v in m.params.items() if k != "iterations"}
+    solver = RefinementSolver(m, m.params["iterations"], max_tets=4 * m.tet_count, max_particles=4 * m.particle_count,
+                              device=gpu, enable_refinement=True, refine_every_n_steps=2, refine_scoring="geometric",
+                              refine_elastic_weight=0.0, refine_geometric_threshold=1.0, refine_density=1000.0, **kw)
+    state = model_state(m, gpu, capacity=4 * m.particle_count)
+    out = state.clone()
+    vol0 = _rest_volume(solver)
+    for k in range(3):
+        solver.step(state, out, None, None, 0.01)
+        state, out = out, state
+        assert np.all(solver.last_stats()[:, 1] <= 1e-6)
+        assert solver.status() == 0
+        assert abs(_rest_volume(solver) - vol0) <= 1e-6
+    log = solver.refine_log
+    print("geometric passes", [(r["fired"], r["new_verts"]) for r in log])
+    assert log[0]["fired"] and log[0]["new_verts"] > 0
+    nv0, nv = m.particle_count, solver.active_particle_count
+    x = solver.rest_particle_q[:nv].double().cpu().numpy()
+    cap = m.shapes[0]
+    axis_c = np.asarray(cap.pos, np.float64)
+    d = np.hypot(x[nv0:, 1] - axis_c[1], x[nv0:, 2] - axis_c[2]) - cap.scale[0]  # capsule along world x
+    edge = 0.3 / 6
+    assert np.all(d <= float(kw["contact_d1"]) + edge), d.max()

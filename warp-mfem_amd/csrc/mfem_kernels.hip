@@ -2124,6 +2124,30 @@ int mfem_spmv(const mfem_solver* S, const float* x, float* y, void* stream) {
   return launched();
 }
 
+__global__ void __launch_bounds__(kBlock) k_vertex_distance(mfem_solver S, float* vdist, int32_t* vsid) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.n_verts) return;
+  const float x[3] = {S.x[3 * i], S.x[3 * i + 1], S.x[3 * i + 2]};
+  float grad[3];
+  int sid = -1;
+  const float d = S.n_shapes > 0 ? shape_distance(S, x, grad, &sid) : 1.0e8f;
+  vdist[i] = d;
+  vsid[i] = sid;
+}
+
+int mfem_contact_distance(const mfem_solver* S, float* vdist, int32_t* vsid, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  if (!vdist || !vsid) return fail(MFEM_EINVAL, "null output");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_vertex_distance, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, st, *S, vdist, vsid);
+  if (int rc = launched()) return rc;
+  if (S->tri_contact && S->n_tris > 0 && S->n_shapes > 0) {
+    hipLaunchKernelGGL(k_tri_contact, dim3(blocks_for(S->n_tris)), dim3(kBlock), 0, st, *S);
+    if (int rc = launched()) return rc;
+  }
+  return MFEM_OK;
+}
+
 int mfem_merit(const mfem_solver* S, float dt, float alpha, double* out, void* stream) {
   if (int rc = check_solver(S)) return rc;
   if (!out) return fail(MFEM_EINVAL, "null output");

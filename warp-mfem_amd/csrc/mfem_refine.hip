@@ -182,6 +182,151 @@ __global__ void __launch_bounds__(kThreads) k_score(Mesh M, Table T, float w_tet
     }
 }
 
+// --- "geometric" scoring (populate_candidates_geometric /
+// populate_tri_candidates_geometric / elastic_density_stats,
+// refinement.py:123-317).  Scores are maxima, so the order of the atomics
+// does not matter; the arithmetic is FP64 with explicit round-to-nearest
+// operations (no contraction), converted once to the fixed-point priority.
+
+// strain-energy density over mu of tet t (0 for degenerate data)
+__device__ double tet_density(const float* tet_score, const float* materials, const float* dminv, int t) {
+  const float* m = dminv + 9 * t;
+  // det of DmInv (row-major), cofactor expansion along the first row
+  const double c0 = __dsub_rn(__dmul_rn(m[4], m[8]), __dmul_rn(m[5], m[7]));
+  const double c1 = __dsub_rn(__dmul_rn(m[3], m[8]), __dmul_rn(m[5], m[6]));
+  const double c2 = __dsub_rn(__dmul_rn(m[3], m[7]), __dmul_rn(m[4], m[6]));
+  const double det = __dadd_rn(__dsub_rn(__dmul_rn(m[0], c0), __dmul_rn(m[1], c1)), __dmul_rn(m[2], c2));
+  const double vol = fabs(__ddiv_rn(1.0, __dmul_rn(6.0, det)));
+  const double mu = materials[3 * t];
+  if (mu > 0.0 && vol > 0.0 && isfinite(vol)) {
+    const double e = tet_score[t] > 0.f ? (double)tet_score[t] : 0.0;
+    return __ddiv_rn(e, __dmul_rn(vol, mu));
+  }
+  return 0.0;
+}
+
+__device__ __forceinline__ double edge_len(const float* x, int a, int b) {
+  const double d0 = __dsub_rn((double)x[3 * b], (double)x[3 * a]);
+  const double d1 = __dsub_rn((double)x[3 * b + 1], (double)x[3 * a + 1]);
+  const double d2 = __dsub_rn((double)x[3 * b + 2], (double)x[3 * a + 2]);
+  return __dsqrt_rn(__dadd_rn(__dadd_rn(__dmul_rn(d0, d0), __dmul_rn(d1, d1)), __dmul_rn(d2, d2)));
+}
+
+__device__ __forceinline__ double proximity(float d, double d1) {
+  const double p = __ddiv_rn(__dsub_rn(d1, (double)d), d1);
+  return p > 0.0 ? p : 0.0;
+}
+
+// per-block FP64 sums of the densities, in a fixed tree order
+__global__ void __launch_bounds__(kThreads) k_density_partial(const float* tet_score, const float* materials,
+                                                              const float* dminv, int n_tets, double* part) {
+  __shared__ double sh[kThreads];
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  sh[threadIdx.x] = t < n_tets ? tet_density(tet_score, materials, dminv, t) : 0.0;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = __dadd_rn(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// total = ordered sum of the block sums (one workgroup)
+__global__ void __launch_bounds__(kThreads) k_density_total(const double* part, int n_part, double* total) {
+  __shared__ double sh[kThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n_part; i += kThreads) acc = __dadd_rn(acc, part[i]);
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = __dadd_rn(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = sh[0];
+}
+
+struct Geo {
+  const float* materials;
+  const float* dminv;
+  const float* vdist;
+  const int32_t* vsid;
+  const double* dens_total;  // sum of the densities over the active tets
+  double h_min, d1, w_e, w_c, w_t, q;
+};
+
+__device__ __forceinline__ void score_max(Table T, int s, double c, double q) {
+  double v = __dmul_rn(c, q);
+  long long f = v > 0.0 ? __double2ll_rn(v) : 0;
+  if (f > kScoreCap) f = kScoreCap;
+  atomicMax(&T.score[s], (unsigned long long)f);
+}
+
+__global__ void __launch_bounds__(kThreads) k_score_geometric(Mesh M, Table T, Geo G, int32_t* status) {
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t >= M.n_tets) return;
+  const int32_t* tv = M.tets + 4 * t;
+  double s_el = tet_density(M.tet_score, G.materials, G.dminv, t);
+  const double total = *G.dens_total;
+  if (total > 0.0 && M.n_tets > 0) {
+    const double mean = __ddiv_rn(total, (double)M.n_tets);
+    const double ex = __dsub_rn(__ddiv_rn(s_el, mean), 1.0);
+    s_el = ex > 0.0 ? ex : 0.0;
+  }
+  double l_max = 0.0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < i; ++j) l_max = fmax(l_max, edge_len(M.x, tv[i], tv[j]));
+  l_max = fmax(l_max, 1.0e-20);
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < i; ++j) {
+      const int a = tv[i], b = tv[j];
+      if (M.inv_mass[a] == 0.f && M.inv_mass[b] == 0.f) continue;
+      const int s = table_insert(T.keys, T.mask, ekey(a, b));
+      if (s < 0) {
+        atomicOr(status, MFEM_REFINE_TABLE_FULL);
+        return;
+      }
+      atomicAdd(&T.cnt[s], 1);
+      atomicMin(&T.owner[s], t);
+      const double L = edge_len(M.x, a, b);
+      if (L < __dmul_rn(2.0, G.h_min)) continue;
+      double s_c = 0.0;
+      const int ends[2] = {a, b};
+      for (int k = 0; k < 2; ++k) {
+        const int sid = G.vsid ? G.vsid[ends[k]] : -1;
+        if (sid >= 0 && sid < M.n_shapes && M.shape_type[sid] == MFEM_SHAPE_CAPSULE)
+          s_c = fmax(s_c, proximity(G.vdist[ends[k]], G.d1));
+      }
+      const double c = __dmul_rn(__dmul_rn(__ddiv_rn(L, G.h_min), __ddiv_rn(L, l_max)),
+                                 __dadd_rn(__dmul_rn(G.w_e, s_el), __dmul_rn(G.w_c, s_c)));
+      score_max(T, s, c, G.q);
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_score_geometric_tri(Mesh M, Table T, Geo G, const int32_t* tris,
+                                                                  int n_tris, const float* tri_dist,
+                                                                  const float* tri_bary) {
+  const int f = blockIdx.x * kThreads + threadIdx.x;
+  if (f >= n_tris) return;
+  const float d = tri_dist[f];
+  if (!((double)d < G.d1)) return;
+  const double prox = proximity(d, G.d1);
+  const int32_t* tv = tris + 3 * f;
+  double l_max = 1.0e-20;
+  for (int k = 0; k < 3; ++k) l_max = fmax(l_max, edge_len(M.x, tv[k], tv[(k + 1) % 3]));
+  for (int k = 0; k < 3; ++k) {
+    // edge opposite corner k
+    const int a = tv[(k + 1) % 3], b = tv[(k + 2) % 3];
+    if (M.inv_mass[a] == 0.f && M.inv_mass[b] == 0.f) continue;
+    const double L = edge_len(M.x, a, b);
+    if (L < __dmul_rn(2.0, G.h_min)) continue;
+    const int s = table_find(T.keys, T.mask, ekey(a, b));
+    if (s < 0) continue;
+    const double w = __dsub_rn(1.0, (double)tri_bary[3 * f + k]);
+    const double c = __dmul_rn(__dmul_rn(__dmul_rn(__dmul_rn(__ddiv_rn(L, G.h_min), __ddiv_rn(L, l_max)), G.w_t), prox), w);
+    score_max(T, s, c, G.q);
+  }
+}
+
 __global__ void k_finalize_scores(Table T, long long pen_int) {
   const uint32_t s = blockIdx.x * kThreads + threadIdx.x;
   if (s > T.mask) return;
@@ -379,7 +524,7 @@ namespace {
 struct RLayout {
   uint32_t tsize;
   size_t off_keys, off_score, off_pen, off_cnt, off_votes, off_state, off_owner, off_newvid, off_split, off_nchild,
-      off_nnew, off_cscan, off_newscan, off_tsplit, off_tchild, off_tscan, off_totals, off_scan, bytes;
+      off_nnew, off_cscan, off_newscan, off_tsplit, off_tchild, off_tscan, off_totals, off_dens, off_scan, bytes;
 };
 RLayout r_layout(long max_tets, long max_tris) {
   RLayout L{};
@@ -405,6 +550,7 @@ RLayout r_layout(long max_tets, long max_tris) {
   L.off_tchild = o; o += al((size_t)(max_tris + 1) * 4);
   L.off_tscan = o; o += al((size_t)(max_tris + 1) * 4);
   L.off_totals = o; o += al(64);
+  L.off_dens = o; o += al(((size_t)max_tets / kThreads + 2) * 8);
   L.off_scan = o;
   const long m = max_tets > max_tris ? max_tets : max_tris;
   o += mfem_scan_workspace_bytes((int)m);
@@ -425,6 +571,13 @@ int mfem_refine_select(const mfem_refine_args* A, void* workspace, size_t worksp
   if (A->n_tets <= 0 || A->n_tets > A->max_tets || A->n_tris < 0 || A->n_tris > A->max_tris)
     return fail(MFEM_EINVAL, "bad tet / triangle counts");
   if (!(A->threshold > 0.f)) return fail(MFEM_EINVAL, "threshold must be > 0");
+  if (A->scoring != 0 && A->scoring != 1) return fail(MFEM_EINVAL, "scoring must be 0 (legacy) or 1 (geometric)");
+  if (A->scoring == 1 && (!(A->min_edge_length > 0.f) || !(A->contact_d1 > 0.f)))
+    return fail(MFEM_EINVAL, "geometric scoring needs min_edge_length > 0 and contact_d1 > 0");
+  if (A->scoring == 1 && (!A->materials || !A->dminv || !A->tet_score))
+    return fail(MFEM_EINVAL, "geometric scoring needs tet_score, materials and dminv");
+  if (A->scoring == 1 && A->vertex_sid && !A->vertex_dist)
+    return fail(MFEM_EINVAL, "vertex_sid needs vertex_dist");
   const RLayout L = r_layout(A->max_tets, A->max_tris);
   if (workspace_bytes < L.bytes) return fail(MFEM_ECAPACITY, "refinement workspace too small");
   char* ws = static_cast<char*>(workspace);
@@ -447,7 +600,23 @@ int mfem_refine_select(const mfem_refine_args* A, void* workspace, size_t worksp
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(kThreads), 0, st, status, 1L, 0);
     if (int rc = launched()) return rc;
     const double q = kScoreOne / (double)A->threshold;
-    hipLaunchKernelGGL(k_score, dim3(blocks(A->n_tets)), dim3(kThreads), 0, st, M, T, A->w_tet, A->w_vertex, q, status);
+    if (A->scoring == 1) {
+      double* part = reinterpret_cast<double*>(ws + L.off_dens);
+      double* total = part + (A->max_tets / kThreads + 1);
+      hipLaunchKernelGGL(k_density_partial, dim3(blocks(A->n_tets)), dim3(kThreads), 0, st, A->tet_score,
+                         A->materials, A->dminv, A->n_tets, part);
+      hipLaunchKernelGGL(k_density_total, dim3(1), dim3(kThreads), 0, st, (const double*)part, (int)blocks(A->n_tets),
+                         total);
+      Geo G{A->materials, A->dminv, A->vertex_dist, A->vertex_sid, total, (double)A->min_edge_length,
+            (double)A->contact_d1, (double)A->w_elastic, (double)A->w_vertex_contact, (double)A->w_tri_contact, q};
+      hipLaunchKernelGGL(k_score_geometric, dim3(blocks(A->n_tets)), dim3(kThreads), 0, st, M, T, G, status);
+      if (A->tri_dist && A->tri_bary && A->n_tris > 0)
+        hipLaunchKernelGGL(k_score_geometric_tri, dim3(blocks(A->n_tris)), dim3(kThreads), 0, st, M, T, G, A->tris,
+                           A->n_tris, A->tri_dist, A->tri_bary);
+    } else {
+      hipLaunchKernelGGL(k_score, dim3(blocks(A->n_tets)), dim3(kThreads), 0, st, M, T, A->w_tet, A->w_vertex, q,
+                         status);
+    }
     if (int rc = launched()) return rc;
     long long pen_int = __builtin_llrint((double)A->penetrating_score * q);
     if (pen_int < 0) pen_int = 0;

@@ -1,7 +1,7 @@
 """One adaptive-refinement pass on the GPU (csrc/mfem_refine.hip).
 
 Replaces refine() of src/mfem/refinement/refinement.py:914-1322 ("legacy"
-scoring) with a deterministic pass: candidate edges scored per incident tet
+and "geometric" scoring) with a deterministic pass: candidate edges scored per incident tet
 (contact barrier energy, elastic energy, edge length; a penetrating midpoint
 always wins), split edges selected greedily by (score, edge key) with at
 most one split edge per tet, every tet on a split edge split in two, new
@@ -24,7 +24,11 @@ class RefineArgs(ctypes.Structure):
                                   "shape_xf", "shape_scale", "stretch", "lam", "materials", "dminv")] + \
                [(n, ctypes.c_int32) for n in ("n_verts", "n_tets", "n_tris", "n_shapes", "max_tets", "max_tris",
                                               "max_verts")] + \
-               [(n, ctypes.c_float) for n in ("threshold", "w_tet", "w_vertex", "penetrating_score", "split_t")]
+               [(n, ctypes.c_float) for n in ("threshold", "w_tet", "w_vertex", "penetrating_score", "split_t")] + \
+               [("scoring", ctypes.c_int32)] + \
+               [(n, ctypes.c_float) for n in ("min_edge_length", "contact_d1", "w_elastic", "w_vertex_contact",
+                                              "w_tri_contact")] + \
+               [(n, vp) for n in ("vertex_dist", "vertex_sid", "tri_dist", "tri_bary")]
 
 
 class RefineOut(ctypes.Structure):
@@ -60,14 +64,33 @@ class Refiner:
 
     def run(self, *, tets, tris, x, v, rest, inv_mass, tet_score, vertex_score, stretch, lam, materials, dminv,
             n_verts: int, n_tets: int, n_tris: int, shapes=None, threshold: float = 10.1, w_tet: float = 0.0,
-            w_vertex: float = 0.1, penetrating_score: float = 1.0e6, split_t: float = 0.5, out: dict) -> dict:
+            w_vertex: float = 0.1, penetrating_score: float = 1.0e6, split_t: float = 0.5, out: dict,
+            scoring: str = "legacy", min_edge_length: float = 0.0, contact_d1: float = 0.0,
+            w_elastic: float = 1.0, w_vertex_contact: float = 1.0, w_tri_contact: float = 1.0,
+            vertex_dist=None, vertex_sid=None, tri_dist=None, tri_bary=None) -> dict:
         """One pass.  Inputs are device tensors of the current mesh (x, v,
         rest sized max_verts: new vertices are appended in place); ``out``
         holds tets / stretch / lam / materials / dminv / tris buffers of the
         capacities.  ``shapes`` = (shape_type, shape_xf, shape_scale) device
-        tensors or None.  Returns the new counts, the number of split edges
-        ("new_verts"), whether the pass was dropped and the rounds taken."""
+        tensors or None.  ``scoring="geometric"`` scores edges as
+        populate_candidates_geometric (refinement.py:196-317) from the tet
+        energies, the per-vertex distances / nearest shapes and, when given,
+        the triangle distances / barycentric weights, with ``threshold`` the
+        geometric threshold.  Returns the new counts, the number of split
+        edges ("new_verts"), whether the pass was dropped and the rounds
+        taken."""
+        if scoring not in ("legacy", "geometric"):
+            raise ValueError(f"Unknown refinement scoring {scoring!r} (expected 'legacy' or 'geometric')")
+        if scoring == "geometric":
+            if vertex_dist is None or vertex_sid is None:
+                raise ValueError("scoring='geometric' needs particle_distance and particle_shape_id")
+            if not (min_edge_length > 0.0 and contact_d1 > 0.0):
+                raise ValueError("scoring='geometric' needs min_edge_length > 0 and contact_d1 > 0")
         A = RefineArgs()
+        A.scoring = 1 if scoring == "geometric" else 0
+        A.min_edge_length, A.contact_d1 = float(min_edge_length), float(contact_d1)
+        A.w_elastic, A.w_vertex_contact, A.w_tri_contact = float(w_elastic), float(w_vertex_contact), float(w_tri_contact)
+        A.vertex_dist, A.vertex_sid, A.tri_dist, A.tri_bary = _p(vertex_dist), _p(vertex_sid), _p(tri_dist), _p(tri_bary)
         A.tets, A.tris, A.x, A.inv_mass = _p(tets), _p(tris), _p(x), _p(inv_mass)
         A.tet_score, A.vertex_score = _p(tet_score), _p(vertex_score)
         if shapes is not None and shapes[0].numel() > 0:

@@ -81,10 +81,9 @@ class RefinementSolver:
                  max_tris: int | None = None, refine_every_n_steps: int = 10, min_refine_score: float = 10.1,
                  device: str = "cuda", **kwargs):
         self._refine_enabled = bool(kwargs.get("enable_refinement", True))
-        if self._refine_enabled and kwargs.get("refine_scoring", "legacy") != "legacy":
-            if kwargs["refine_scoring"] not in ("legacy", "geometric"):
-                raise ValueError(f"Unknown refine_scoring {kwargs['refine_scoring']!r} (expected 'legacy' or 'geometric')")
-            raise NotImplementedError("refine_scoring='geometric' is not implemented by mfem_amd (use 'legacy')")
+        self._refine_scoring = kwargs.get("refine_scoring", "legacy")
+        if self._refine_scoring not in ("legacy", "geometric"):
+            raise ValueError(f"Unknown refine_scoring {self._refine_scoring!r} (expected 'legacy' or 'geometric')")
         if kwargs.get("shell_mu", 0.0) > 0.0:
             raise NotImplementedError("membrane shell energy (shell_mu > 0) is not implemented by mfem_amd")
         energy = kwargs.get("energy", "arap")
@@ -111,6 +110,9 @@ class RefinementSolver:
         self.iterations = int(iterations)
         self.max_tets, self.max_particles, self.max_tris = Tc, Vc, Fc
         self.refine_every_n_steps = refine_every_n_steps
+        # the geometric score is dimensionless: its own threshold (solver.py:141-142)
+        if self._refine_scoring == "geometric":
+            min_refine_score = float(kwargs.get("refine_geometric_threshold", 1.0))
         self.min_refine_score = min_refine_score
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
@@ -207,6 +209,15 @@ class RefinementSolver:
             w_vertex=float(kwargs.get("refine_vertex_score_weight", 0.1)),
             penetrating_score=float(kwargs.get("penetrating_edge_score", 1.0e6)),
             split_t=float(kwargs.get("refine_split_position", 0.5)))
+        if self._refine_scoring == "geometric":
+            # solver.py:71-87: min edge length defaults to the barrier range
+            min_edge = kwargs.get("refine_min_edge_length", None)
+            self._refine_kw.update(
+                scoring="geometric", contact_d1=float(kwargs.get("contact_d1", 1.0e-1)),
+                min_edge_length=float(min_edge) if min_edge is not None else float(kwargs.get("contact_d1", 1.0e-1)),
+                w_elastic=float(kwargs.get("refine_elastic_weight", 1.0)),
+                w_vertex_contact=float(kwargs.get("refine_contact_weight", 1.0)),
+                w_tri_contact=float(kwargs.get("refine_tri_contact_weight", 1.0)))
         self._pass_counter = 0
         self._refiner = None
         self.refine_log: list[dict] = []
@@ -390,11 +401,22 @@ class RefinementSolver:
                                  "tris": torch.zeros(max(self.max_tris, 1), 3, dtype=torch.int32, device=dev)}
             S = self._S
             shapes = (b["shape_type"], b["shape_xf"], b["shape_scale"]) if S.n_shapes > 0 else None
+            geo = {}
+            if self._refine_scoring == "geometric":
+                # distances for the current shape poses (solver.py:449-466)
+                if "vdist" not in b:
+                    b["vdist"] = torch.zeros(self.max_particles, dtype=torch.float32, device=self.device)
+                    b["vsid"] = torch.zeros(self.max_particles, dtype=torch.int32, device=self.device)
+                _ffi.call("mfem_contact_distance", ctypes.byref(S), ctypes.c_void_p(b["vdist"].data_ptr()),
+                          ctypes.c_void_p(b["vsid"].data_ptr()), self._stream())
+                tri_on = bool(S.tri_contact and S.n_tris > 0)
+                geo = dict(vertex_dist=b["vdist"], vertex_sid=b["vsid"],
+                           tri_dist=b["tri_dist"] if tri_on else None, tri_bary=b["tri_bary"] if tri_on else None)
             r = self._refiner.run(tets=b["tets"], tris=b["tris"], x=state.particle_q, v=state.particle_qd,
                                   rest=b["rest"], inv_mass=b["inv_mass"], tet_score=b["psi"], vertex_score=b["eB"],
                                   stretch=b["s"], lam=b["lam"], materials=b["materials"], dminv=b["dminv"],
                                   n_verts=S.n_verts, n_tets=S.n_tets, n_tris=S.n_tris, shapes=shapes,
-                                  out=self._ref_out, **self._refine_kw)
+                                  out=self._ref_out, **self._refine_kw, **geo)
             res.update(r)
             if r["new_verts"] > 0 and not r["dropped"]:
                 o, nt, nf = self._ref_out, r["n_tets"], r["n_tris"]
