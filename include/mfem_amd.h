@@ -415,6 +415,30 @@ int mfem_pcg_variant_count(void);
 int mfem_line_search_max_blocks(void);
 int mfem_pcg_variant_info(const mfem_solver* S, int k, int32_t* info);
 
+/* System-matrix pattern and assembly gather lists on the GPU, after a
+   topology change (replaces the warp.sparse topology rebuild,
+   src/mfem/refinement/solver.py:685-715): the same arrays, in the same
+   order, as a sequential construction (all lists come from sorted 64-bit
+   keys).  Two calls with one workspace: mfem_pattern_count writes rowptr
+   [n_verts + 1], chunk_ptr [ceil(n_verts / 64) + 1] and counts = {nnz,
+   n_slots}; the host sizes the outputs from counts, then mfem_pattern_fill
+   writes cols [nnz], sell_col [n_slots * 64], diag_lane [n_verts],
+   contrib_ptr [n_slots * 64 + 1], contrib_src [16 n_tets], vinc_ptr
+   [n_verts + 1], vinc_src [4 n_tets], tcon_ptr [n_slots * 64 + 1],
+   tcon_src [9 n_tris], vtri_ptr [n_verts + 1], vtri_src [3 n_tris];
+   status[0] = 1 when a triangle edge is not a tet edge. */
+size_t mfem_pattern_workspace_bytes(int max_verts, int max_tets, int max_tris);
+int mfem_pattern_count(const int32_t* tets, int n_tets, int n_verts, int max_verts, int max_tets, int max_tris,
+                       void* workspace, size_t workspace_bytes, int32_t* rowptr, int32_t* chunk_ptr, int32_t* counts,
+                       void* stream);
+int mfem_pattern_fill(const int32_t* tets, int n_tets, const int32_t* tris, int n_tris, int n_verts, int max_verts,
+                      int max_tets, int max_tris, void* workspace, size_t workspace_bytes, const int32_t* rowptr,
+                      const int32_t* chunk_ptr, int n_slots, int32_t* cols, int32_t* sell_col, int32_t* diag_lane,
+                      int32_t* contrib_ptr, int32_t* contrib_src, int32_t* vinc_ptr, int32_t* vinc_src,
+                      int32_t* tcon_ptr, int32_t* tcon_src, int32_t* vtri_ptr, int32_t* vtri_src, int32_t* status,
+                      void* stream);
+const char* mfem_pattern_last_error(void);
+
 /* Contact.compute_distance (src/mfem/refinement/contact.py:98-164, 625-681):
    per vertex the signed distance to the nearest shape and its id (vdist,
    vsid: [n_verts]); with triangle contact on, the triangle - capsule

@@ -56,7 +56,8 @@ ENTRY_POINTS = [
     "mfem_broadphase_workspace_bytes", "mfem_soft_rigid_distance", "mfem_pair_barrier", "mfem_collide_last_error",
     "mfem_refine_workspace_bytes", "mfem_refine_select", "mfem_refine_count", "mfem_refine_apply",
     "mfem_refine_last_error", "mfem_pcg_variant_count", "mfem_pcg_variant_info", "mfem_line_search_max_blocks",
-    "mfem_contact_distance",
+    "mfem_contact_distance", "mfem_pattern_workspace_bytes", "mfem_pattern_count", "mfem_pattern_fill",
+    "mfem_pattern_last_error",
 ]
 
 
@@ -91,6 +92,12 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mfem_spmv.argtypes = [P, vp, vp, vp]
     lib.mfem_merit.argtypes = [P, ctypes.c_float, ctypes.c_float, vp, vp]
     lib.mfem_contact_distance.argtypes = [P, vp, vp, vp]
+    i = ctypes.c_int
+    lib.mfem_pattern_workspace_bytes.restype = ctypes.c_size_t
+    lib.mfem_pattern_workspace_bytes.argtypes = [i, i, i]
+    lib.mfem_pattern_count.argtypes = [vp, i, i, i, i, i, vp, ctypes.c_size_t, vp, vp, vp, vp]
+    lib.mfem_pattern_fill.argtypes = [vp, i, vp, i, i, i, i, i, vp, ctypes.c_size_t, vp, vp, i] + [vp] * 13
+    lib.mfem_pattern_last_error.restype = ctypes.c_char_p
     lib.mfem_ws_features.argtypes = [P, ctypes.c_float, ctypes.c_int, vp]
     lib.mfem_ws_apply.argtypes = [P, vp]
     lib.mfem_mlp_forward.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]

@@ -168,3 +168,71 @@ def sell_to_csr_values(pat: SystemPattern, sell_val: np.ndarray) -> np.ndarray:
     sv = np.asarray(sell_val).reshape(pat.n_slots, 9, 64)
     lane = pat.csr_lane
     return sv[lane // 64, :, lane % 64].reshape(-1, 3, 3)
+
+
+class GpuPatternBuilder:
+    """build_system_pattern + build_triangle_lists on the GPU
+    (csrc/mfem_pattern.hip), for meshes up to the given capacities: the same
+    arrays, in the same order.  ``build`` returns the device tensors and a
+    host SystemPattern view (rowptr, cols, chunk_ptr, csr_lane, sell_col,
+    diag_lane copied back; the gather lists stay on the device)."""
+
+    def __init__(self, max_verts: int, max_tets: int, max_tris: int, device):
+        import torch
+
+        from . import _ffi
+
+        self.lib = _ffi.load()
+        self.torch = torch
+        self.device = torch.device(device)
+        self.caps = (int(max_verts), int(max_tets), int(max(max_tris, 1)))
+        nb = int(self.lib.mfem_pattern_workspace_bytes(*self.caps))
+        self.ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        self.counts = torch.zeros(2, dtype=torch.int32, device=self.device)
+        self.status = torch.zeros(1, dtype=torch.int32, device=self.device)
+
+    def _call(self, name, *args):
+        from . import _ffi
+
+        if getattr(self.lib, name)(*args) != 0:
+            raise _ffi.MfemError(f"{name}: {self.lib.mfem_pattern_last_error().decode()}")
+
+    def build(self, tets, n_tets: int, tris, n_tris: int, n_verts: int):
+        import ctypes
+
+        torch = self.torch
+        vp = ctypes.c_void_p
+        dev = self.device
+        i32 = lambda n: torch.zeros(max(int(n), 1), dtype=torch.int32, device=dev)
+        st = vp(torch.cuda.current_stream(dev).cuda_stream)
+        n_chunks = (n_verts + 63) // 64
+        out = {"rowptr": i32(n_verts + 1), "chunk_ptr": i32(n_chunks + 1)}
+        mv, mt, mf = self.caps
+        ws, nb = vp(self.ws.data_ptr()), ctypes.c_size_t(self.ws.numel())
+        self._call("mfem_pattern_count", vp(tets.data_ptr()), n_tets, n_verts, mv, mt, mf, ws, nb,
+                   vp(out["rowptr"].data_ptr()), vp(out["chunk_ptr"].data_ptr()), vp(self.counts.data_ptr()), st)
+        nnz, n_slots = (int(v) for v in self.counts.tolist())
+        n_lanes = n_slots * 64
+        out.update(cols=i32(nnz), sell_col=i32(n_lanes), diag_lane=i32(n_verts), contrib_ptr=i32(n_lanes + 1),
+                   contrib_src=i32(16 * n_tets), vinc_ptr=i32(n_verts + 1), vinc_src=i32(4 * n_tets),
+                   tcon_ptr=i32(n_lanes + 1), tcon_src=i32(9 * n_tris), vtri_ptr=i32(n_verts + 1),
+                   vtri_src=i32(3 * n_tris))
+        names = ("cols", "sell_col", "diag_lane", "contrib_ptr", "contrib_src", "vinc_ptr", "vinc_src", "tcon_ptr",
+                 "tcon_src", "vtri_ptr", "vtri_src")
+        self._call("mfem_pattern_fill", vp(tets.data_ptr()), n_tets, vp(tris.data_ptr() if n_tris else 0), n_tris,
+                   n_verts, mv, mt, mf, ws, nb, vp(out["rowptr"].data_ptr()), vp(out["chunk_ptr"].data_ptr()),
+                   n_slots, *[vp(out[k].data_ptr()) for k in names], vp(self.status.data_ptr()), st)
+        if int(self.status.item()) != 0:
+            raise ValueError("a surface triangle edge is not an edge of the tet mesh")
+        rowptr = out["rowptr"].cpu().numpy()
+        cols = out["cols"][:nnz].cpu().numpy()
+        chunk_ptr = out["chunk_ptr"].cpu().numpy()
+        rows = np.repeat(np.arange(n_verts, dtype=np.int64), np.diff(rowptr.astype(np.int64)))
+        k_in_row = np.arange(nnz, dtype=np.int64) - rowptr[rows].astype(np.int64)
+        csr_lane = (chunk_ptr[rows // 64].astype(np.int64) + k_in_row) * 64 + rows % 64
+        empty = np.zeros(0, np.int32)
+        pat = SystemPattern(n_verts=n_verts, n_chunks=n_chunks, n_slots=n_slots, rowptr=rowptr, cols=cols,
+                            chunk_ptr=chunk_ptr, sell_col=out["sell_col"].cpu().numpy(),
+                            diag_lane=out["diag_lane"].cpu().numpy(), csr_lane=csr_lane, contrib_ptr=empty,
+                            contrib_src=empty, vinc_ptr=empty, vinc_src=empty)
+        return pat, out

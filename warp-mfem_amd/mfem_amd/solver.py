@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from . import _ffi
-from .pattern import build_system_pattern, build_triangle_lists
+from .pattern import GpuPatternBuilder
 
 _ENERGIES = {"arap": 0, "neohookean": 1}
 
@@ -256,6 +256,7 @@ class RefinementSolver:
             if hasattr(S, name):
                 setattr(S, name, t.data_ptr())
         self._S = S
+        self._pattern_builder = None
         self._set_topology(nv, nt, len(tris))
         self.warm_start_net = None
         self.set_warm_start(kwargs.get("warm_start"))
@@ -268,33 +269,19 @@ class RefinementSolver:
     # -- topology -------------------------------------------------------------
     def _set_topology(self, n_verts: int, n_tets: int, n_tris: int) -> None:
         """(Re)builds the system pattern and the gather lists of the active
-        mesh (host, numpy: once per topology) and points the struct at them."""
+        mesh (on the GPU, csrc/mfem_pattern.hip: once per topology) and points
+        the struct at them."""
         b, dev = self.buffers, self.device
-        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=dev)
-        tets = b["tets"][:n_tets].cpu().numpy()
-        tris = b["tris"][:n_tris].cpu().numpy()
-        pat = build_system_pattern(n_verts, tets)
+        if self._pattern_builder is None:
+            self._pattern_builder = GpuPatternBuilder(self.max_particles, self.max_tets, self.max_tris, dev)
+        pat, out = self._pattern_builder.build(b["tets"], n_tets, b["tris"], n_tris, n_verts)
         self.pattern = pat
-        b["chunk_ptr"] = i32(pat.chunk_ptr)
-        b["sell_col"] = i32(pat.sell_col)
+        for k in ("chunk_ptr", "sell_col", "diag_lane", "contrib_ptr", "contrib_src", "vinc_ptr", "vinc_src",
+                  "tcon_ptr", "tcon_src", "vtri_ptr", "vtri_src"):
+            b[k] = out[k]
         b["sell_val"] = torch.zeros(pat.n_slots * 576, dtype=torch.float32, device=dev)
-        b["diag_lane"] = i32(pat.diag_lane)
-        b["contrib_ptr"] = i32(pat.contrib_ptr)
-        b["contrib_src"] = i32(pat.contrib_src if len(pat.contrib_src) else np.zeros(1))
-        b["vinc_ptr"] = i32(pat.vinc_ptr)
-        b["vinc_src"] = i32(pat.vinc_src)
         b["pz"] = torch.zeros(2, pat.n_chunks * 64, 3, dtype=torch.float32, device=dev)
         b["px"] = torch.zeros(2, pat.n_chunks * 64, 3, dtype=torch.float64, device=dev)
-        if n_tris:
-            tcon_ptr, tcon_src, vtri_ptr, vtri_src = build_triangle_lists(pat, tris)
-        else:
-            tcon_ptr = np.zeros(pat.n_slots * 64 + 1, np.int32)
-            tcon_src = vtri_src = np.zeros(1, np.int32)
-            vtri_ptr = np.zeros(n_verts + 1, np.int32)
-        b["tcon_ptr"] = i32(tcon_ptr)
-        b["tcon_src"] = i32(tcon_src if len(tcon_src) else np.zeros(1))
-        b["vtri_ptr"] = i32(vtri_ptr)
-        b["vtri_src"] = i32(vtri_src if len(vtri_src) else np.zeros(1))
         S = self._S
         S.n_verts, S.n_tets, S.n_tris = int(n_verts), int(n_tets), int(n_tris)
         S.n_chunks, S.n_slots = pat.n_chunks, pat.n_slots
@@ -371,8 +358,8 @@ class RefinementSolver:
         nv, nt = self.active_particle_count, self.active_tet_count
         vol = b["vol"][:nt].double().cpu().numpy()
         tets = b["tets"][:nt].cpu().numpy()
-        m = np.zeros(nv)
-        np.add.at(m, tets.reshape(-1), np.repeat(self._refine_density * vol / 4.0, 4))
+        # (bincount adds the quarter volumes in element order, as add.at would)
+        m = np.bincount(tets.reshape(-1), weights=np.repeat(self._refine_density * vol / 4.0, 4), minlength=nv)[:nv]
         pinned = self._pinned[:nv].cpu().numpy()
         b["mass"][:nv] = torch.as_tensor(m, dtype=torch.float32, device=self.device)
         b["inv_mass"][:nv] = torch.as_tensor(np.where(pinned, 0.0, 1.0 / np.maximum(m, 1e-30)), dtype=torch.float32,
@@ -425,7 +412,10 @@ class RefinementSolver:
                 b["lam"][:nt] = o["lam"][:nt]
                 b["materials"][:nt] = o["materials"][:nt]
                 b["dminv"][:nt] = o["dminv"][:nt]
-                b["vol"][:nt] = (1.0 / (6.0 * torch.linalg.det(o["dminv"][:nt].double().view(-1, 3, 3)))).float()
+                d = o["dminv"][:nt].double()
+                det = (d[:, 0] * (d[:, 4] * d[:, 8] - d[:, 5] * d[:, 7]) - d[:, 1] * (d[:, 3] * d[:, 8] - d[:, 5] * d[:, 6])
+                       + d[:, 2] * (d[:, 3] * d[:, 7] - d[:, 4] * d[:, 6]))
+                b["vol"][:nt] = (1.0 / (6.0 * det)).float()
                 if nf:
                     b["tris"][:nf] = o["tris"][:nf]
                 nv0 = self.active_particle_count
