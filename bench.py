@@ -253,7 +253,7 @@ def main():
         "preconditioner": {"plain": "block_jacobi",
                            "value": f"block_jacobi + {args.local_sweeps} workgroup-local sweeps (omega "
                                     f"{args.local_omega})" if args.local_sweeps else "block_jacobi"},
-        "pcg": args.pcg,
+        "pcg": tune["chosen"]["mode"] if tune else args.pcg,
         "learned_warm_start": learned,
         "reductions": "f64",
         "plain_value": value_plain,

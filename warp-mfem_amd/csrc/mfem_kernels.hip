@@ -744,6 +744,12 @@ __device__ __forceinline__ void spmv_row_resident(const float (&val)[RW][9], con
       a2 += (T)v[6] * g[k][0] + (T)v[7] * g[k][1] + (T)v[8] * g[k][2];
     }
   }
+  if constexpr (sizeof(T) > sizeof(float)) {
+    // FP64 products: materialise the sums here (sunk into the caller's
+    // later branches they keep every gathered value and converted matrix
+    // entry live at once, which spills)
+    asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2));
+  }
   out[0] = a0;
   out[1] = a1;
   out[2] = a2;
