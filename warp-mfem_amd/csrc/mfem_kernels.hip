@@ -579,10 +579,34 @@ __device__ __forceinline__ T pub_load(const T* p) {
   return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// v + v of the lane that DPP control CTRL selects within each row of 16 lanes
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return v + __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// value of lane l (wave-uniform)
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)b, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Sum over the 64 lanes of a wave (every lane active), wave-uniform and in
+// one fixed order: butterflies within each row of 16 lanes through DPP lane
+// selects (lane ^ 1, lane ^ 2, 8-lane mirror, 16-lane mirror; a few cycles
+// each, against an LDS round trip per step for a shuffle), then the four row
+// sums read out of lanes 0, 16, 32, 48.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v = dpp_add<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = dpp_add<0x141>(v);  // row_half_mirror
+  v = dpp_add<0x140>(v);  // row_mirror
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 
 template <int NV>
@@ -667,7 +691,10 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
     for (int w = 1; w < kWaves; ++w) a += tot[w][i];
     v[i] = a;
   }
-  __syncthreads();
+  // (no closing barrier: the next call writes tot only after its first
+  // barrier, which every thread reaches after these reads; before it, it
+  // writes sm, read here by no one, and clears abort_flag, which is clear
+  // already, since a set flag ends the solve)
   return true;
 }
 
