@@ -823,6 +823,39 @@ __device__ __forceinline__ void spmv_rows64(const mfem_solver& S, const int (&ch
   }
 }
 
+// cache-policy bits of a direct-to-LDS load: SC1, device scope (the same
+// coherence as the agent-scope loads of published vectors)
+constexpr int kCachePolicyDevice = 16;
+// 64-byte lines of a published FP64 vector staged per workgroup (32 KB of LDS)
+constexpr int kHaloLines = 512;
+
+// resident rows against the staged lines: off[k] is slot k's column's FP64
+// offset in hv (3 consecutive values)
+template <int RW>
+__device__ __forceinline__ void spmv_row_lds(const float (&val)[RW][9], const int (&off)[RW], const double* hv,
+                                             double* out) {
+  constexpr int GRP = 4;  // LDS reads in flight per group (registers)
+  double a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+  for (int kb = 0; kb < RW; kb += GRP) {
+#pragma unroll
+    for (int k = kb; k < kb + GRP && k < RW; ++k) {
+      const double* g = hv + off[k];
+      const double g0 = g[0], g1 = g[1], g2 = g[2];
+      const float* v = val[k];
+      a0 += (double)v[0] * g0 + (double)v[1] * g1 + (double)v[2] * g2;
+      a1 += (double)v[3] * g0 + (double)v[4] * g1 + (double)v[5] * g2;
+      a2 += (double)v[6] * g0 + (double)v[7] * g1 + (double)v[8] * g2;
+    }
+    // sums materialised per group, reads not moved across (see
+    // spmv_row_resident)
+    asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2)::"memory");
+  }
+  out[0] = a0;
+  out[1] = a1;
+  out[2] = a2;
+}
+
 template <int CPW, typename T>
 __device__ __forceinline__ void apply_M(const float (&M)[CPW][9], int k, const double* r, T* z) {
 #pragma unroll
@@ -1070,6 +1103,128 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
     // workgroup barrier ends this call's reads)
   };
 
+  // Pipelined, resident rows: the FP64 SpMV input is staged per workgroup.
+  // The workgroup's columns touch a few hundred distinct 64-byte lines of the
+  // published vector (each 24-byte row spans one or two); gathering them lane
+  // by lane costs ~25 line lookups per wave instruction, 48 instructions per
+  // wave.  Instead the lines are copied once per SpMV into LDS with
+  // coalesced direct-to-LDS loads (16 bytes a lane, device scope) and the
+  // gathers read LDS.  Set-up, once per launch: mark the lines in an LDS
+  // bitmap, count them (exclusive prefix over the bitmap words), list them in
+  // ascending order (a row that spans two lines finds them adjacent) and turn
+  // each resident slot's column into its LDS offset.  Slot 0 is a line of
+  // zeros for the padding slots.  A workgroup with more lines than fit
+  // streams its rows from memory instead.
+  constexpr bool HALO = PIPE && !PUB32 && RW > 0;
+  constexpr int kHaloQ = kHaloLines * 4 / kBlock;  // 16-byte quarters per thread
+  __shared__ double hv[HALO ? kHaloLines * 8 : 1];
+  // hl[1 + j]: line of staged slot j + 1, -1 past the last one
+  __shared__ int hl[HALO ? kHaloLines + 1 : 1];
+  __shared__ int hcount;
+  bool staged = false;
+  if constexpr (HALO) {
+    uint32_t* bm = reinterpret_cast<uint32_t*>(hv);
+    const long long n_lines = (long long)nrow_pad * 3 / 8;
+    const int nwords = (int)((n_lines + 31) >> 5);
+    // bitmap and prefix share the line buffer (dead once the offsets are set)
+    if (2LL * nwords <= (long long)kHaloLines * 16) {
+      uint32_t* pre = bm + nwords;
+      const int c0 = own[0] ? S.chunk_ptr[chunk[0]] : 0;
+      const int wd = own[0] ? S.chunk_ptr[chunk[0] + 1] - c0 : 0;
+      for (int i = threadIdx.x; i < nwords; i += kBlock) bm[i] = 0u;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < RW; ++k) {
+        if (k < wd) {
+          const long long b0 = 24LL * rb.col[k];
+          const int l0 = (int)(b0 >> 6), l1 = (int)((b0 + 23) >> 6);
+          atomicOr(&bm[l0 >> 5], 1u << (l0 & 31));
+          if (l1 != l0) atomicOr(&bm[l1 >> 5], 1u << (l1 & 31));
+        }
+      }
+      __syncthreads();
+      // exclusive prefix of the words' bit counts: contiguous word segments
+      // per thread, then a scan of the 256 segment totals
+      const int seg = (nwords + kBlock - 1) / kBlock;
+      const int w0 = min((int)threadIdx.x * seg, nwords), w1 = min(w0 + seg, nwords);
+      int mine = 0;
+      for (int i = w0; i < w1; ++i) mine += __popc(bm[i]);
+      int incl = mine;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+      }
+      if (lane == 63) hl[threadIdx.x >> 6] = incl;  // wave totals (hl reused)
+      __syncthreads();
+      int base = 0;
+      for (int wv = 0; wv < (int)(threadIdx.x >> 6); ++wv) base += hl[wv];
+      if (threadIdx.x == kBlock - 1) hcount = base + incl;
+      __syncthreads();
+      const int H = hcount;
+      staged = H <= kHaloLines - 1;
+      if (staged) {
+        for (int i = 1 + H + (int)threadIdx.x; i <= kHaloLines; i += kBlock) hl[i] = -1;
+        int run = base + incl - mine;
+        for (int i = w0; i < w1; ++i) {
+          pre[i] = (uint32_t)run;
+          uint32_t bits = bm[i];
+          while (bits) {
+            const int j = __ffs(bits) - 1;
+            bits &= bits - 1u;
+            hl[1 + run++] = i * 32 + j;
+          }
+        }
+      }
+      __syncthreads();
+      if (staged) {
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+          int off = 0;
+          if (k < wd) {
+            const int c = rb.col[k];
+            const int l0 = (int)((24LL * c) >> 6);
+            const uint32_t wbits = bm[l0 >> 5] & ((1u << (l0 & 31)) - 1u);
+            off = (1 + (int)pre[l0 >> 5] + __popc(wbits)) * 8 + ((3 * c) & 7);
+          }
+          rb.col[k] = off;
+        }
+      }
+      __syncthreads();
+      if (staged && threadIdx.x < 8) hv[threadIdx.x] = 0.0;
+      // (the first staging comes after a grid reduction, whose workgroup
+      // barrier orders these writes before it)
+    }
+  }
+  // y = H vec on the resident rows, FP64 products (pipelined variant)
+  auto spmv_resident64 = [&](const double* vec, double (&y)[CPW][3]) {
+    if constexpr (HALO) {
+      if (staged) {
+        int hline[kHaloQ];
+#pragma unroll
+        for (int i = 0; i < kHaloQ; ++i) hline[i] = hl[1 + ((threadIdx.x + kBlock * i) >> 2)];
+#pragma unroll
+        for (int i = 0; i < kHaloQ; ++i) {
+          if (hline[i] >= 0) {
+            const int q = threadIdx.x + kBlock * i;
+            // lane l of the wave lands at base + 16 l: quarter q at hv[8 + 2 q]
+            __builtin_amdgcn_global_load_lds(vec + (size_t)hline[i] * 8 + (q & 3) * 2,
+                                             (__attribute__((address_space(3))) void*)&hv[8 + (q & ~63) * 2], 16,
+                                             0, kCachePolicyDevice);
+          }
+        }
+        wait_vmem();
+        __syncthreads();
+        spmv_row_lds<RW>(rb.val, rb.col, hv, y[0]);
+      } else {
+        // (too many lines for the buffer: rows streamed from memory)
+        spmv_rows64<CPW, RW, false>(S, chunk, own, lane, rb, vec, y);
+      }
+    } else {
+      spmv_rows64<CPW, RW, true>(S, chunk, own, lane, rb, vec, y);
+    }
+  };
+
   // (re)start from the current r.  Standard: z = M r published, (r.z, r.r)
   // reduced.  Pipelined: u = M r published (FP64), then w = H u.
   auto restart = [&](double& rz, double& rr) -> bool {
@@ -1115,7 +1270,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       double red2[1] = {rr};
       if (!grid_reduce(gs, red2)) return false;
       rr = red2[0];
-      spmv_rows64<CPW, RW, true>(S, chunk, own, lane, rb, ub, w);
+      spmv_resident64(ub, w);
     } else {
       float* zb = pzbuf(pub);
       precondition();
@@ -1328,7 +1483,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
 #pragma unroll
             for (int a = 0; a < 3; ++a) n[k][a] = (double)y[k][a];
         } else {
-          spmv_rows64<CPW, RW, true>(S, chunk, own, lane, rb, pxbuf(pubx - 1), n);
+          spmv_resident64(pxbuf(pubx - 1), n);
         }
         PCG_LAP(0);
         double alpha;
