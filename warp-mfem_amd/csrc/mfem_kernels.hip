@@ -1122,6 +1122,9 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
   __shared__ int hl[HALO ? kHaloLines + 1 : 1];
   __shared__ int hcount;
   bool staged = false;
+  // per-wave transpose slice for the coalesced publication of m
+  __shared__ double tr[(PIPE && !PUB32) ? kWaves * 192 : 1];
+  double* trw = tr + ((PIPE && !PUB32) ? (threadIdx.x >> 6) * 192 : 0);
   if constexpr (HALO) {
     uint32_t* bm = reinterpret_cast<uint32_t*>(hv);
     const long long n_lines = (long long)nrow_pad * 3 / 8;
@@ -1224,6 +1227,13 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       spmv_rows64<CPW, RW, true>(S, chunk, own, lane, rb, vec, y);
     }
   };
+  // y = H vec in FP64 for the starts and the true-residual checks: staged
+  // when the pipelined variant stages, else rows streamed from memory (the
+  // resident FP64 form would hold too many registers in the standard one)
+  auto spmv_check64 = [&](const double* vec, double (&y)[CPW][3]) {
+    if constexpr (HALO) spmv_resident64(vec, y);
+    else spmv_rows64<CPW, RW, false>(S, chunk, own, lane, rb, vec, y);
+  };
 
   // (re)start from the current r.  Standard: z = M r published, (r.z, r.r)
   // reduced.  Pipelined: u = M r published (FP64), then w = H u.
@@ -1300,7 +1310,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
     double rz = 0.0, rr = 0.0;
     {
       double y[CPW][3];
-      spmv_rows64<CPW, RW, false>(S, chunk, own, lane, rb, px0, y);
+      spmv_check64(px0, y);
       reload_b();
 #pragma unroll
       for (int k = 0; k < CPW; ++k)
@@ -1347,7 +1357,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       for (int c = 0; c < 2; ++c) {
         double* cb = pxbuf(pubx - 1);
         double y[CPW][3];
-        spmv_rows64<CPW, RW, false>(S, chunk, own, lane, rb, cb, y);
+        spmv_check64(cb, y);
         reload_b();
         double ph = 0.0, rq = 0.0;
 #pragma unroll
@@ -1404,14 +1414,27 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         float* mb32 = pzbuf(pub);
 #pragma unroll
         for (int k = 0; k < CPW; ++k) {
-          if (!own[k]) continue;
+          if (!own[k]) continue;  // (uniform over the wave)
           const int row = chunk[k] * 64 + lane;
           double m[3];
           apply_M<CPW, double>(M, k, w[k], m);
+          if constexpr (PUB32) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) pub_store(&mb32[3 * row + a], (float)m[a]);
+          } else {
+            // the chunk's 192 values go out as three contiguous 512-byte
+            // stores (transposed through this wave's LDS slice; LDS
+            // operations of a wave complete in order) instead of three
+            // 24-byte-strided ones that each touch 24 lines partially
+#pragma unroll
+            for (int a = 0; a < 3; ++a) trw[3 * lane + a] = m[a];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int j = 0; j < 3; ++j) pub_store(&mb[(size_t)chunk[k] * 192 + lane + 64 * j], trw[lane + 64 * j]);
+            __builtin_amdgcn_wave_barrier();
+          }
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
-            if constexpr (PUB32) pub_store(&mb32[3 * row + a], (float)m[a]);
-            else pub_store(&mb[3 * row + a], m[a]);
             gamma += r[k][a] * u[k][a];
             delta += w[k][a] * u[k][a];
             rr += r[k][a] * r[k][a];
@@ -1444,7 +1467,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         double rt[CPW][3];
         {
           double y[CPW][3];
-          spmv_rows64<CPW, RW, false>(S, chunk, own, lane, rb, xb, y);
+          spmv_check64(xb, y);
           reload_b();
 #pragma unroll
           for (int k = 0; k < CPW; ++k)
