@@ -34,6 +34,12 @@ static constexpr int kWaves = kBlock / 64;
 // this long means a workgroup never arrived; the kernel then records the error
 // and exits instead of spinning forever.
 static constexpr uint64_t kBarrierTimeout = 500000000ull;
+// A grid-barrier wait looks at the error bits (and the limit above) only
+// after this many ticks (20 us), so that a normal wait costs one memory
+// round trip per poll.
+static constexpr uint64_t kStatusCheckAfter = 2000ull;
+// partials read per lane per batch of poll loads (6 x 32 = 192 workgroups)
+static constexpr int kPollPasses = 6;
 
 static thread_local std::string g_last_error;
 
@@ -610,7 +616,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 template <int NV>
-__device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
+__device__ __forceinline__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
   static_assert(NV >= 1 && NV <= 4, "at most 4 values per reduction");
   __shared__ double sm[kWaves][4];
   __shared__ double tot[kWaves][4];
@@ -645,38 +651,72 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
     pub_store(&words[((size_t)slot * MFEM_RED_MAX_BLOCKS + blockIdx.x) * 8 + threadIdx.x],
               ((uint64_t)ep << 32) | half);
   }
-  double acc[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+  // Reading: eight lanes per partial, lane j loading word j, so that one
+  // load instruction fetches eight partials' lines whole (one request per
+  // line instead of one per word: every workgroup polls every partial, and
+  // the request count, not the latency alone, sets the cost of a poll).
+  // Each lane polls its own word until its tag is the current epoch; the
+  // even lane of a pair then takes its neighbour's half (DPP) and owns
+  // value j / 2.  Up to kPollPasses partials per lane per batch of loads.
+  const int j = threadIdx.x & 7;
+  const int grp = threadIdx.x >> 3;  // 32 partials per pass of the block
+  double a = 0.0;                    // value j / 2 of this lane's partials, in workgroup order
   bool to = false;
-  for (int b = gs.wg0 + threadIdx.x; b < gs.wg1 && !to; b += kBlock) {
-    const uint64_t* src = words + ((size_t)slot * MFEM_RED_MAX_BLOCKS + b) * 8;
+  for (int base = gs.wg0; base < gs.wg1; base += 32 * kPollPasses) {
+    uint64_t wv[kPollPasses];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      uint64_t lo = pub_load(&src[2 * i]), hi = pub_load(&src[2 * i + 1]);
-      if ((uint32_t)(lo >> 32) != ep || (uint32_t)(hi >> 32) != ep) {
-        const uint64_t t0 = wall_clock64();
-        while ((uint32_t)(lo >> 32) != ep || (uint32_t)(hi >> 32) != ep) {
-          if (wall_clock64() - t0 > kBarrierTimeout || pub_load(&gs.status[MFEM_STATUS_ERRBITS]) != 0) {
-            to = true;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          lo = pub_load(&src[2 * i]);
-          hi = pub_load(&src[2 * i + 1]);
+    for (int p = 0; p < kPollPasses; ++p) {
+      const int b = base + grp + 32 * p;
+      // (words not read count as arrived)
+      wv[p] = (b < gs.wg1 && j < 2 * NV) ? pub_load(&words[((size_t)slot * MFEM_RED_MAX_BLOCKS + b) * 8 + j])
+                                          : ((uint64_t)ep << 32);
+    }
+    // poll until every word of this lane carries the current epoch,
+    // reloading the stale ones together (one round trip per poll)
+    auto pending = [&]() {
+      bool any = false;
+#pragma unroll
+      for (int p = 0; p < kPollPasses; ++p) any |= (uint32_t)(wv[p] >> 32) != ep;
+      return any;
+    };
+    if (pending() && !to) {
+      const uint64_t t0 = wall_clock64();
+      do {
+        // error bits and the time limit are looked at only once a wait is
+        // long: the status load would add a round trip to every poll
+        const uint64_t dt = wall_clock64() - t0;
+        if (dt > kStatusCheckAfter && (dt > kBarrierTimeout || pub_load(&gs.status[MFEM_STATUS_ERRBITS]) != 0)) {
+          to = true;
+          break;
         }
-        if (to) break;
-      }
-      acc[i] += __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int p = 0; p < kPollPasses; ++p) {
+          if ((uint32_t)(wv[p] >> 32) != ep)
+            wv[p] = pub_load(&words[((size_t)slot * MFEM_RED_MAX_BLOCKS + base + grp + 32 * p) * 8 + j]);
+        }
+      } while (pending());
+    }
+    // (all lanes active again: DPP reads every lane of the quad)
+#pragma unroll
+    for (int p = 0; p < kPollPasses; ++p) {
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)wv[p], 0xB1, 0xF, 0xF, false);
+      const bool mine = (j & 1) == 0 && j < 2 * NV && base + grp + 32 * p < gs.wg1;
+      if (mine) a += __longlong_as_double((long long)(((uint64_t)hi << 32) | (uint32_t)wv[p]));
     }
   }
   if (to) {
     atomicOr(&gs.status[MFEM_STATUS_ERRBITS], 1);
     abort_flag = 1;
   }
+  // per wave: value i sits in lanes 2 i + 8 g (g = 0..7); pairs of groups
+  // one row apart through DPP (row_ror 8), then the four rows read out in
+  // order
+  a = dpp_add<0x128>(a);
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
-    const double w = wave_sum(acc[i]);
+    const double w = (lane_value(a, 2 * i) + lane_value(a, 16 + 2 * i)) +
+                     (lane_value(a, 32 + 2 * i) + lane_value(a, 48 + 2 * i));
     if (lane == 0) tot[wave][i] = w;
   }
   __syncthreads();
@@ -686,10 +726,10 @@ __device__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
   }
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
-    double a = tot[0][i];
+    double t = tot[0][i];
 #pragma unroll
-    for (int w = 1; w < kWaves; ++w) a += tot[w][i];
-    v[i] = a;
+    for (int w = 1; w < kWaves; ++w) t += tot[w][i];
+    v[i] = lane_value(t, 0);  // (uniform: scalar registers)
   }
   // (no closing barrier: the next call writes tot only after its first
   // barrier, which every thread reaches after these reads; before it, it
@@ -1403,7 +1443,12 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
     }
     if (!restart(rz, rr)) return;
     bool first = true;
-    double gamma_prev = 1.0, alpha_prev = 1.0, beta = 0.0;
+    double beta = 0.0;
+    // pipelined: the previous gamma and alpha, one copy per wave in LDS
+    // (wave-uniform; held in registers they are spilled to scratch, whose
+    // reload would sit on the critical path after each reduction)
+    __shared__ double carried[kWaves][2];
+    const int cw = threadIdx.x >> 6;
     PCG_LAP(5);
     for (;;) {
       double gamma = 0.0, delta = 0.0;
@@ -1514,8 +1559,8 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
           beta = 0.0;
           alpha = gamma / delta;
         } else {
-          beta = gamma / gamma_prev;
-          alpha = gamma / (delta - beta * gamma / alpha_prev);
+          beta = gamma / carried[cw][0];
+          alpha = gamma / (delta - beta * gamma / carried[cw][1]);
         }
         if (!(alpha > 0.0) || !isfinite(alpha)) {  // breakdown: fall to the residual check
           it = S.cg_maxiter;
@@ -1539,8 +1584,10 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
           }
         }
         PCG_LAP(2);
-        gamma_prev = gamma;
-        alpha_prev = alpha;
+        if (lane == 0) {
+          carried[cw][0] = gamma;
+          carried[cw][1] = alpha;
+        }
         first = false;
         ++it;
         continue;
