@@ -950,7 +950,8 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
   float M[CPW][9], b[CPW][3];
   // standard variant: p, q, z in FP32
   float p[CPW][3], q[CPW][3], z[CPW][3];
-  // pipelined variant: u, w, p, s, q, z all FP64
+  // pipelined variant: w, p, s, z in FP64; u = M r is recomputed where used
+  // (u at a restart and u, q of the FP32-published form are held)
   double u[CPW][3], w[CPW][3], pd[CPW][3], sd[CPW][3], qd[CPW][3], zd[CPW][3];
   RowBlocks<RW> rb;
   int pub = 0, pubx = 0;  // publish counters: buffer = count & 1
@@ -1444,6 +1445,11 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
     if (!restart(rz, rr)) return;
     bool first = true;
     double beta = 0.0;
+    // pipelined: m = M w already published by the previous update, with the
+    // dot products of the new vectors (the update issues the stores first
+    // and overlaps the rest of its work with their completion)
+    bool pre = false;
+    double gamma_pre = 0.0, delta_pre = 0.0, rr_pre = 0.0;
     // pipelined: the previous gamma and alpha, one copy per wave in LDS
     // (wave-uniform; held in registers they are spilled to scratch, whose
     // reload would sit on the critical path after each reduction)
@@ -1457,9 +1463,14 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         rr = 0.0;
         double* mb = pxbuf(pubx);
         float* mb32 = pzbuf(pub);
+        if (pre) {
+          gamma = gamma_pre;
+          delta = delta_pre;
+          rr = rr_pre;
+        }
 #pragma unroll
         for (int k = 0; k < CPW; ++k) {
-          if (!own[k]) continue;  // (uniform over the wave)
+          if (!own[k] || pre) continue;  // (uniform over the wave)
           const int row = chunk[k] * 64 + lane;
           double m[3];
           apply_M<CPW, double>(M, k, w[k], m);
@@ -1478,15 +1489,26 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
             for (int j = 0; j < 3; ++j) pub_store(&mb[(size_t)chunk[k] * 192 + lane + 64 * j], trw[lane + 64 * j]);
             __builtin_amdgcn_wave_barrier();
           }
+          // u = M r: held by recurrence in the FP32-published form, else
+          // recomputed from r where used (exact in exact arithmetic; fewer
+          // registers and no drift between u and M r)
+          double ul[3];
+          if constexpr (PUB32) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) ul[a] = u[k][a];
+          } else {
+            apply_M<CPW, double>(M, k, r[k], ul);
+          }
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
-            gamma += r[k][a] * u[k][a];
-            delta += w[k][a] * u[k][a];
+            gamma += r[k][a] * ul[a];
+            delta += w[k][a] * ul[a];
             rr += r[k][a] * r[k][a];
           }
         }
         if constexpr (PUB32) ++pub;
-        else ++pubx;
+        else if (!pre) ++pubx;
+        pre = false;
         PCG_LAP(2);
         double red[3] = {gamma, delta, rr};
         if (!grid_reduce(gs, red)) return;
@@ -1566,21 +1588,67 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
           it = S.cg_maxiter;
           continue;
         }
+        if constexpr (!PUB32) {
+          // w first; the next m = M w goes out at once, and the remaining
+          // recurrences and the next dot products run while it drains
+          double* mb = pxbuf(pubx);
+          double g2 = 0.0, d2 = 0.0, r2 = 0.0;
 #pragma unroll
-        for (int k = 0; k < CPW; ++k) {
-          if (!own[k]) continue;
-          double m[3];
-          apply_M<CPW, double>(M, k, w[k], m);
+          for (int k = 0; k < CPW; ++k) {
+            if (!own[k]) continue;  // (uniform over the wave)
+            double wo[3], mn[3], ul[3];
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            zd[k][a] = n[k][a] + beta * zd[k][a];
-            qd[k][a] = m[a] + beta * qd[k][a];
-            sd[k][a] = w[k][a] + beta * sd[k][a];
-            pd[k][a] = u[k][a] + beta * pd[k][a];
-            x[k][a] += alpha * pd[k][a];
-            r[k][a] -= alpha * sd[k][a];
-            u[k][a] -= alpha * qd[k][a];
-            w[k][a] -= alpha * zd[k][a];
+            for (int a = 0; a < 3; ++a) {
+              wo[a] = w[k][a];
+              zd[k][a] = n[k][a] + beta * zd[k][a];
+              w[k][a] -= alpha * zd[k][a];
+            }
+            apply_M<CPW, double>(M, k, w[k], mn);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) trw[3 * lane + a] = mn[a];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int j = 0; j < 3; ++j) pub_store(&mb[(size_t)chunk[k] * 192 + lane + 64 * j], trw[lane + 64 * j]);
+            __builtin_amdgcn_wave_barrier();
+            // u = M r before and after the r update (no q recurrence)
+            apply_M<CPW, double>(M, k, r[k], ul);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              sd[k][a] = wo[a] + beta * sd[k][a];
+              pd[k][a] = ul[a] + beta * pd[k][a];
+              x[k][a] += alpha * pd[k][a];
+              r[k][a] -= alpha * sd[k][a];
+            }
+            apply_M<CPW, double>(M, k, r[k], ul);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              g2 += r[k][a] * ul[a];
+              d2 += w[k][a] * ul[a];
+              r2 += r[k][a] * r[k][a];
+            }
+          }
+          ++pubx;
+          pre = true;
+          gamma_pre = g2;
+          delta_pre = d2;
+          rr_pre = r2;
+        } else {
+#pragma unroll
+          for (int k = 0; k < CPW; ++k) {
+            if (!own[k]) continue;
+            double m[3];
+            apply_M<CPW, double>(M, k, w[k], m);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              zd[k][a] = n[k][a] + beta * zd[k][a];
+              qd[k][a] = m[a] + beta * qd[k][a];
+              sd[k][a] = w[k][a] + beta * sd[k][a];
+              pd[k][a] = u[k][a] + beta * pd[k][a];
+              x[k][a] += alpha * pd[k][a];
+              r[k][a] -= alpha * sd[k][a];
+              u[k][a] -= alpha * qd[k][a];
+              w[k][a] -= alpha * zd[k][a];
+            }
           }
         }
         PCG_LAP(2);
