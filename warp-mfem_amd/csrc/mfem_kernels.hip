@@ -336,6 +336,31 @@ __device__ __forceinline__ void gather_tet(const mfem_solver& S, int e, float (&
   load3(S.x, tv.w, xv[3]);
 }
 
+// Wave-cooperative store of whole per-tet records of N floats (record e at
+// arr + N e) from registers, through this wave's LDS slab: a record-per-lane
+// store touches one 64-byte line per lane and instruction, the wave's records
+// are one contiguous span, so lane t stores elements t, t + 64, ... of the
+// span (a few lines per instruction) after the slab has transposed them.  Slab rows have an odd stride (no bank conflicts).  LDS
+// operations of a wave complete in order.  nvalid = records of the wave that
+// exist (the others are not written).
+constexpr int slab_stride(int n) { return n | 1; }
+constexpr int kSlabFloats = 64 * 73;  // rows of up to 72 floats
+template <int N>
+__device__ __forceinline__ void wave_store(float* slab, float* dst, const float* v, int nvalid, int lane) {
+  constexpr int W = slab_stride(N);
+  static_assert(64 * W <= kSlabFloats, "slab size");
+#pragma unroll
+  for (int q = 0; q < N; ++q) slab[lane * W + q] = v[q];
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int t = lane + 64 * k;
+    const int l = t / N;
+    if (l < nvalid) dst[t] = slab[l * W + (t - l * N)];
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 __global__ void __launch_bounds__(kBlock) k_init_stretch(mfem_solver S) {
   int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= S.n_tets) return;
@@ -352,9 +377,16 @@ __global__ void __launch_bounds__(kBlock) k_init_stretch(mfem_solver S) {
   for (int k = 0; k < 6; ++k) S.s[6 * e + k] = S6[k];
 }
 
-__global__ void __launch_bounds__(kBlock) k_element(mfem_solver S) {
-  int e = blockIdx.x * kBlock + threadIdx.x;
-  if (e >= S.n_tets) return;
+__global__ void __launch_bounds__(kBlock, 2) k_element(mfem_solver S) {
+  // per-tet outputs leave through wave_store; lanes past the last tet compute
+  // the last tet again and store nothing
+  __shared__ float slabs[kWaves][kSlabFloats];
+  const int lane = threadIdx.x & 63;
+  const int e0 = blockIdx.x * kBlock + (threadIdx.x & ~63);
+  const int nvalid = min(64, S.n_tets - e0);
+  if (nvalid <= 0) return;  // (the whole wave)
+  const int e = min(e0 + lane, S.n_tets - 1);
+  float* slab = slabs[threadIdx.x >> 6];
   float xv[4][3];
   int4 tv;
   gather_tet(S, e, xv, tv);
@@ -406,21 +438,22 @@ __global__ void __launch_bounds__(kBlock) k_element(mfem_solver S) {
     hg[a] = acc;
   }
   // stores needed by the local solve and the line search
+  wave_store<72>(slab, S.G + (size_t)72 * e0, G, nvalid, lane);
+  {
+    float h21[21], i21[21];
 #pragma unroll
-  for (int q = 0; q < 72; ++q) S.G[72 * e + q] = G[q];
+    for (int i = 0; i < 6; ++i)
 #pragma unroll
-  for (int i = 0; i < 6; ++i)
-#pragma unroll
-    for (int j = i; j < 6; ++j) {
-      S.Hl[21 * e + sym21(i, j)] = Hl[i * 6 + j];
-      S.Hsinv[21 * e + sym21(i, j)] = Hi[i * 6 + j];
-    }
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    S.gl[6 * e + k] = gl[k];
-    S.gs[6 * e + k] = gs[k];
+      for (int j = i; j < 6; ++j) {
+        h21[sym21(i, j)] = Hl[i * 6 + j];
+        i21[sym21(i, j)] = Hi[i * 6 + j];
+      }
+    wave_store<21>(slab, S.Hl + (size_t)21 * e0, h21, nvalid, lane);
+    wave_store<21>(slab, S.Hsinv + (size_t)21 * e0, i21, nvalid, lane);
   }
-  S.psi[e] = psi;
+  wave_store<6>(slab, S.gl + (size_t)6 * e0, gl, nvalid, lane);
+  wave_store<6>(slab, S.gs + (size_t)6 * e0, gs, nvalid, lane);
+  if (lane < nvalid) S.psi[e] = psi;
   // HG = H_l G  (6x12)
   float HG[72];
 #pragma unroll
@@ -433,17 +466,21 @@ __global__ void __launch_bounds__(kBlock) k_element(mfem_solver S) {
       HG[a * 12 + k] = acc;
     }
   // force f = G^T H_l g_l and the 10 unique blocks of K = G^T H_l G
+  {
+    float fe[12];
 #pragma unroll
-  for (int k = 0; k < 12; ++k) {
-    float acc = 0.f;
+    for (int k = 0; k < 12; ++k) {
+      float acc = 0.f;
 #pragma unroll
-    for (int a = 0; a < 6; ++a) acc += G[a * 12 + k] * hg[a];
-    S.fe[12 * e + k] = acc;
+      for (int a = 0; a < 6; ++a) acc += G[a * 12 + k] * hg[a];
+      fe[k] = acc;
+    }
+    wave_store<12>(slab, S.fe + (size_t)12 * e0, fe, nvalid, lane);
   }
   const size_t T = (size_t)S.n_tets;
 #pragma unroll
   for (int p = 0; p < 10; ++p) {
-    float* dst = S.Kb + ((size_t)p * T + e) * 9;
+    float kb[9];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -451,8 +488,9 @@ __global__ void __launch_bounds__(kBlock) k_element(mfem_solver S) {
         float acc = 0.f;
 #pragma unroll
         for (int a = 0; a < 6; ++a) acc += G[a * 12 + 3 * PA[p] + r] * HG[a * 12 + 3 * PB[p] + cc];
-        dst[3 * r + cc] = acc;
+        kb[3 * r + cc] = acc;
       }
+    wave_store<9>(slab, S.Kb + ((size_t)p * T + e0) * 9, kb, nvalid, lane);
   }
 }
 
