@@ -312,7 +312,8 @@ def main():
             "bytes_per_cg_iter": bytes_per_iter,
             "traffic_note": "traffic = measured HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
                             "profiles/hbm_traffic_octopus_204k.json); below the algorithmic bytes because the "
-                            "matrix rows stay in VGPRs for the whole solve and the vectors in L2 / Infinity Cache",
+                            "matrix rows stay in VGPRs for the whole solve: an iteration moves only the published "
+                            "vector (staged per workgroup in LDS)",
             "us_per_cg_iter": float(np.sum(pcg_ms[active]) * 1e3 / max(1.0, np.sum(iters_flat[active]))),
         },
     }
