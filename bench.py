@@ -276,9 +276,10 @@ def main():
                                          "local_sweeps", "ms_per_step", "us_per_iter", "cg_iters")}
                       for r in tune["table"]]}
     if learned:
-        config["pcg_start_frac"] = {name: float((st[:, :, 14] == k).mean()) for k, name in
-                                    enumerate(["x_tilde_minus_x", "learned", "prev_newton_step"])}
-        config["learned_guess_residual_ratio_median"] = float(np.median(st[:, :, 15]))
+        config["pcg_start_frac"] = {"x_tilde_minus_x": float((st[:, :, 14] == 0).mean()),
+                                    "galerkin": float((st[:, :, 14] == 3).mean())}
+        # |g - H x0| / |g - H (x~ - x)| of the Galerkin start
+        config["start_residual_ratio_median"] = float(np.median(st[:, :, 15]))
         config["max_abs_position_diff_vs_plain"] = pos_diff
         config["training"] = train_info
         config["ablation_prev_newton_step_only"] = {
@@ -513,8 +514,8 @@ def run_batch(args):
                       for r in tune["table"]]}
     if learned:
         config["training"] = train_info
-        config["pcg_start_frac"] = {name: float((st[..., 14] == k).mean()) for k, name in
-                                    enumerate(["x_tilde_minus_x", "learned", "prev_newton_step"])}
+        config["pcg_start_frac"] = {"x_tilde_minus_x": float((st[..., 14] == 0).mean()),
+                                    "galerkin": float((st[..., 14] == 3).mean())}
         config["ablation_block_jacobi"] = {
             "value": T * args.steps / wall_j, "ms_per_step": wall_j * 1e3 / args.steps,
             "cg_iters_per_step_launch": float(st_j[..., 0].max(axis=1).sum(axis=1).mean())}

@@ -42,10 +42,10 @@ extern "C" {
 /* stats[iter][k]: 0 cg_iters, 1 cg_relres (FP64, true residual), 2 ls_trials,
    3 ls_accepted, 4 alpha, 5 merit f0, 6 merit f1, 7 cg_restarts,
    8..13 PCG phase device-clock ticks (SpMV, reduce 1, update, reduce 2,
-   residual checks, prologue), 14 start used with the learned warm start
-   (0 x~ - x, 1 learned guess, 2 previous Newton step: the lowest
-   quadratic-model energy), 15 |g - H x0_learned| / |g - H x0_plain|
-   (learned warm start only) */
+   residual checks, prologue), 14 start used (0 x~ - x; 3 the learned warm
+   start's Galerkin start, the lowest quadratic-model energy over x~ - x,
+   the learned guess and the last three Newton steps), 15 |g - H x0| /
+   |g - H (x~ - x)| of the Galerkin start */
 #define MFEM_STATS_PER_ITER 16
 #define MFEM_MAX_STATS_ITERS 64
 #define MFEM_RED_MAX_BLOCKS 2048
@@ -54,7 +54,7 @@ extern "C" {
 /* pcg_flags bits */
 #define MFEM_PCG_PIPELINED 1   /* one grid reduction per CG iteration (pipelined PCG) */
 #define MFEM_PCG_STREAM_ROWS 2 /* stream matrix rows from HBM instead of keeping them in VGPRs */
-#define MFEM_PCG_LEARNED_GUESS 4 /* start from the lowest-energy of x~ - x, the learned guess, the previous step */
+#define MFEM_PCG_LEARNED_GUESS 4 /* start from the lowest-energy combination of x~ - x, the learned guess and the last three Newton steps (needs dx_hist) */
 
 /* status[] words */
 #define MFEM_STATUS_ERRBITS 0 /* bit 0: grid barrier timed out; the persistent kernels (PCG, line search)
@@ -201,6 +201,12 @@ typedef struct mfem_solver {
      Standard (non-pipelined) PCG variants only. */
   int32_t local_sweeps;
   float local_omega;
+  /* Newton-step history for the learned warm start's Galerkin start (no
+     counterpart in the reference): [2][n_chunks*64][3], the two steps
+     before dx, rotated by the PCG when it writes dx; null = no history
+     (and the plain start).  Buffers dx and dx_hist hold at least
+     n_chunks*64 rows (the PCG reads them whole, in 64-byte lines). */
+  double* dx_hist;
 } mfem_solver;
 
 /* ABI self-description (checked against the Python mirror by the tests) */

@@ -1383,8 +1383,9 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
     return true;
   };
 
-  int guess_used = 0;  // 0 x~ - x, 1 learned, 2 previous Newton step
+  int guess_used = 0;  // 0 x~ - x, 3 Galerkin start over the candidates
   double guess_ratio = 0.0;
+  double guess_rrp = 0.0;  // |g - H x0|^2 of the plain start (Galerkin start only)
   if (bnorm2 > 0.0) {
     double rz = 0.0, rr = 0.0;
     {
@@ -1396,91 +1397,170 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
 #pragma unroll
         for (int a = 0; a < 3; ++a) r[k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
     }
-    if ((S.pcg_flags & MFEM_PCG_LEARNED_GUESS) && S.guess) {
-      // learned warm start: three candidate starts, x~ - x (the plain guess),
-      // the learned guess and the previous Newton step, each with its
-      // residual r = g - H x0 and quadratic-model energy
-      // Phi(x0) = x0.H x0 / 2 - g.x0 = -x0.(g + r) / 2, which orders the
-      // starts by the energy norm of their error (the norm CG minimises);
-      // the lowest-energy start is kept.
-      double phi0 = 0.0, r0 = 0.0;
+    if ((S.pcg_flags & MFEM_PCG_LEARNED_GUESS) && S.guess && S.dx_hist) {
+      // Learned warm start, Galerkin form.  Five candidate starts c_0..c_4:
+      // x~ - x (the plain guess), the learned guess, the previous Newton step
+      // and the two before it.  The start is the x0 in their span with the
+      // lowest quadratic-model energy Phi(x0) = x0.H x0 / 2 - g.x0, i.e.
+      // the smallest energy-norm error, the norm CG minimises: x0 = C a
+      // with (C^T H C) a = C^T g.  It is never worse than the best single
+      // candidate.  Gram entries from one SpMV per candidate (the histories
+      // were written by the previous launch, so only the learned guess is
+      // published), five 4-value grid reductions, a 5 x 5 Cholesky that
+      // drops candidates already in the span of the earlier ones (pivot <=
+      // 1e-10 of the diagonal: equal or empty histories), then r0 = g - H x0.
+      const double* h0 = S.dx_hist;
+      const double* h1 = S.dx_hist + nrow_pad * 3;
+      auto cget = [&](int i, int k, int a) -> double {
+        const int row = chunk[k] * 64 + lane;
+        if (!live[k]) return 0.0;
+        if (i == 0) return x[k][a];
+        if (i == 1) return (double)S.guess[3 * row + a];
+        if (i == 2) return S.dx[3 * row + a];
+        if (i == 3) return h0[3 * row + a];
+        return h1[3 * row + a];
+      };
+      // sum over this lane's rows of c_i . y (y: an SpMV result) or c_i . g
+      auto cdot = [&](int i, const double (&y)[CPW][3]) -> double {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < CPW; ++k)
+#pragma unroll
+          for (int a = 0; a < 3; ++a) acc += cget(i, k, a) * y[k][a];
+        return acc;
+      };
+      auto gdot = [&](int i) -> double {
+        reload_b();
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < CPW; ++k)
+#pragma unroll
+          for (int a = 0; a < 3; ++a) acc += cget(i, k, a) * (double)b[k][a];
+        return acc;
+      };
+      double rrp = 0.0;  // |g - H c_0|^2 (plain start)
 #pragma unroll
       for (int k = 0; k < CPW; ++k)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          phi0 -= 0.5 * x[k][a] * ((double)b[k][a] + r[k][a]);
-          r0 += r[k][a] * r[k][a];
+        for (int a = 0; a < 3; ++a) rrp += r[k][a] * r[k][a];
+      {
+        double* gb = pxbuf(pubx);
+#pragma unroll
+        for (int k = 0; k < CPW; ++k) {
+          if (!own[k]) continue;
+          const int row = chunk[k] * 64 + lane;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) pub_store(&gb[3 * row + a], cget(1, k, a));
         }
-      // candidate c of a row: 1 learned guess, 2 previous Newton step
-      auto cand = [&](int c, int row, int a) -> double {
-        return c == 1 ? (double)S.guess[3 * row + a] : S.dx[3 * row + a];
-      };
-      double* gb = pxbuf(pubx);
+        ++pubx;
+      }
+      double Gm[5][5], fv[5];
+      {
+        double y[CPW][3];  // H c_0 = g - r
+        reload_b();
+#pragma unroll
+        for (int k = 0; k < CPW; ++k)
+#pragma unroll
+          for (int a = 0; a < 3; ++a) y[k][a] = live[k] ? (double)b[k][a] - r[k][a] : 0.0;
+        double P[4] = {cdot(0, y), cdot(1, y), cdot(2, y), cdot(3, y)};
+        double Q[4] = {cdot(4, y), gdot(0), gdot(1), gdot(2)};
+        if (!grid_reduce(gs, P)) return;  // (also publishes the learned guess)
+        if (!grid_reduce(gs, Q)) return;
+        Gm[0][0] = P[0], Gm[0][1] = P[1], Gm[0][2] = P[2], Gm[0][3] = P[3], Gm[0][4] = Q[0];
+        fv[0] = Q[1], fv[1] = Q[2], fv[2] = Q[3];
+      }
+      double R4[4], T4[4], U4[4];
+      {
+        double y[CPW][3];
+        spmv_check64(pxbuf(pubx - 1), y);  // H c_1
+        R4[0] = cdot(1, y), R4[1] = cdot(2, y), R4[2] = cdot(3, y), R4[3] = cdot(4, y);
+        __syncthreads();  // (the staged SpMV's line buffer is reused)
+        spmv_check64(S.dx, y);  // H c_2
+        T4[0] = cdot(2, y), T4[1] = cdot(3, y), T4[2] = cdot(4, y), T4[3] = gdot(3);
+        __syncthreads();
+        spmv_check64(h0, y);  // H c_3
+        U4[0] = cdot(3, y), U4[1] = cdot(4, y), U4[2] = gdot(4);
+        __syncthreads();
+        spmv_check64(h1, y);  // H c_4
+        U4[3] = cdot(4, y);
+      }
+      if (!grid_reduce(gs, R4)) return;
+      if (!grid_reduce(gs, T4)) return;
+      if (!grid_reduce(gs, U4)) return;
+      Gm[1][1] = R4[0], Gm[1][2] = R4[1], Gm[1][3] = R4[2], Gm[1][4] = R4[3];
+      Gm[2][2] = T4[0], Gm[2][3] = T4[1], Gm[2][4] = T4[2], fv[3] = T4[3];
+      Gm[3][3] = U4[0], Gm[3][4] = U4[1], fv[4] = U4[2], Gm[4][4] = U4[3];
+      // pivoted Cholesky of the Gram matrix (same scalars in every workgroup);
+      // the factor L (lower triangle, diagonal included) overwrites Gm in
+      // place: Gm holds the Gram entries in its upper triangle
+      double z[5], cf[5];
+      bool kept[5];
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        const double gjj = Gm[j][j];
+        double d = gjj;
+#pragma unroll
+        for (int q = 0; q < j; ++q) d -= kept[q] ? Gm[j][q] * Gm[j][q] : 0.0;
+        kept[j] = gjj > 0.0 && d > 1e-10 * gjj;
+        Gm[j][j] = kept[j] ? sqrt(d) : 1.0;
+#pragma unroll
+        for (int i = j + 1; i < 5; ++i) {
+          double v = Gm[j][i];
+#pragma unroll
+          for (int q = 0; q < j; ++q) v -= kept[q] ? Gm[i][q] * Gm[j][q] : 0.0;
+          Gm[i][j] = kept[j] ? v / Gm[j][j] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        double v = fv[j];
+#pragma unroll
+        for (int q = 0; q < j; ++q) v -= Gm[j][q] * z[q];
+        z[j] = kept[j] ? v / Gm[j][j] : 0.0;
+      }
+#pragma unroll
+      for (int j = 4; j >= 0; --j) {
+        double v = z[j];
+#pragma unroll
+        for (int i = j + 1; i < 5; ++i) v -= Gm[i][j] * cf[i];
+        cf[j] = kept[j] ? v / Gm[j][j] : 0.0;
+      }
+      // x0 = C a, published for its residual
+      double* xb = pxbuf(pubx);
 #pragma unroll
       for (int k = 0; k < CPW; ++k) {
-        if (!own[k]) continue;
-        const int row = chunk[k] * 64 + lane;
+        double xn[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) pub_store(&gb[3 * row + a], live[k] ? cand(1, row, a) : 0.0);
+        for (int a = 0; a < 3; ++a)
+          xn[a] = cf[0] * cget(0, k, a) + cf[1] * cget(1, k, a) + cf[2] * cget(2, k, a) + cf[3] * cget(3, k, a) +
+                  cf[4] * cget(4, k, a);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) x[k][a] = xn[a];
+        if (own[k]) {
+          const int row = chunk[k] * 64 + lane;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) pub_store(&xb[3 * row + a], x[k][a]);
+        }
       }
       ++pubx;
-      double red0[2] = {phi0, r0};
+      double red0[1] = {rrp};
       if (!grid_reduce(gs, red0)) return;
-      phi0 = red0[0];
-      r0 = red0[1];
-      // (the plain guess's SpMV input is no longer read by any workgroup:
-      // its buffer takes the previous Newton step)
-      double rc[2][CPW][3];
-      double phic[2], rrc[2];
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        double* cb = pxbuf(pubx - 1);
+      rrp = red0[0];
+      {
         double y[CPW][3];
-        spmv_check64(cb, y);
+        spmv_check64(xb, y);
         reload_b();
-        double ph = 0.0, rq = 0.0;
 #pragma unroll
-        for (int k = 0; k < CPW; ++k) {
-          const int row = chunk[k] * 64 + lane;
+        for (int k = 0; k < CPW; ++k)
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            rc[c][k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
-            ph -= live[k] ? 0.5 * cand(c + 1, row, a) * ((double)b[k][a] + rc[c][k][a]) : 0.0;
-            rq += rc[c][k][a] * rc[c][k][a];
-          }
-        }
-        if (c == 0) {
-          double* nb = pxbuf(pubx);
-#pragma unroll
-          for (int k = 0; k < CPW; ++k) {
-            if (!own[k]) continue;
-            const int row = chunk[k] * 64 + lane;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) pub_store(&nb[3 * row + a], live[k] ? cand(2, row, a) : 0.0);
-          }
-          ++pubx;
-        }
-        double redc[2] = {ph, rq};
-        if (!grid_reduce(gs, redc)) return;
-        phic[c] = redc[0];
-        rrc[c] = redc[1];
-      }
-      guess_ratio = r0 > 0.0 ? sqrt(rrc[0] / r0) : 0.0;
-      const int best = (phic[0] <= phic[1]) ? (phic[0] < phi0 ? 1 : 0) : (phic[1] < phi0 ? 2 : 0);
-      if (best != 0) {
-        guess_used = best;
-#pragma unroll
-        for (int k = 0; k < CPW; ++k) {
-          const int row = chunk[k] * 64 + lane;
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            r[k][a] = rc[best - 1][k][a];
-            x[k][a] = live[k] ? cand(best, row, a) : 0.0;
-          }
-        }
+          for (int a = 0; a < 3; ++a) r[k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
+        guess_used = 3;
+        guess_rrp = rrp;
       }
     }
     if (!restart(rz, rr)) return;
+    // (rr: |g - H x0|^2 of the start, reduced by restart)
+    if (guess_used == 3) guess_ratio = guess_rrp > 0.0 ? sqrt(rr / guess_rrp) : 0.0;
     bool first = true;
     double beta = 0.0;
     // pipelined: m = M w already published by the previous update, with the
@@ -1762,11 +1842,22 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       }
     }
   }
-  // write the solution (g == 0 gives dx = 0)
+  // write the solution (g == 0 gives dx = 0); the previous two steps move
+  // down the history (no other workgroup reads these rows any more: every
+  // workgroup has passed the final reduction)
 #pragma unroll
   for (int k = 0; k < CPW; ++k) {
     if (!live[k]) continue;
     const int row = chunk[k] * 64 + lane;
+    if (S.dx_hist) {
+      double* h0 = S.dx_hist;
+      double* h1 = S.dx_hist + nrow_pad * 3;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        h1[3 * row + a] = h0[3 * row + a];
+        h0[3 * row + a] = S.dx[3 * row + a];
+      }
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) S.dx[3 * row + a] = bnorm2 > 0.0 ? x[k][a] : 0.0;
   }
@@ -2288,7 +2379,7 @@ int launched() {
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
   X(lag_sid) X(G) X(Hl) X(gl) X(Hsinv) X(gs) X(psi) X(Kb) X(fe) X(gx) X(gB) X(gD) X(Hv) X(dist) X(eB)      \
-  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega)
+  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega) X(dx_hist)
 
 extern "C" {
 

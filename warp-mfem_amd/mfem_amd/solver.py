@@ -185,7 +185,12 @@ class RefinementSolver:
         b["Hv"] = zf(Vc, 9)
         for n in ("dist", "eB", "eD"):
             b[n] = zf(Vc)
-        b["dx"] = torch.zeros(Vc, 3, dtype=torch.float64, device=dev)
+        # dx and its history: the PCG reads them whole in 64-byte lines over
+        # n_chunks * 64 rows, so the allocations are padded to a multiple of
+        # 64 rows (dx is a view of its first Vc rows)
+        Vpad = -(-Vc // 64) * 64
+        b["dx"] = torch.zeros(Vpad, 3, dtype=torch.float64, device=dev)[:Vc]
+        b["dx_hist"] = torch.zeros(2 * Vpad * 3, dtype=torch.float64, device=dev)
         b["ds"] = zf(Tc, 6)
         b["red"] = torch.zeros(2 * 2 * _ffi.MFEM_RED_MAX_BLOCKS * 8, dtype=torch.float64, device=dev)
         b["flags"] = torch.zeros(_ffi.MFEM_RED_MAX_BLOCKS, dtype=torch.int32, device=dev)
@@ -285,6 +290,8 @@ class RefinementSolver:
         S = self._S
         S.n_verts, S.n_tets, S.n_tris = int(n_verts), int(n_tets), int(n_tris)
         S.n_chunks, S.n_slots = pat.n_chunks, pat.n_slots
+        # the history's row stride is n_chunks * 64: it restarts empty
+        b["dx_hist"].zero_()
         S.max_width = int(np.diff(pat.chunk_ptr).max())
         S.tri_contact = int(self._tri_contact_kw and n_tris > 0)
         S.pcg_variant = -1  # block counts change with the topology: back to the automatic choice
@@ -453,8 +460,8 @@ class RefinementSolver:
     def set_warm_start(self, net) -> None:
         """Attach a trained ``learned.WarmStartNet`` (or None for plain PCG).
         With a network, every PCG solve first evaluates it and starts from the
-        lowest quadratic-model energy of x~ - x, the learned guess and the
-        previous Newton step."""
+        lowest quadratic-model energy over the span of x~ - x, the learned
+        guess and the last three Newton steps (a Galerkin start)."""
         self.warm_start_net = net
         if net is None:
             self._S.pcg_flags &= ~_ffi.PCG_LEARNED_GUESS
@@ -625,7 +632,7 @@ class RefinementSolver:
             if not info[4]:
                 S.pcg_variant = -1
 
-    _STEP_KEYS = ("s", "lam", "dx", "stats", "x_start", "x_tilde", "lag_n", "lag_fn", "lag_sid")
+    _STEP_KEYS = ("s", "lam", "dx", "dx_hist", "stats", "x_start", "x_tilde", "lag_n", "lag_fn", "lag_sid")
 
     def snapshot(self) -> dict:
         """Copies of the solver-held per-tet / per-vertex state that a step
