@@ -15,7 +15,8 @@ model selection by CG iterations over two validation steps (the trained
 correction or none); the autotuner's choice of PCG variant and workgroup-local
 preconditioner sweeps (0, 2 or 4) on whole steps; then K timed steps with plain
 PCG (the reference's start and block Jacobi), K with the learned warm start
-and the tuned preconditioner (the value), and as ablations K with the
+(the Galerkin start over x~ - x, the learned guess and the last three Newton
+steps) and the tuned preconditioner (the value), and as ablations K with the
 network's correction switched off (previous Newton step as the guess) and K
 with the learned start under block Jacobi, all from the same state and
 solver state.  One "step" = RefinementSolver.step on the whole body.
