@@ -44,7 +44,8 @@ extern "C" {
    8..13 PCG phase device-clock ticks (SpMV, reduce 1, update, reduce 2,
    residual checks, prologue), 14 start used (0 x~ - x; 3 the learned warm
    start's Galerkin start, the lowest quadratic-model energy over x~ - x,
-   the learned guess and the last three Newton steps), 15 |g - H x0| /
+   the learned guess, the last three Newton steps and the same Newton
+   iteration's solution one time step earlier), 15 |g - H x0| /
    |g - H (x~ - x)| of the Galerkin start */
 #define MFEM_STATS_PER_ITER 16
 #define MFEM_MAX_STATS_ITERS 64
@@ -54,7 +55,7 @@ extern "C" {
 /* pcg_flags bits */
 #define MFEM_PCG_PIPELINED 1   /* one grid reduction per CG iteration (pipelined PCG) */
 #define MFEM_PCG_STREAM_ROWS 2 /* stream matrix rows from HBM instead of keeping them in VGPRs */
-#define MFEM_PCG_LEARNED_GUESS 4 /* start from the lowest-energy combination of x~ - x, the learned guess and the last three Newton steps (needs dx_hist) */
+#define MFEM_PCG_LEARNED_GUESS 4 /* start from the lowest-energy combination of x~ - x, the learned guess, the last three Newton steps and the previous step's same iteration (needs dx_hist) */
 
 /* status[] words */
 #define MFEM_STATUS_ERRBITS 0 /* bit 0: grid barrier timed out; the persistent kernels (PCG, line search)
@@ -207,6 +208,12 @@ typedef struct mfem_solver {
      (and the plain start).  Buffers dx and dx_hist hold at least
      n_chunks*64 rows (the PCG reads them whole, in 64-byte lines). */
   double* dx_hist;
+  /* each Newton iteration's solution in the previous time step,
+     [dx_same_iters][n_chunks*64][3], written by the PCG of that iteration
+     and used by the next step's (the Galerkin start's sixth candidate; no
+     counterpart in the reference); null = none */
+  double* dx_same;
+  int32_t dx_same_iters;
 } mfem_solver;
 
 /* ABI self-description (checked against the Python mirror by the tests) */

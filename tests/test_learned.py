@@ -238,8 +238,9 @@ def _galerkin_host(H, g, cands, thr=1e-10):
 @pytest.mark.parametrize("jitter", [0.0, 0.05, 3.0])
 def test_galerkin_start_matches_host(gpu, jitter):
     """The PCG starts from the lowest quadratic-model energy over the span of
-    x~ - x, the learned guess and the last three Newton steps (stats[14] =
-    3): the start's residual ratio |g - H x0| / |g - H (x~ - x)|
+    x~ - x, the learned guess, the last three Newton steps and the same Newton
+    iteration's solution one step earlier (stats[14] = 3): the start's
+    residual ratio |g - H x0| / |g - H (x~ - x)|
     (stats[15]) equals the one of the same Galerkin start computed here in
     FP64 on the host from the assembled matrix and the solver's buffers, and
     that start's energy is at most every candidate's; nets from exact
@@ -266,9 +267,10 @@ def test_galerkin_start_matches_host(gpu, jitter):
         H = solver.system_csr().tocsr()
         g = b["g"][:n].double().cpu().numpy().reshape(-1)
         hist = b["dx_hist"].cpu().numpy()
+        same = b["dx_same"].cpu().numpy()[3 * rp * it: 3 * rp * it + 3 * n]
         c0 = (b["x_tilde"][:n].double() - out.particle_q[:n].double()).cpu().numpy().reshape(-1)
         cands = [c0, b["guess"][:n].double().cpu().numpy().reshape(-1), b["dx"][:n].cpu().numpy().reshape(-1),
-                 hist[: 3 * n], hist[3 * rp: 3 * rp + 3 * n]]
+                 hist[: 3 * n], hist[3 * rp: 3 * rp + 3 * n], same]
         x0, margin = _galerkin_host(H, g, cands)
         phi = lambda c: 0.5 * c @ (H @ c) - g @ c
         assert phi(x0) <= min(phi(c) for c in cands) + 1e-9 * abs(phi(x0))

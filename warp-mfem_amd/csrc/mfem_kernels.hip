@@ -1398,19 +1398,23 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         for (int a = 0; a < 3; ++a) r[k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
     }
     if ((S.pcg_flags & MFEM_PCG_LEARNED_GUESS) && S.guess && S.dx_hist) {
-      // Learned warm start, Galerkin form.  Five candidate starts c_0..c_4:
-      // x~ - x (the plain guess), the learned guess, the previous Newton step
-      // and the two before it.  The start is the x0 in their span with the
+      // Learned warm start, Galerkin form.  Six candidate starts c_0..c_5:
+      // x~ - x (the plain guess), the learned guess, the previous Newton step,
+      // the two before it, and this Newton iteration's solution in the
+      // previous time step.  The start is the x0 in their span with the
       // lowest quadratic-model energy Phi(x0) = x0.H x0 / 2 - g.x0, i.e.
       // the smallest energy-norm error, the norm CG minimises: x0 = C a
       // with (C^T H C) a = C^T g.  It is never worse than the best single
       // candidate.  Gram entries from one SpMV per candidate (the histories
       // were written by the previous launch, so only the learned guess is
-      // published), five 4-value grid reductions, a 5 x 5 Cholesky that
+      // published), seven 4-value grid reductions, a 6 x 6 Cholesky that
       // drops candidates already in the span of the earlier ones (pivot <=
       // 1e-10 of the diagonal: equal or empty histories), then r0 = g - H x0.
       const double* h0 = S.dx_hist;
       const double* h1 = S.dx_hist + nrow_pad * 3;
+      // c_5: this Newton iteration's solution in the previous time step
+      const bool has5 = S.dx_same && newton_iter >= 0 && newton_iter < S.dx_same_iters;
+      const double* h5 = has5 ? S.dx_same + (size_t)newton_iter * nrow_pad * 3 : nullptr;
       auto cget = [&](int i, int k, int a) -> double {
         const int row = chunk[k] * 64 + lane;
         if (!live[k]) return 0.0;
@@ -1418,7 +1422,8 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         if (i == 1) return (double)S.guess[3 * row + a];
         if (i == 2) return S.dx[3 * row + a];
         if (i == 3) return h0[3 * row + a];
-        return h1[3 * row + a];
+        if (i == 4) return h1[3 * row + a];
+        return has5 ? h5[3 * row + a] : 0.0;
       };
       // sum over this lane's rows of c_i . y (y: an SpMV result) or c_i . g
       auto cdot = [&](int i, const double (&y)[CPW][3]) -> double {
@@ -1454,7 +1459,20 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         }
         ++pubx;
       }
-      double Gm[5][5], fv[5];
+      // Gram entries (upper triangle) and right-hand sides, reduced in
+      // groups of four between the SpMVs and kept in LDS (uniform)
+      __shared__ double gal[28];
+      auto keep4 = [&](double (&v)[4], int o0, int o1, int o2, int o3) {
+        if (threadIdx.x == 0) {
+          gal[o0] = v[0];
+          gal[o1] = v[1];
+          gal[o2] = v[2];
+          gal[o3] = v[3];
+        }
+      };
+      // gal index of G_ij (i <= j) and of f_i
+      auto gi = [](int i, int j) { return i * 6 - i * (i - 1) / 2 + (j - i); };
+      constexpr int F0 = 21;
       {
         double y[CPW][3];  // H c_0 = g - r
         reload_b();
@@ -1463,40 +1481,60 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
 #pragma unroll
           for (int a = 0; a < 3; ++a) y[k][a] = live[k] ? (double)b[k][a] - r[k][a] : 0.0;
         double P[4] = {cdot(0, y), cdot(1, y), cdot(2, y), cdot(3, y)};
-        double Q[4] = {cdot(4, y), gdot(0), gdot(1), gdot(2)};
+        double Q[4] = {cdot(4, y), cdot(5, y), gdot(0), gdot(1)};
+        double Rr[4] = {gdot(2), gdot(3), gdot(4), gdot(5)};
         if (!grid_reduce(gs, P)) return;  // (also publishes the learned guess)
+        keep4(P, gi(0, 0), gi(0, 1), gi(0, 2), gi(0, 3));
         if (!grid_reduce(gs, Q)) return;
-        Gm[0][0] = P[0], Gm[0][1] = P[1], Gm[0][2] = P[2], Gm[0][3] = P[3], Gm[0][4] = Q[0];
-        fv[0] = Q[1], fv[1] = Q[2], fv[2] = Q[3];
+        keep4(Q, gi(0, 4), gi(0, 5), F0 + 0, F0 + 1);
+        if (!grid_reduce(gs, Rr)) return;
+        keep4(Rr, F0 + 2, F0 + 3, F0 + 4, F0 + 5);
       }
-      double R4[4], T4[4], U4[4];
       {
+        // (each grid reduction's workgroup barrier also frees the staged
+        // SpMV's line buffer for the next SpMV)
         double y[CPW][3];
         spmv_check64(pxbuf(pubx - 1), y);  // H c_1
-        R4[0] = cdot(1, y), R4[1] = cdot(2, y), R4[2] = cdot(3, y), R4[3] = cdot(4, y);
-        __syncthreads();  // (the staged SpMV's line buffer is reused)
+        double A4[4] = {cdot(1, y), cdot(2, y), cdot(3, y), cdot(4, y)};
+        double g15 = cdot(5, y);
+        if (!grid_reduce(gs, A4)) return;
+        keep4(A4, gi(1, 1), gi(1, 2), gi(1, 3), gi(1, 4));
         spmv_check64(S.dx, y);  // H c_2
-        T4[0] = cdot(2, y), T4[1] = cdot(3, y), T4[2] = cdot(4, y), T4[3] = gdot(3);
-        __syncthreads();
+        double B4[4] = {g15, cdot(2, y), cdot(3, y), cdot(4, y)};
+        double g25 = cdot(5, y);
+        if (!grid_reduce(gs, B4)) return;
+        keep4(B4, gi(1, 5), gi(2, 2), gi(2, 3), gi(2, 4));
         spmv_check64(h0, y);  // H c_3
-        U4[0] = cdot(3, y), U4[1] = cdot(4, y), U4[2] = gdot(4);
-        __syncthreads();
+        double C4[4] = {g25, cdot(3, y), cdot(4, y), cdot(5, y)};
+        if (!grid_reduce(gs, C4)) return;
+        keep4(C4, gi(2, 5), gi(3, 3), gi(3, 4), gi(3, 5));
         spmv_check64(h1, y);  // H c_4
-        U4[3] = cdot(4, y);
+        double g44 = cdot(4, y), g45 = cdot(5, y);
+        __syncthreads();
+        double g55 = 0.0;
+        if (has5) {
+          spmv_check64(h5, y);  // H c_5
+          g55 = cdot(5, y);
+        }
+        double D4[4] = {g44, g45, g55, 0.0};
+        if (!grid_reduce(gs, D4)) return;
+        keep4(D4, gi(4, 4), gi(4, 5), gi(5, 5), 27);
       }
-      if (!grid_reduce(gs, R4)) return;
-      if (!grid_reduce(gs, T4)) return;
-      if (!grid_reduce(gs, U4)) return;
-      Gm[1][1] = R4[0], Gm[1][2] = R4[1], Gm[1][3] = R4[2], Gm[1][4] = R4[3];
-      Gm[2][2] = T4[0], Gm[2][3] = T4[1], Gm[2][4] = T4[2], fv[3] = T4[3];
-      Gm[3][3] = U4[0], Gm[3][4] = U4[1], fv[4] = U4[2], Gm[4][4] = U4[3];
+      __syncthreads();
+      double Gm[6][6], fv[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        fv[i] = gal[F0 + i];
+#pragma unroll
+        for (int j = i; j < 6; ++j) Gm[i][j] = gal[gi(i, j)];
+      }
       // pivoted Cholesky of the Gram matrix (same scalars in every workgroup);
       // the factor L (lower triangle, diagonal included) overwrites Gm in
       // place: Gm holds the Gram entries in its upper triangle
-      double z[5], cf[5];
-      bool kept[5];
+      double z[6], cf[6];
+      bool kept[6];
 #pragma unroll
-      for (int j = 0; j < 5; ++j) {
+      for (int j = 0; j < 6; ++j) {
         const double gjj = Gm[j][j];
         double d = gjj;
 #pragma unroll
@@ -1504,7 +1542,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         kept[j] = gjj > 0.0 && d > 1e-10 * gjj;
         Gm[j][j] = kept[j] ? sqrt(d) : 1.0;
 #pragma unroll
-        for (int i = j + 1; i < 5; ++i) {
+        for (int i = j + 1; i < 6; ++i) {
           double v = Gm[j][i];
 #pragma unroll
           for (int q = 0; q < j; ++q) v -= kept[q] ? Gm[i][q] * Gm[j][q] : 0.0;
@@ -1512,17 +1550,17 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         }
       }
 #pragma unroll
-      for (int j = 0; j < 5; ++j) {
+      for (int j = 0; j < 6; ++j) {
         double v = fv[j];
 #pragma unroll
         for (int q = 0; q < j; ++q) v -= Gm[j][q] * z[q];
         z[j] = kept[j] ? v / Gm[j][j] : 0.0;
       }
 #pragma unroll
-      for (int j = 4; j >= 0; --j) {
+      for (int j = 5; j >= 0; --j) {
         double v = z[j];
 #pragma unroll
-        for (int i = j + 1; i < 5; ++i) v -= Gm[i][j] * cf[i];
+        for (int i = j + 1; i < 6; ++i) v -= Gm[i][j] * cf[i];
         cf[j] = kept[j] ? v / Gm[j][j] : 0.0;
       }
       // x0 = C a, published for its residual
@@ -1533,7 +1571,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
 #pragma unroll
         for (int a = 0; a < 3; ++a)
           xn[a] = cf[0] * cget(0, k, a) + cf[1] * cget(1, k, a) + cf[2] * cget(2, k, a) + cf[3] * cget(3, k, a) +
-                  cf[4] * cget(4, k, a);
+                  cf[4] * cget(4, k, a) + cf[5] * cget(5, k, a);
 #pragma unroll
         for (int a = 0; a < 3; ++a) x[k][a] = xn[a];
         if (own[k]) {
@@ -1857,6 +1895,11 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         h1[3 * row + a] = h0[3 * row + a];
         h0[3 * row + a] = S.dx[3 * row + a];
       }
+    }
+    if (S.dx_same && newton_iter >= 0 && newton_iter < S.dx_same_iters) {
+      double* hs = S.dx_same + (size_t)newton_iter * nrow_pad * 3;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) hs[3 * row + a] = bnorm2 > 0.0 ? x[k][a] : 0.0;
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) S.dx[3 * row + a] = bnorm2 > 0.0 ? x[k][a] : 0.0;
@@ -2379,7 +2422,7 @@ int launched() {
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
   X(lag_sid) X(G) X(Hl) X(gl) X(Hsinv) X(gs) X(psi) X(Kb) X(fe) X(gx) X(gB) X(gD) X(Hv) X(dist) X(eB)      \
-  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega) X(dx_hist)
+  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega) X(dx_hist) X(dx_same) X(dx_same_iters)
 
 extern "C" {
 

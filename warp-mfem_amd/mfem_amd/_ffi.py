@@ -31,7 +31,7 @@ FIELDS = [
     "tcon_ptr", "tcon_src", "vtri_ptr", "vtri_src", "tri_dist", "tri_bary", "tri_nrm", "tri_sid",
 )] + [("n_bodies", c_i32), ("ls_blocks", c_i32)] + [(n, c_ptr) for n in (
     "body_wg", "body_vert", "body_tet", "body_tri", "body_ls_wg", "body_epoch", "tet_energy",
-)] + [("body_wg_align", c_i32), ("local_sweeps", c_i32), ("local_omega", c_f32), ("dx_hist", c_ptr)]
+)] + [("body_wg_align", c_i32), ("local_sweeps", c_i32), ("local_omega", c_f32), ("dx_hist", c_ptr), ("dx_same", c_ptr), ("dx_same_iters", c_i32)]
 
 
 class MfemSolverStruct(ctypes.Structure):

@@ -191,6 +191,7 @@ class RefinementSolver:
         Vpad = -(-Vc // 64) * 64
         b["dx"] = torch.zeros(Vpad, 3, dtype=torch.float64, device=dev)[:Vc]
         b["dx_hist"] = torch.zeros(2 * Vpad * 3, dtype=torch.float64, device=dev)
+        b["dx_same"] = torch.zeros(int(iterations) * Vpad * 3, dtype=torch.float64, device=dev)
         b["ds"] = zf(Tc, 6)
         b["red"] = torch.zeros(2 * 2 * _ffi.MFEM_RED_MAX_BLOCKS * 8, dtype=torch.float64, device=dev)
         b["flags"] = torch.zeros(_ffi.MFEM_RED_MAX_BLOCKS, dtype=torch.int32, device=dev)
@@ -237,6 +238,7 @@ class RefinementSolver:
                                       float(kwargs.get("pcg_local_omega", 0.5)), S)
         S.energy = _ENERGIES[energy]
         S.iterations = self.iterations
+        S.dx_same_iters = self.iterations
         S.cg_maxiter = int(kwargs.get("cg_max_iterations", 1000))
         S.precond = int(bool(kwargs.get("preconditioner", True)))
         S.line_search = int(bool(kwargs.get("line_search", True)))
@@ -290,8 +292,9 @@ class RefinementSolver:
         S = self._S
         S.n_verts, S.n_tets, S.n_tris = int(n_verts), int(n_tets), int(n_tris)
         S.n_chunks, S.n_slots = pat.n_chunks, pat.n_slots
-        # the history's row stride is n_chunks * 64: it restarts empty
+        # the histories' row stride is n_chunks * 64: they restart empty
         b["dx_hist"].zero_()
+        b["dx_same"].zero_()
         S.max_width = int(np.diff(pat.chunk_ptr).max())
         S.tri_contact = int(self._tri_contact_kw and n_tris > 0)
         S.pcg_variant = -1  # block counts change with the topology: back to the automatic choice
@@ -632,7 +635,7 @@ class RefinementSolver:
             if not info[4]:
                 S.pcg_variant = -1
 
-    _STEP_KEYS = ("s", "lam", "dx", "dx_hist", "stats", "x_start", "x_tilde", "lag_n", "lag_fn", "lag_sid")
+    _STEP_KEYS = ("s", "lam", "dx", "dx_hist", "dx_same", "stats", "x_start", "x_tilde", "lag_n", "lag_fn", "lag_sid")
 
     def snapshot(self) -> dict:
         """Copies of the solver-held per-tet / per-vertex state that a step
