@@ -87,7 +87,7 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-struct FwdLds {
+struct __align__(16) FwdLds {
   uint16_t x[kTile][kIn];    // input tile (BF16)
   uint16_t h1[kTile][kHid];  // gelu(z1) (BF16)
   uint16_t h2[kTile][kHid];  // gelu(z2) (BF16)
@@ -122,8 +122,10 @@ __device__ void load_weights(WeightLds& W, const uint16_t* wbf, const float* wf3
 // forward pass of one 64-sample tile already staged in L.x; z1 / z2 (may be
 // null) receive the FP32 pre-activations
 __device__ void forward_tile(const WeightLds& W, FwdLds& L, int lane, float (*z1)[kHid], float (*z2)[kHid]) {
-  // layer 1: z1 = x W1 + b1 (64 x 64)
+  // layer 1: z1 = x W1 + b1 (64 x 64); the four column blocks of a row
+  // block are independent (unrolled: their LDS reads and MFMAs overlap)
   for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
       v4f acc = {0.f, 0.f, 0.f, 0.f};
       acc = mfma(frag_a(&L.x[0][0], kIn, 16 * rb, 0, lane), frag_b(&W.w1[0][0], kHid, 0, 16 * cb, lane), acc);
@@ -139,6 +141,7 @@ __device__ void forward_tile(const WeightLds& W, FwdLds& L, int lane, float (*z1
   __builtin_amdgcn_wave_barrier();
   // layer 2: z2 = h1 W2 + b2
   for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
       v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -169,7 +172,7 @@ __device__ void forward_tile(const WeightLds& W, FwdLds& L, int lane, float (*z1
 
 // Inference: Y[n][0..2] for n < N
 __global__ void __launch_bounds__(256) k_mlp_forward(const uint16_t* X, int N, const uint16_t* wbf, const float* wf32,
-                                                     float* Y) {
+                                                     float* Y, bool x16) {
   __shared__ WeightLds W;
   __shared__ FwdLds Ls[kBlockW];
   load_weights(W, wbf, wf32);
@@ -179,8 +182,21 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const uint16_t* X, int N, c
   const int ntiles = (N + kTile - 1) / kTile;
   for (int t = blockIdx.x * kBlockW + wv; t < ntiles; t += gridDim.x * kBlockW) {
     const int n = t * kTile + lane;
+    if (x16) {
+      // the sample's 16 BF16 features as two 16-byte loads (X 16-byte aligned)
+      uint4 q0 = make_uint4(0u, 0u, 0u, 0u), q1 = q0;
+      if (n < N) {
+        const uint4* xr = reinterpret_cast<const uint4*>(X + (size_t)n * kIn);
+        q0 = xr[0];
+        q1 = xr[1];
+      }
+      uint4* lx = reinterpret_cast<uint4*>(&L.x[lane][0]);
+      lx[0] = q0;
+      lx[1] = q1;
+    } else {
 #pragma unroll
-    for (int c = 0; c < kIn; ++c) L.x[lane][c] = n < N ? X[(size_t)n * kIn + c] : (uint16_t)0;
+      for (int c = 0; c < kIn; ++c) L.x[lane][c] = n < N ? X[(size_t)n * kIn + c] : (uint16_t)0;
+    }
     __builtin_amdgcn_wave_barrier();
     forward_tile(W, L, lane, nullptr, nullptr);
     if (n < N)
@@ -371,7 +387,8 @@ int mfem_mlp_forward(const uint16_t* X, int n, const uint16_t* w_bf16, const flo
   const int ntiles = (n + kTile - 1) / kTile;
   int nb = (ntiles + kBlockW - 1) / kBlockW;
   if (nb > 4096) nb = 4096;
-  hipLaunchKernelGGL(k_mlp_forward, dim3(nb), dim3(256), 0, (hipStream_t)stream, X, n, w_bf16, w_f32, y);
+  const bool x16 = (reinterpret_cast<uintptr_t>(X) & 15u) == 0;
+  hipLaunchKernelGGL(k_mlp_forward, dim3(nb), dim3(256), 0, (hipStream_t)stream, X, n, w_bf16, w_f32, y, x16);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     g_mlp_err = hipGetErrorString(e);
