@@ -623,6 +623,12 @@ __device__ __forceinline__ T pub_load(const T* p) {
   return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// v with its two lowest mantissa bits replaced by tag & 3 (a publication tag)
+__device__ __forceinline__ double tagged(double v, int tag) {
+  const long long b = __double_as_longlong(v);
+  return __longlong_as_double((b & ~3LL) | (long long)(tag & 3));
+}
+
 // v + v of the lane that DPP control CTRL selects within each row of 16 lanes
 template <int CTRL>
 __device__ __forceinline__ double dpp_add(double v) {
@@ -653,20 +659,33 @@ __device__ __forceinline__ double wave_sum(double v) {
   return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 
+// workgroup state of the grid reductions (one set per kernel)
+struct ReduceLds {
+  double sm[kWaves][4];
+  double tot[kWaves][4];
+  int abort_flag;
+};
+__device__ __forceinline__ ReduceLds& reduce_lds() {
+  __shared__ ReduceLds l;
+  return l;
+}
+
+// First half of a grid reduction: the workgroup's partials go out.  With
+// barrier = true every wave first waits for its earlier stores to complete,
+// so that the reduction also orders them before every workgroup's later
+// reads; without it the reduction carries values only.
 template <int NV>
-__device__ __forceinline__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
+__device__ __forceinline__ void grid_post(GridSync& gs, const double (&v)[NV], bool barrier) {
   static_assert(NV >= 1 && NV <= 4, "at most 4 values per reduction");
-  __shared__ double sm[kWaves][4];
-  __shared__ double tot[kWaves][4];
-  __shared__ int abort_flag;
+  ReduceLds& L = reduce_lds();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) abort_flag = 0;
+  if (threadIdx.x == 0) L.abort_flag = 0;
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     double w = wave_sum(v[i]);
-    if (lane == 0) sm[wave][i] = w;
+    if (lane == 0) L.sm[wave][i] = w;
   }
-  wait_vmem();  // this wave's published data has reached the coherence point
+  if (barrier) wait_vmem();  // this wave's published data has reached the coherence point
   __syncthreads();
   gs.epoch += 1u;
   const uint32_t ep = gs.epoch;
@@ -681,14 +700,25 @@ __device__ __forceinline__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
   uint64_t* words = reinterpret_cast<uint64_t*>(gs.red) + gs.region;
   if (threadIdx.x < 2 * NV) {
     const int i = threadIdx.x >> 1;
-    double acc = sm[0][i];
+    double acc = L.sm[0][i];
 #pragma unroll
-    for (int w = 1; w < kWaves; ++w) acc += sm[w][i];
+    for (int w = 1; w < kWaves; ++w) acc += L.sm[w][i];
     const uint64_t bits = (uint64_t)__double_as_longlong(acc);
     const uint32_t half = (threadIdx.x & 1) ? (uint32_t)(bits >> 32) : (uint32_t)bits;
     pub_store(&words[((size_t)slot * MFEM_RED_MAX_BLOCKS + blockIdx.x) * 8 + threadIdx.x],
               ((uint64_t)ep << 32) | half);
   }
+}
+
+// Second half: wait for every workgroup's partials of the last grid_post and
+// sum them.  Work placed between the two halves overlaps the round trip.
+template <int NV>
+__device__ __forceinline__ bool grid_wait(GridSync& gs, double (&v)[NV]) {
+  ReduceLds& L = reduce_lds();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t ep = gs.epoch;
+  const int slot = ep & 1u;
+  const uint64_t* words = reinterpret_cast<const uint64_t*>(gs.red) + gs.region;
   // Reading: eight lanes per partial, lane j loading word j, so that one
   // load instruction fetches eight partials' lines whole (one request per
   // line instead of one per word: every workgroup polls every partial, and
@@ -745,7 +775,7 @@ __device__ __forceinline__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
   }
   if (to) {
     atomicOr(&gs.status[MFEM_STATUS_ERRBITS], 1);
-    abort_flag = 1;
+    L.abort_flag = 1;
   }
   // per wave: value i sits in lanes 2 i + 8 g (g = 0..7); pairs of groups
   // one row apart through DPP (row_ror 8), then the four rows read out in
@@ -755,25 +785,31 @@ __device__ __forceinline__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
   for (int i = 0; i < NV; ++i) {
     const double w = (lane_value(a, 2 * i) + lane_value(a, 16 + 2 * i)) +
                      (lane_value(a, 32 + 2 * i) + lane_value(a, 48 + 2 * i));
-    if (lane == 0) tot[wave][i] = w;
+    if (lane == 0) L.tot[wave][i] = w;
   }
   __syncthreads();
-  if (abort_flag) {
+  if (L.abort_flag) {
     gs.abort = true;
     return false;
   }
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
-    double t = tot[0][i];
+    double t = L.tot[0][i];
 #pragma unroll
-    for (int w = 1; w < kWaves; ++w) t += tot[w][i];
+    for (int w = 1; w < kWaves; ++w) t += L.tot[w][i];
     v[i] = lane_value(t, 0);  // (uniform: scalar registers)
   }
-  // (no closing barrier: the next call writes tot only after its first
+  // (no closing barrier: the next reduction writes tot only after its first
   // barrier, which every thread reaches after these reads; before it, it
   // writes sm, read here by no one, and clears abort_flag, which is clear
   // already, since a set flag ends the solve)
   return true;
+}
+
+template <int NV>
+__device__ __forceinline__ bool grid_reduce(GridSync& gs, double (&v)[NV]) {
+  grid_post(gs, v, true);
+  return grid_wait(gs, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -965,6 +1001,10 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
   static_assert(!LOWREG || (CPW == 1 && RW == 0 && MODE == 0), "LOWREG: standard, streamed, one chunk per wave");
   constexpr bool PIPE = MODE != 0;
   constexpr bool PUB32 = MODE == 2;  // pipelined: publish the SpMV input in FP32
+  // pipelined with resident rows: the FP64 SpMV input is staged in LDS, and
+  // every FP64 publication carries its publication count in the two lowest
+  // mantissa bits (a relative change below 1e-15)
+  constexpr bool HALO = PIPE && !PUB32 && RW > 0;
   // a barrier timed out earlier: flags and epoch are inconsistent until the
   // host clears status and flags, so do nothing
   if (S.status[MFEM_STATUS_ERRBITS] != 0) return;
@@ -995,6 +1035,11 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
   int pub = 0, pubx = 0;  // publish counters: buffer = count & 1
   auto pzbuf = [&](int k) { return S.pz + (size_t)(k & 1) * nrow_pad * 3; };
   auto pxbuf = [&](int k) { return S.px + (size_t)(k & 1) * nrow_pad * 3; };
+  // FP64 publication number k (buffer k & 1): the value with tag k & 3
+  auto tagx = [&](double v, int k) -> double {
+    if constexpr (HALO) return tagged(v, k);
+    else return v;
+  };
 
   double bb = 0.0;
   double* px0 = pxbuf(pubx);
@@ -1024,7 +1069,13 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
     }
     if (own[k]) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) pub_store(&px0[3 * row + a], x[k][a]);
+      for (int a = 0; a < 3; ++a) {
+        pub_store(&px0[3 * row + a], tagx(x[k][a], 0));
+        // the other buffer as publication -1 (tag 3): whatever an earlier
+        // launch left there cannot carry the tag of this launch's first
+        // publication into it
+        if constexpr (HALO) pub_store(&pxbuf(1)[3 * row + a], tagx(x[k][a], 3));
+      }
     }
   }
   ++pubx;
@@ -1064,9 +1115,109 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
     tk[k] += tnow - tprev;                \
     tprev = tnow;                         \
   }
-  double red1[1] = {bb};
+  // Pipelined, resident rows: the FP64 SpMV input is staged per workgroup.
+  // The workgroup's columns touch a few hundred distinct 64-byte lines of the
+  // published vector (each 24-byte row spans one or two); gathering them lane
+  // by lane costs ~25 line lookups per wave instruction, 48 instructions per
+  // wave.  Instead the lines are copied once per SpMV into LDS with
+  // coalesced direct-to-LDS loads (16 bytes a lane, device scope) and the
+  // gathers read LDS.  Set-up, once per launch: mark the lines in an LDS
+  // bitmap, count them (exclusive prefix over the bitmap words), list them in
+  // ascending order (a row that spans two lines finds them adjacent) and turn
+  // each resident slot's column into its LDS offset.  Slot 0 is a line of
+  // zeros for the padding slots.  A workgroup with more lines than fit
+  // streams its rows from memory instead.
+  constexpr int kHaloQ = kHaloLines * 4 / kBlock;  // 16-byte quarters per thread
+  __shared__ double hv[HALO ? kHaloLines * 8 : 1];
+  // hl[1 + j]: line of staged slot j + 1, -1 past the last one
+  __shared__ int hl[HALO ? kHaloLines + 1 : 1];
+  __shared__ int hcount;
+  bool staged = false;
+  // per-wave transpose slice for the coalesced publication of m
+  __shared__ double tr[(PIPE && !PUB32) ? kWaves * 192 : 1];
+  double* trw = tr + ((PIPE && !PUB32) ? (threadIdx.x >> 6) * 192 : 0);
+  if constexpr (HALO) {
+    uint32_t* bm = reinterpret_cast<uint32_t*>(hv);
+    const long long n_lines = (long long)nrow_pad * 3 / 8;
+    const int nwords = (int)((n_lines + 31) >> 5);
+    // bitmap and prefix share the line buffer (dead once the offsets are set)
+    if (2LL * nwords <= (long long)kHaloLines * 16) {
+      uint32_t* pre = bm + nwords;
+      const int c0 = own[0] ? S.chunk_ptr[chunk[0]] : 0;
+      const int wd = own[0] ? S.chunk_ptr[chunk[0] + 1] - c0 : 0;
+      for (int i = threadIdx.x; i < nwords; i += kBlock) bm[i] = 0u;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < RW; ++k) {
+        if (k < wd) {
+          const long long b0 = 24LL * rb.col[k];
+          const int l0 = (int)(b0 >> 6), l1 = (int)((b0 + 23) >> 6);
+          atomicOr(&bm[l0 >> 5], 1u << (l0 & 31));
+          if (l1 != l0) atomicOr(&bm[l1 >> 5], 1u << (l1 & 31));
+        }
+      }
+      __syncthreads();
+      // exclusive prefix of the words' bit counts: contiguous word segments
+      // per thread, then a scan of the 256 segment totals
+      const int seg = (nwords + kBlock - 1) / kBlock;
+      const int w0 = min((int)threadIdx.x * seg, nwords), w1 = min(w0 + seg, nwords);
+      int mine = 0;
+      for (int i = w0; i < w1; ++i) mine += __popc(bm[i]);
+      int incl = mine;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+      }
+      if (lane == 63) hl[threadIdx.x >> 6] = incl;  // wave totals (hl reused)
+      __syncthreads();
+      int base = 0;
+      for (int wv = 0; wv < (int)(threadIdx.x >> 6); ++wv) base += hl[wv];
+      if (threadIdx.x == kBlock - 1) hcount = base + incl;
+      __syncthreads();
+      const int H = hcount;
+      staged = H <= kHaloLines - 1;
+      if (staged) {
+        for (int i = 1 + H + (int)threadIdx.x; i <= kHaloLines; i += kBlock) hl[i] = -1;
+        int run = base + incl - mine;
+        for (int i = w0; i < w1; ++i) {
+          pre[i] = (uint32_t)run;
+          uint32_t bits = bm[i];
+          while (bits) {
+            const int j = __ffs(bits) - 1;
+            bits &= bits - 1u;
+            hl[1 + run++] = i * 32 + j;
+          }
+        }
+      }
+      __syncthreads();
+      if (staged) {
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+          int off = 0;
+          if (k < wd) {
+            const int c = rb.col[k];
+            const int l0 = (int)((24LL * c) >> 6);
+            const uint32_t wbits = bm[l0 >> 5] & ((1u << (l0 & 31)) - 1u);
+            off = (1 + (int)pre[l0 >> 5] + __popc(wbits)) * 8 + ((3 * c) & 7);
+          }
+          rb.col[k] = off;
+        }
+      }
+      __syncthreads();
+      if (staged && threadIdx.x < 8) hv[threadIdx.x] = 0.0;
+      // (the first staging comes after a grid reduction, whose workgroup
+      // barrier orders these writes before it)
+    }
+  }
+  // (a workgroup with too many lines for its buffer streams its SpMV input
+  // and needs the barrier before every SpMV: counted here)
+  double red1[2] = {bb, (HALO && !staged && threadIdx.x == 0) ? 1.0 : 0.0};
   if (!grid_reduce(gs, red1)) return;
   const double bnorm2 = red1[0];
+  // pipelined, every workgroup staged: the main loop's SpMV overlaps its
+  // grid reduction and checks the publication tags of what it stages
+  const bool ovl = HALO && red1[1] == 0.0;
   const double tol2 = (double)S.cg_tol * (double)S.cg_tol * bnorm2;
   // the recurrence runs to a tighter residual so that the FP64 re-check
   // normally passes without a restart: 0.5 tol with FP32 directions
@@ -1184,102 +1335,6 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
     // workgroup barrier ends this call's reads)
   };
 
-  // Pipelined, resident rows: the FP64 SpMV input is staged per workgroup.
-  // The workgroup's columns touch a few hundred distinct 64-byte lines of the
-  // published vector (each 24-byte row spans one or two); gathering them lane
-  // by lane costs ~25 line lookups per wave instruction, 48 instructions per
-  // wave.  Instead the lines are copied once per SpMV into LDS with
-  // coalesced direct-to-LDS loads (16 bytes a lane, device scope) and the
-  // gathers read LDS.  Set-up, once per launch: mark the lines in an LDS
-  // bitmap, count them (exclusive prefix over the bitmap words), list them in
-  // ascending order (a row that spans two lines finds them adjacent) and turn
-  // each resident slot's column into its LDS offset.  Slot 0 is a line of
-  // zeros for the padding slots.  A workgroup with more lines than fit
-  // streams its rows from memory instead.
-  constexpr bool HALO = PIPE && !PUB32 && RW > 0;
-  constexpr int kHaloQ = kHaloLines * 4 / kBlock;  // 16-byte quarters per thread
-  __shared__ double hv[HALO ? kHaloLines * 8 : 1];
-  // hl[1 + j]: line of staged slot j + 1, -1 past the last one
-  __shared__ int hl[HALO ? kHaloLines + 1 : 1];
-  __shared__ int hcount;
-  bool staged = false;
-  // per-wave transpose slice for the coalesced publication of m
-  __shared__ double tr[(PIPE && !PUB32) ? kWaves * 192 : 1];
-  double* trw = tr + ((PIPE && !PUB32) ? (threadIdx.x >> 6) * 192 : 0);
-  if constexpr (HALO) {
-    uint32_t* bm = reinterpret_cast<uint32_t*>(hv);
-    const long long n_lines = (long long)nrow_pad * 3 / 8;
-    const int nwords = (int)((n_lines + 31) >> 5);
-    // bitmap and prefix share the line buffer (dead once the offsets are set)
-    if (2LL * nwords <= (long long)kHaloLines * 16) {
-      uint32_t* pre = bm + nwords;
-      const int c0 = own[0] ? S.chunk_ptr[chunk[0]] : 0;
-      const int wd = own[0] ? S.chunk_ptr[chunk[0] + 1] - c0 : 0;
-      for (int i = threadIdx.x; i < nwords; i += kBlock) bm[i] = 0u;
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < RW; ++k) {
-        if (k < wd) {
-          const long long b0 = 24LL * rb.col[k];
-          const int l0 = (int)(b0 >> 6), l1 = (int)((b0 + 23) >> 6);
-          atomicOr(&bm[l0 >> 5], 1u << (l0 & 31));
-          if (l1 != l0) atomicOr(&bm[l1 >> 5], 1u << (l1 & 31));
-        }
-      }
-      __syncthreads();
-      // exclusive prefix of the words' bit counts: contiguous word segments
-      // per thread, then a scan of the 256 segment totals
-      const int seg = (nwords + kBlock - 1) / kBlock;
-      const int w0 = min((int)threadIdx.x * seg, nwords), w1 = min(w0 + seg, nwords);
-      int mine = 0;
-      for (int i = w0; i < w1; ++i) mine += __popc(bm[i]);
-      int incl = mine;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int t = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += t;
-      }
-      if (lane == 63) hl[threadIdx.x >> 6] = incl;  // wave totals (hl reused)
-      __syncthreads();
-      int base = 0;
-      for (int wv = 0; wv < (int)(threadIdx.x >> 6); ++wv) base += hl[wv];
-      if (threadIdx.x == kBlock - 1) hcount = base + incl;
-      __syncthreads();
-      const int H = hcount;
-      staged = H <= kHaloLines - 1;
-      if (staged) {
-        for (int i = 1 + H + (int)threadIdx.x; i <= kHaloLines; i += kBlock) hl[i] = -1;
-        int run = base + incl - mine;
-        for (int i = w0; i < w1; ++i) {
-          pre[i] = (uint32_t)run;
-          uint32_t bits = bm[i];
-          while (bits) {
-            const int j = __ffs(bits) - 1;
-            bits &= bits - 1u;
-            hl[1 + run++] = i * 32 + j;
-          }
-        }
-      }
-      __syncthreads();
-      if (staged) {
-#pragma unroll
-        for (int k = 0; k < RW; ++k) {
-          int off = 0;
-          if (k < wd) {
-            const int c = rb.col[k];
-            const int l0 = (int)((24LL * c) >> 6);
-            const uint32_t wbits = bm[l0 >> 5] & ((1u << (l0 & 31)) - 1u);
-            off = (1 + (int)pre[l0 >> 5] + __popc(wbits)) * 8 + ((3 * c) & 7);
-          }
-          rb.col[k] = off;
-        }
-      }
-      __syncthreads();
-      if (staged && threadIdx.x < 8) hv[threadIdx.x] = 0.0;
-      // (the first staging comes after a grid reduction, whose workgroup
-      // barrier orders these writes before it)
-    }
-  }
   // y = H vec on the resident rows, FP64 products (pipelined variant)
   auto spmv_resident64 = [&](const double* vec, double (&y)[CPW][3]) {
     if constexpr (HALO) {
@@ -1306,6 +1361,75 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       }
     } else {
       spmv_rows64<CPW, RW, true>(S, chunk, own, lane, rb, vec, y);
+    }
+  };
+  // The main loop's SpMV without a barrier before it: vec is publication
+  // number tag (mod 4).  The lines are staged at once; each lane then checks
+  // the tags of the two values of each quarter it loaded and the wave loads
+  // again every quarter that some lane found stale (a neighbour's stores
+  // still in flight), until all carry the tag.  A buffer is rewritten two
+  // publications later, after a grid reduction that every workgroup enters
+  // only after its SpMV of this one, so a stale value carries tag - 2.
+  // The wait is bounded like a grid reduction's.
+  __shared__ int stage_abort;
+  if (threadIdx.x == 0) stage_abort = 0;  // (ordered by the next workgroup barrier)
+  auto spmv_tagged = [&](const double* vec, int tag, double (&y)[CPW][3]) -> bool {
+    if constexpr (HALO) {
+      int hline[kHaloQ];
+#pragma unroll
+      for (int i = 0; i < kHaloQ; ++i) hline[i] = hl[1 + ((threadIdx.x + kBlock * i) >> 2)];
+      auto load_q = [&](int i) {
+        const int q = threadIdx.x + kBlock * i;
+        __builtin_amdgcn_global_load_lds(vec + (size_t)hline[i] * 8 + (q & 3) * 2,
+                                         (__attribute__((address_space(3))) void*)&hv[8 + (q & ~63) * 2], 16, 0,
+                                         kCachePolicyDevice);
+      };
+#pragma unroll
+      for (int i = 0; i < kHaloQ; ++i)
+        if (hline[i] >= 0) load_q(i);
+      wait_vmem();
+      // bit i: quarter i holds a value without the tag
+      auto stale_bits = [&]() {
+        uint32_t st = 0;
+#pragma unroll
+        for (int i = 0; i < kHaloQ; ++i) {
+          if (hline[i] >= 0) {
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(&hv[8 + 2 * (threadIdx.x + kBlock * i)]);
+            if (((w[0] ^ (uint32_t)tag) | (w[2] ^ (uint32_t)tag)) & 3u) st |= 1u << i;
+          }
+        }
+        return st;
+      };
+      uint32_t st = stale_bits();
+      bool to = false;
+      if (__ballot(st != 0)) {
+        const uint64_t t0 = wall_clock64();
+        do {
+          const uint64_t dt = wall_clock64() - t0;
+          if (dt > kStatusCheckAfter &&
+              (dt > kBarrierTimeout || pub_load(&S.status[MFEM_STATUS_ERRBITS]) != 0)) {
+            to = true;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+          for (int i = 0; i < kHaloQ; ++i)
+            if (__ballot((st >> i) & 1u))  // (uniform: every lane of the quarter loads again)
+              if (hline[i] >= 0) load_q(i);
+          wait_vmem();
+          st = stale_bits();
+        } while (__ballot(st != 0));
+      }
+      if (to) {
+        atomicOr(&S.status[MFEM_STATUS_ERRBITS], 1);
+        stage_abort = 1;
+      }
+      __syncthreads();
+      if (stage_abort) return false;
+      spmv_row_lds<RW>(rb.val, rb.col, hv, y[0]);
+      return true;
+    } else {
+      return false;
     }
   };
   // y = H vec in FP64 for the starts and the true-residual checks: staged
@@ -1353,7 +1477,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         apply_M<CPW, double>(M, k, r[k], u[k]);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          pub_store(&ub[3 * row + a], u[k][a]);
+          pub_store(&ub[3 * row + a], tagx(u[k][a], pubx));
           rr += r[k][a] * r[k][a];
         }
       }
@@ -1457,7 +1581,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
           if (!own[k]) continue;
           const int row = chunk[k] * 64 + lane;
 #pragma unroll
-          for (int a = 0; a < 3; ++a) pub_store(&gb[3 * row + a], cget(1, k, a));
+          for (int a = 0; a < 3; ++a) pub_store(&gb[3 * row + a], tagx(cget(1, k, a), pubx));
         }
         ++pubx;
       }
@@ -1579,7 +1703,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         if (own[k]) {
           const int row = chunk[k] * 64 + lane;
 #pragma unroll
-          for (int a = 0; a < 3; ++a) pub_store(&xb[3 * row + a], x[k][a]);
+          for (int a = 0; a < 3; ++a) pub_store(&xb[3 * row + a], tagx(x[k][a], pubx));
         }
       }
       ++pubx;
@@ -1616,6 +1740,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
     PCG_LAP(5);
     for (;;) {
       double gamma = 0.0, delta = 0.0;
+      double n[CPW][3];  // pipelined: H m
       if constexpr (PIPE) {
         // gamma = r.u, delta = w.u, rr = r.r; m = M w published with them
         rr = 0.0;
@@ -1644,7 +1769,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
             for (int a = 0; a < 3; ++a) trw[3 * lane + a] = m[a];
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int j = 0; j < 3; ++j) pub_store(&mb[(size_t)chunk[k] * 192 + lane + 64 * j], trw[lane + 64 * j]);
+            for (int j = 0; j < 3; ++j) pub_store(&mb[(size_t)chunk[k] * 192 + lane + 64 * j], tagx(trw[lane + 64 * j], pubx));
             __builtin_amdgcn_wave_barrier();
           }
           // u = M r: held by recurrence in the FP32-published form, else
@@ -1669,7 +1794,13 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         pre = false;
         PCG_LAP(2);
         double red[3] = {gamma, delta, rr};
-        if (!grid_reduce(gs, red)) return;
+        if (ovl) {
+          // n = H m runs inside the reduction's round trip
+          grid_post(gs, red, false);
+          if (!spmv_tagged(pxbuf(pubx - 1), (pubx - 1) & 3, n)) return;
+          PCG_LAP(0);
+          if (!grid_wait(gs, red)) return;
+        } else if (!grid_reduce(gs, red)) return;
         gamma = red[0];
         delta = red[1];
         rr = red[2];
@@ -1683,7 +1814,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
           if (!own[k]) continue;
           const int row = chunk[k] * 64 + lane;
 #pragma unroll
-          for (int a = 0; a < 3; ++a) pub_store(&xb[3 * row + a], x[k][a]);
+          for (int a = 0; a < 3; ++a) pub_store(&xb[3 * row + a], tagx(x[k][a], pubx));
         }
         ++pubx;
         double red0[1] = {0.0};
@@ -1721,8 +1852,8 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         continue;
       }
       if constexpr (PIPE) {
-        // n = H m; scalars; all vector recurrences local (FP64)
-        double n[CPW][3];
+        // n = H m (unless overlapped above); scalars; all vector recurrences
+        // local (FP64)
         if constexpr (PUB32) {
           float y[CPW][3];
           spmv_rows<CPW, RW>(S, chunk, own, lane, rb, pzbuf(pub - 1), y);
@@ -1730,10 +1861,10 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
           for (int k = 0; k < CPW; ++k)
 #pragma unroll
             for (int a = 0; a < 3; ++a) n[k][a] = (double)y[k][a];
-        } else {
+        } else if (!ovl) {
           spmv_resident64(pxbuf(pubx - 1), n);
         }
-        PCG_LAP(0);
+        if (!ovl) PCG_LAP(0);
         double alpha;
         if (first) {
           beta = 0.0;
@@ -1766,7 +1897,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
             for (int a = 0; a < 3; ++a) trw[3 * lane + a] = mn[a];
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int j = 0; j < 3; ++j) pub_store(&mb[(size_t)chunk[k] * 192 + lane + 64 * j], trw[lane + 64 * j]);
+            for (int j = 0; j < 3; ++j) pub_store(&mb[(size_t)chunk[k] * 192 + lane + 64 * j], tagx(trw[lane + 64 * j], pubx));
             __builtin_amdgcn_wave_barrier();
             // u = M r before and after the r update (no q recurrence)
             apply_M<CPW, double>(M, k, r[k], ul);
