@@ -153,7 +153,9 @@ typedef struct mfem_solver {
   float* ds;    /* [n_tets][6] stretch update */
   /* persistent-kernel scratch */
   float* pz;    /* [2][n_chunks*64][3] published PCG vector (double-buffered) */
-  double* px;   /* [2][n_chunks*64][3] published FP64 PCG vector (residual checks, pipelined variant) */
+  double* px;   /* [2][n_chunks*64][3] published FP64 PCG vector (residual checks, pipelined variant;
+                   scratch owned by the solve: its values carry publication tags in the two lowest
+                   mantissa bits) */
   double* red;  /* [2 kernels][2][MFEM_RED_MAX_BLOCKS][8] 64-bit words: epoch-tagged halves of the grid
                    reduction partials (PCG region, line-search region) */
   uint32_t* flags; /* [MFEM_RED_MAX_BLOCKS] grid barrier arrival epochs */
