@@ -1388,15 +1388,23 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       for (int i = 0; i < kHaloQ; ++i)
         if (hline[i] >= 0) load_q(i);
       wait_vmem();
-      // bit i: quarter i holds a value without the tag
+      // bit i: quarter i holds a value without the tag.  All quarters are
+      // read, without branches, so that the LDS reads go out together (a
+      // quarter past the staged lines reads quarter 0 and is ignored).
       auto stale_bits = [&]() {
+        uint32_t lo[kHaloQ][2];
+#pragma unroll
+        for (int i = 0; i < kHaloQ; ++i) {
+          const int q = hline[i] >= 0 ? (int)threadIdx.x + kBlock * i : 0;
+          const uint32_t* w = reinterpret_cast<const uint32_t*>(&hv[8 + 2 * q]);
+          lo[i][0] = w[0];
+          lo[i][1] = w[2];
+        }
         uint32_t st = 0;
 #pragma unroll
         for (int i = 0; i < kHaloQ; ++i) {
-          if (hline[i] >= 0) {
-            const uint32_t* w = reinterpret_cast<const uint32_t*>(&hv[8 + 2 * (threadIdx.x + kBlock * i)]);
-            if (((w[0] ^ (uint32_t)tag) | (w[2] ^ (uint32_t)tag)) & 3u) st |= 1u << i;
-          }
+          const uint32_t bad = (((lo[i][0] ^ (uint32_t)tag) | (lo[i][1] ^ (uint32_t)tag)) & 3u) != 0u;
+          st |= (bad & (uint32_t)(hline[i] >= 0)) << i;
         }
         return st;
       };
