@@ -1740,9 +1740,14 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
     // and overlaps the rest of its work with their completion)
     bool pre = false;
     double gamma_pre = 0.0, delta_pre = 0.0, rr_pre = 0.0;
-    // pipelined: the previous gamma and alpha, one copy per wave in LDS
-    // (wave-uniform; held in registers they are spilled to scratch, whose
-    // reload would sit on the critical path after each reduction)
+    // pipelined: 1 / gamma and gamma * alpha of the previous iteration, one
+    // copy per wave in LDS (wave-uniform; held in registers they are spilled
+    // to scratch, whose reload would sit on the critical path after each
+    // reduction).  With them
+    //   beta = gamma / gamma_prev = gamma * (1 / gamma_prev),
+    //   alpha = gamma / (delta - beta gamma / alpha_prev)
+    //         = gamma P / (delta P - gamma^2),  P = gamma_prev alpha_prev,
+    // one division after the reduction instead of three in sequence.
     __shared__ double carried[kWaves][2];
     const int cw = threadIdx.x >> 6;
     PCG_LAP(5);
@@ -1878,13 +1883,17 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
           beta = 0.0;
           alpha = gamma / delta;
         } else {
-          beta = gamma / carried[cw][0];
-          alpha = gamma / (delta - beta * gamma / carried[cw][1]);
+          const double P = carried[cw][1];
+          beta = gamma * carried[cw][0];
+          alpha = gamma * P / (delta * P - gamma * gamma);
         }
         if (!(alpha > 0.0) || !isfinite(alpha)) {  // breakdown: fall to the residual check
           it = S.cg_maxiter;
           continue;
         }
+        // the next iteration's carried scalars, computed here so that the
+        // division overlaps the vector updates
+        const double ig_next = 1.0 / gamma, P_next = gamma * alpha;
         if constexpr (!PUB32) {
           // w first; the next m = M w goes out at once, and the remaining
           // recurrences and the next dot products run while it drains
@@ -1950,8 +1959,8 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         }
         PCG_LAP(2);
         if (lane == 0) {
-          carried[cw][0] = gamma;
-          carried[cw][1] = alpha;
+          carried[cw][0] = ig_next;
+          carried[cw][1] = P_next;
         }
         first = false;
         ++it;
