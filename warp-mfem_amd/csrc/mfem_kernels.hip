@@ -2502,6 +2502,9 @@ const PcgVariant kPcgVariants[] = {
     PCG_V(1, 16, 0), PCG_V(1, 0, 0), {1, 0, 2, reinterpret_cast<const void*>(&k_pcg_lowreg)},
     PCG_V(2, 0, 0), PCG_V(4, 0, 0), PCG_V(8, 0, 0),
     PCG_V(1, 16, 1), PCG_V(1, 0, 1), PCG_V(2, 0, 1), PCG_V(4, 0, 1), PCG_V(8, 0, 1),
+    // rows of at most 15 blocks (valence <= 14 in a tetrahedral mesh, the
+    // common case): no padding slot in every row's SpMV
+    PCG_V(1, 15, 1),
 };
 #undef PCG_V
 constexpr int kNumPcgVariants = sizeof(kPcgVariants) / sizeof(kPcgVariants[0]);
