@@ -501,8 +501,19 @@ __device__ __forceinline__ size_t val_index(int lane_slot, int q) {
   return ((size_t)(lane_slot >> 6) * 9 + q) * 64 + (lane_slot & 63);
 }
 
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) round-robin
+// by workgroup index.  This bijection of [0, gridDim.x) gives each XCD one
+// contiguous range of blocks instead of every eighth block, so that gathers
+// from neighbouring rows (records of shared tets) meet in the same L2.
+__device__ __forceinline__ int xcd_block() {
+  constexpr int kXcds = 8;
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds, k = b / kXcds;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 __global__ void __launch_bounds__(kBlock) k_assemble(mfem_solver S) {
-  int l = blockIdx.x * kBlock + threadIdx.x;
+  int l = xcd_block() * kBlock + threadIdx.x;
   if (l >= S.n_slots * 64) return;
   float acc[9];
 #pragma unroll
@@ -546,7 +557,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble(mfem_solver S) {
 
 // g = (H_l G)^T g_l - g_x - grad B - grad D; diagonal += (M + Q)/dt^2 + H_B + H_D
 __global__ void __launch_bounds__(kBlock) k_rhs(mfem_solver S, float dt) {
-  int i = blockIdx.x * kBlock + threadIdx.x;
+  int i = xcd_block() * kBlock + threadIdx.x;
   if (i >= S.n_verts) return;
   float g[3];
 #pragma unroll
