@@ -967,7 +967,14 @@ __device__ __forceinline__ void spmv_row_lds(const float (&val)[RW][9], const in
     for (int k = kb; k < kb + GRP && k < RW; ++k) {
       const double* g = hv + off[k];
       const double g0 = g[0], g1 = g[1], g2 = g[2];
-      const float* v = val[k];
+      // (opaque per use: the FP32 values are converted here, not hoisted out
+      // of the solve loop as FP64 copies that overflow into AGPRs)
+      float v[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        v[q] = val[k][q];
+        asm volatile("" : "+v"(v[q]));
+      }
       a0 += (double)v[0] * g0 + (double)v[1] * g1 + (double)v[2] * g2;
       a1 += (double)v[3] * g0 + (double)v[4] * g1 + (double)v[5] * g2;
       a2 += (double)v[6] * g0 + (double)v[7] * g1 + (double)v[8] * g2;
