@@ -2680,7 +2680,18 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
     sel = S->pcg_variant;
     nblocks = (unsigned)info[3];
   }
-  for (int k = 0; k < kNumPcgVariants && sel < 0; ++k) {
+  for (int k0 = 0; k0 < kNumPcgVariants && sel < 0; ++k0) {
+    // resident rows: the narrowest form of this kind that holds the widest
+    // chunk stands in for the wider ones
+    int k = k0;
+    if (kPcgVariants[k0].rw >= S->max_width) {
+      for (int j = 0; j < kNumPcgVariants; ++j) {
+        const PcgVariant& u = kPcgVariants[j];
+        if (u.mode == kPcgVariants[k0].mode && u.cpw == kPcgVariants[k0].cpw && u.rw >= S->max_width &&
+            u.rw < kPcgVariants[k].rw)
+          k = j;
+      }
+    }
     const PcgVariant& v = kPcgVariants[k];
     const bool want = (S->pcg_flags & MFEM_PCG_PIPELINED) != 0;
     if ((v.mode == 1) != want) continue;
