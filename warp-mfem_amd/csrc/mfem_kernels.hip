@@ -2337,24 +2337,34 @@ __device__ double tri_merit(const mfem_solver& S, int t, float a) {
   return (double)(S.kappa * e) * ((double)wl[0] + (double)wl[1] + (double)wl[2]);
 }
 
-// (g_x + grad B + grad D - G^T lambda) . dx for vertex i
-__device__ double vertex_slope(const mfem_solver& S, int i) {
-  float gr[3];
+// (g_x + grad B + grad D) . dx for vertex i: the vertex part of the slope
+__device__ double vertex_slope_own(const mfem_solver& S, int i) {
+  double acc = 0.0;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) gr[k] = S.gx[3 * i + k] + S.gB[3 * i + k] + S.gD[3 * i + k];
-  const int k0 = S.vinc_ptr[i], k1 = S.vinc_ptr[i + 1];
-  for (int kk = k0; kk < k1; ++kk) {
-    int src = S.vinc_src[kk];
-    int e = src >> 2, a = src & 3;
+  for (int k = 0; k < 3; ++k)
+    acc += (double)(S.gx[3 * i + k] + S.gB[3 * i + k] + S.gD[3 * i + k]) * S.dx[3 * i + k];
+  return acc;
+}
+
+// -(G^T lambda)_e . dx over the tet's four vertices: the multiplier part of
+// the vertices' slopes, summed per tet (each G record read once, whole)
+__device__ double tet_lambda_slope(const mfem_solver& S, int e) {
+  const int4 tv = reinterpret_cast<const int4*>(S.tets)[e];
+  const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+  float lam[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) lam[q] = S.lam[6 * e + q];
+  double acc = 0.0;
+#pragma unroll 1
+  for (int a = 0; a < 4; ++a)  // (one vertex's 18 Jacobian entries at a time: registers)
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      float acc = 0.f;
+      float t = 0.f;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) acc += S.G[72 * e + q * 12 + 3 * a + r] * S.lam[6 * e + q];
-      gr[r] -= acc;
+      for (int q = 0; q < 6; ++q) t += S.G[72 * e + q * 12 + 3 * a + r] * lam[q];
+      acc -= (double)t * S.dx[3 * vid[a] + r];
     }
-  }
-  return (double)gr[0] * S.dx[3 * i] + (double)gr[1] * S.dx[3 * i + 1] + (double)gr[2] * S.dx[3 * i + 2];
+  return acc;
 }
 
 __device__ double tet_slope(const mfem_solver& S, int e) {
@@ -2387,12 +2397,12 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
   double f = 0.0, sl = 0.0, fa = 0.0;
   for (int i = v0 + tid; i < v1; i += nth) {
     f += vertex_merit(S, i, 0.f, dt);
-    sl += vertex_slope(S, i);
+    sl += vertex_slope_own(S, i);
     fa += vertex_merit(S, i, a_first, dt);
   }
   for (int e = e0 + tid; e < e1; e += nth) {
     f += tet_merit(S, e, 0.f);
-    sl += tet_slope(S, e);
+    sl += tet_slope(S, e) + tet_lambda_slope(S, e);
     fa += tet_merit(S, e, a_first);
   }
   if (tri_contact_on(S))
