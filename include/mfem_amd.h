@@ -58,8 +58,9 @@ extern "C" {
 #define MFEM_PCG_LEARNED_GUESS 4 /* start from the lowest-energy combination of x~ - x, the learned guess, the last three Newton steps and the previous step's same iteration (needs dx_hist) */
 
 /* status[] words */
-#define MFEM_STATUS_ERRBITS 0 /* bit 0: grid barrier timed out; the persistent kernels (PCG, line search)
-                                 then return at once until status and flags are cleared */
+#define MFEM_STATUS_ERRBITS 0 /* bit 0: grid barrier timed out; every kernel of a step (the persistent
+                                 PCG and line search included) then returns at once, leaving the state
+                                 untouched, until status and flags are cleared */
 #define MFEM_STATUS_EPOCH 1   /* grid-barrier epoch counter (internal) */
 
 /*
@@ -460,6 +461,15 @@ const char* mfem_pattern_last_error(void);
    distance, closest-point barycentric weights, normal and capsule id into
    S->tri_dist / tri_bary / tri_nrm / tri_sid.  At the bound state x. */
 int mfem_contact_distance(const mfem_solver* S, float* vdist, int32_t* vsid, void* stream);
+
+/* World poses of the rigid shapes for this step (contact.py:74-82,
+   query_min_signed_distance): shape_xf[s] = body_q[shape_body[s]] *
+   shape_local[s] (p(3), q(4) xyzw; rotation of the local pose applied
+   first) for a shape attached to body shape_body[s] < n_body, and
+   shape_local[s] for a static shape (shape_body[s] < 0).  All device
+   pointers. */
+int mfem_pose_shapes(const float* body_q, int n_body, const int32_t* shape_body, const float* shape_local,
+                     int n_shapes, float* shape_xf, void* stream);
 
 /* y = H x with the assembled SELL matrix (warp.sparse.bsr_mv). */
 int mfem_spmv(const mfem_solver* S, const float* x, float* y, void* stream);

@@ -156,6 +156,49 @@ class Kernels:
         self.lib.orc_barrier(real(d), real(d0), real(d1), self._p(q), self._p(out))
         return out
 
+    def _real_t(self):
+        return ctypes.c_double if self.bits == 64 else ctypes.c_float
+
+    def friction(self, x, x_start, n, lamn, mu, ev):
+        """Lagged friction of one vertex: (energy, gradient (3,), Hessian (3, 3))."""
+        x, xs, n = (self._arr(v, 3) for v in (x, x_start, n))
+        g, H = np.zeros(3, self.dt), np.zeros(9, self.dt)
+        real = self._real_t()
+        self.lib.orc_friction.restype = real
+        E = self.lib.orc_friction(self._p(x), self._p(xs), self._p(n), real(lamn), real(mu), real(ev), self._p(g),
+                                  self._p(H))
+        return float(E), g, H.reshape(3, 3)
+
+    def shape_sdf(self, shape_type, xf, scale, x):
+        """Signed distance of x to one shape posed by xf = p(3), q(4) (xyzw):
+        (d, gradient (3,), Hessian (3, 3))."""
+        xf, sc, x = self._arr(xf, 7), self._arr(scale, 3), self._arr(x, 3)
+        g, H = np.zeros(3, self.dt), np.zeros(9, self.dt)
+        self.lib.orc_shape_sdf.restype = self._real_t()
+        d = self.lib.orc_shape_sdf(ctypes.c_int(shape_type), self._p(xf), self._p(sc), self._p(x), self._p(g),
+                                   self._p(H))
+        return float(d), g, H.reshape(3, 3)
+
+    def tri_barrier(self, verts, shapes, d0, d1, kappa):
+        """One triangle against capsules ``shapes`` = [(type, xf(7), scale(3))]:
+        (energy, per-corner gradient (3, 3), Hessian blocks (3, 3, 3, 3),
+        distance, barycentric weights, normal)."""
+        v = self._arr(verts, 9)
+        types = np.ascontiguousarray([s[0] for s in shapes], dtype=np.int32)
+        xf = self._arr(np.concatenate([np.asarray(s[1], float) for s in shapes]))
+        sc = self._arr(np.concatenate([np.asarray(s[2], float) for s in shapes]))
+        q = self.barrier(d0, 0.0, d1)
+        q = self._arr(q, 3)
+        d = np.zeros(1, self.dt)
+        w, n, g, H = np.zeros(3, self.dt), np.zeros(3, self.dt), np.zeros(9, self.dt), np.zeros(81, self.dt)
+        real = self._real_t()
+        self.lib.orc_tri_barrier.restype = real
+        E = self.lib.orc_tri_barrier(self._p(v[0:3].copy()), self._p(v[3:6].copy()), self._p(v[6:9].copy()),
+                                     ctypes.c_int(len(shapes)), types.ctypes.data_as(ctypes.c_void_p), self._p(xf),
+                                     self._p(sc), real(d0), real(d1), real(kappa), self._p(q), self._p(d),
+                                     self._p(w), self._p(n), self._p(g), self._p(H))
+        return float(E), g.reshape(3, 3), H.reshape(3, 3, 3, 3), float(d[0]), w, n
+
     def closest_point_triangle_bary(self, x, a, b, c):
         x, a, b, c = (self._arr(v, 3) for v in (x, a, b, c))
         w = np.zeros(3, self.dt)
@@ -200,8 +243,9 @@ class Oracle:
             self.buf("tris")[:] = scene.tri_indices.reshape(-1)
         for k, sh in enumerate(scene.shapes):
             self.buf("shape_type")[k] = sh.type
-            self.buf("shape_xf")[7 * k:7 * k + 7] = np.concatenate([sh.pos, sh.quat])
             self.buf("shape_scale")[3 * k:3 * k + 3] = sh.scale
+        if nshapes:
+            self.set_body_q(getattr(scene, "body_q", None), scene)
         self.buf("x")[:] = scene.particle_q.reshape(-1)
         self.buf("v")[:] = scene.particle_qd.reshape(-1)
         p = dict(scene.params)
@@ -222,6 +266,27 @@ class Oracle:
         if self.lib.orc_finalize(self.ctx) != 0:
             raise ValueError("bad topology")
         self.lib.orc_init_stretch(self.ctx)
+
+    def set_body_q(self, body_q, scene=None) -> None:
+        """World poses of the shapes for rigid body poses body_q (n_bodies, 7):
+        body_q[shape.body] * (shape.pos, shape.quat), or the shape's own pose
+        for a static shape (src/mfem/refinement/contact.py:74-82)."""
+        scene = scene if scene is not None else self._scene
+        self._scene = scene
+        out = []
+        for sh in scene.shapes:
+            p, q = np.asarray(sh.pos, float), np.asarray(sh.quat, float)
+            b = getattr(sh, "body", -1)
+            if b >= 0:
+                t = np.asarray(body_q, float).reshape(-1, 7)[b]
+                u, w = t[3:6], t[6]
+                p = t[:3] + p + 2.0 * np.cross(u, np.cross(u, p) + w * p)
+                x1, y1, z1, w1 = t[3:]
+                x2, y2, z2, w2 = q
+                q = np.array([w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2, w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
+                              w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2, w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2])
+            out.append(np.concatenate([p, q]))
+        self.buf("shape_xf")[:] = np.concatenate(out)
 
     def __del__(self):
         try:

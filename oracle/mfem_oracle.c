@@ -17,7 +17,7 @@
  *                           src/mfem/refinement/utils.py:46-177
  *   ARAP / Neo-Hookean      src/mfem/refinement/elastic_energy.py:106-407,
  *                           src/mfem/utils.py:56-97
- *   GMW81 + PSD fix         src/mfem/numerical/gmw81.py:18-86, psd_fix.py:105-191
+ *   GMW81 + PSD fix         src/mfem/numerical/gmw81.py:18-86, psd_fix.py:24-105
  *   Schur blocks            src/mfem/refinement/kernels.py:189-249
  *   assembly                src/mfem/refinement/solver.py:641-799
  *   block Jacobi            src/mfem/refinement/kernels.py:32-53, solver.py:581-609
@@ -32,7 +32,9 @@
  *                           src/mfem/collision.py:276-322
  *
  * Functions with no check in tests/ that pins them are listed as
- * "parity unpinned" in the header comment of tests/test_oracle.py.
+ * "parity unpinned" in the header comment of tests/test_oracle.py; the pins
+ * of friction, shape distances, the triangle barrier, the local solve and
+ * the GMW81 correction are in tests/test_oracle_pins.py.
  */
 #include <math.h>
 #include <stdint.h>
@@ -455,6 +457,23 @@ static real friction_f1_over_y(real y, real ev) {
   if (y >= ev) return 1 / (y > (real)1e-12 ? y : (real)1e-12);
   return -y / (ev * ev) + 2 / ev;
 }
+/* lagged IPC friction of one vertex (contact.py:304-387): tangential slip
+   u_T = (I - n n^T)(x - x_start), E = mu lamn f0(|u_T|, ev), grad =
+   mu lamn (f1/y) u_T, hess = mu lamn (f1/y) (I - n n^T); ev = eps_v dt.
+   grad / hess may be null. */
+EXPORT real orc_friction(const real* x, const real* xs, const real* n, real lamn, real mu, real ev, real* grad,
+                         real* hess) {
+  real u[3] = {x[0] - xs[0], x[1] - xs[1], x[2] - xs[2]};
+  real un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+  real ut[3] = {u[0] - un * n[0], u[1] - un * n[1], u[2] - un * n[2]};
+  real y = (real)sqrt((double)(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2])), sc = mu * lamn;
+  real f1y = friction_f1_over_y(y, ev);
+  if (grad) for (int k = 0; k < 3; ++k) grad[k] = sc * f1y * ut[k];
+  if (hess)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) hess[a * 3 + b] = sc * f1y * ((a == b ? 1 : 0) - n[a] * n[b]);
+  return sc * friction_f0(y, ev);
+}
 /* quaternion (x, y, z, w) -> row-major rotation */
 static void quat_to_mat(const real* q, real* R) {
   real x = q[0], y = q[1], z = q[2], w = q[3];
@@ -809,17 +828,19 @@ static void vertex_distance(orc_ctx* C, const real* x, real* d, real* grad, real
   if (sid) *sid = im;
 }
 
-static void tri_capsule(orc_ctx* C, const real* a, const real* b, const real* c, real* dmin, real* bary, real* nrm, int* sid) {
+static void tri_capsule_shapes(int nshapes, const int* shape_type, const real* shape_xf, const real* shape_scale,
+                               const real* a, const real* b, const real* c, real* dmin, real* bary, real* nrm,
+                               int* sid) {
   *dmin = (real)1e8;
   bary[0] = bary[1] = bary[2] = (real)(1.0 / 3.0);
   nrm[0] = nrm[1] = nrm[2] = 0;
   *sid = -1;
-  for (int k = 0; k < C->nshapes; ++k) {
-    if (C->shape_type[k] != SHAPE_CAPSULE) continue;
-    const real* xf = C->shape_xf + 7 * k;
+  for (int k = 0; k < nshapes; ++k) {
+    if (shape_type[k] != SHAPE_CAPSULE) continue;
+    const real* xf = shape_xf + 7 * k;
     real R[9];
     quat_to_mat(xf + 3, R);
-    real hh = C->shape_scale[3 * k + 1], rad = C->shape_scale[3 * k + 0];
+    real hh = shape_scale[3 * k + 1], rad = shape_scale[3 * k + 0];
     real p0[3], p1[3];
     for (int i = 0; i < 3; ++i) { p0[i] = xf[i] - R[i * 3 + 2] * hh; p1[i] = xf[i] + R[i * 3 + 2] * hh; }
     real w[3], q[3];
@@ -850,6 +871,38 @@ static void tri_capsule(orc_ctx* C, const real* a, const real* b, const real* c,
       *sid = k;
     }
   }
+}
+
+static void tri_capsule(orc_ctx* C, const real* a, const real* b, const real* c, real* dmin, real* bary, real* nrm,
+                        int* sid) {
+  tri_capsule_shapes(C->nshapes, C->shape_type, C->shape_xf, C->shape_scale, a, b, c, dmin, bary, nrm, sid);
+}
+
+/* One surface triangle against capsules (contact.py:553-775): distance d of
+   the closest triangle point to the nearest capsule minus its radius, the
+   barycentric weights w and normal n of that point, energy kappa b(d),
+   gradient w_a kappa b'(d) n per corner (grad[3][3]) and the Gauss-Newton
+   Hessian blocks w_a w_b kappa max(b''(d), 0) n n^T (hess[3][3][3][3],
+   corner a, corner b, row, column); zero outside d < d1.  q = b(d0), b'(d0),
+   b''(d0) of the quadratic extension.  Returns the energy. */
+EXPORT real orc_tri_barrier(const real* a, const real* b, const real* c, int nshapes, const int* shape_type,
+                            const real* shape_xf, const real* shape_scale, real d0, real d1, real kappa,
+                            const real* q, real* d_out, real* bary, real* nrm, real* grad, real* hess) {
+  int sid;
+  real d;
+  tri_capsule_shapes(nshapes, shape_type, shape_xf, shape_scale, a, b, c, &d, bary, nrm, &sid);
+  *d_out = d;
+  real bb[3] = {0, 0, 0};
+  if (d < d1) orc_barrier(d, d0, d1, q, bb);
+  real h2 = bb[2] > 0 ? bb[2] : 0;
+  for (int i = 0; i < 3; ++i)
+    for (int r = 0; r < 3; ++r) grad[3 * i + r] = d < d1 ? bary[i] * kappa * bb[1] * nrm[r] : 0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int r = 0; r < 3; ++r)
+        for (int cc = 0; cc < 3; ++cc)
+          hess[((i * 3 + j) * 3 + r) * 3 + cc] = d < d1 ? bary[i] * bary[j] * kappa * h2 * nrm[r] * nrm[cc] : 0;
+  return d < d1 ? kappa * bb[0] : 0;
 }
 
 /* start of step (solver.py:499-520): x_tilde, friction anchor, lagged force */
@@ -893,17 +946,7 @@ static void friction_at(orc_ctx* C, int i, const real* x, real dt, real* E, real
   if (!(lamn > 0 && sid >= 0)) return;
   real mu = C->shape_mu[sid];
   if (!(mu > 0)) return;
-  const real* n = C->lag_n + 3 * i;
-  real u[3] = {x[0] - C->xstart[3 * i], x[1] - C->xstart[3 * i + 1], x[2] - C->xstart[3 * i + 2]};
-  real un = dot3(u, n);
-  real ut[3] = {u[0] - un * n[0], u[1] - un * n[1], u[2] - un * n[2]};
-  real y = norm3(ut), ev = C->p[P_EPSV] * dt, sc = mu * lamn;
-  real f1y = friction_f1_over_y(y, ev);
-  *E = sc * friction_f0(y, ev);
-  if (grad) for (int k = 0; k < 3; ++k) grad[k] = sc * f1y * ut[k];
-  if (hess)
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) hess[a * 3 + b] = sc * f1y * ((a == b ? 1 : 0) - n[a] * n[b]);
+  *E = orc_friction(x, C->xstart + 3 * i, C->lag_n + 3 * i, lamn, mu, C->p[P_EPSV] * dt, grad, hess);
 }
 
 static int pin_of(orc_ctx* C, int i) {

@@ -325,13 +325,12 @@ def test_moving_capsule_triangle_contact_matches_oracle(gpu):
     solver, state = _make(m, gpu)
     o = Oracle(m, 64)
     out = state.clone()
-    sh = m.shapes[0]
-    pose = np.concatenate([sh.pos, sh.quat]).astype(np.float64)
+    pose = m.body_q[0].astype(np.float64).copy()  # the capsule is rigid body 0
     min_tri = []
     for k in range(6):
         pose[2] -= 0.0015  # 1.5 mm per step into the slab's top face
         state.body_q = torch.as_tensor(pose[None, :], dtype=torch.float32, device=gpu)
-        o.buf("shape_xf")[:] = pose
+        o.set_body_q(pose[None, :])
         solver.step(state, out, None, None, dt)
         state, out = out, state
         o.step(dt)
@@ -346,7 +345,7 @@ def test_moving_capsule_triangle_contact_matches_oracle(gpu):
     # the same run without triangle contact ends elsewhere
     solver2, state2 = _make(m, gpu, tri_contact=False)
     out2 = state2.clone()
-    pose = np.concatenate([sh.pos, sh.quat]).astype(np.float64)
+    pose = m.body_q[0].astype(np.float64).copy()
     for k in range(6):
         pose[2] -= 0.0015
         state2.body_q = torch.as_tensor(pose[None, :], dtype=torch.float32, device=gpu)

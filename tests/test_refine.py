@@ -325,7 +325,7 @@ def test_solver_geometric_refinement_near_the_tool(gpu):
     nv0, nv = m.particle_count, solver.active_particle_count
     x = solver.rest_particle_q[:nv].double().cpu().numpy()
     cap = m.shapes[0]
-    axis_c = np.asarray(cap.pos, np.float64)
+    axis_c = np.asarray(m.body_q[0][:3], np.float64)  # the capsule is rigid body 0, at the body's origin
     d = np.hypot(x[nv0:, 1] - axis_c[1], x[nv0:, 2] - axis_c[2]) - cap.scale[0]  # capsule along world x
     edge = 0.3 / 6
     assert np.all(d <= float(kw["contact_d1"]) + edge), d.max()

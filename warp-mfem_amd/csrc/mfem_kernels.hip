@@ -70,9 +70,36 @@ __device__ __forceinline__ constexpr int sym21(int i, int j) {
 __device__ constexpr int PA[10] = {0, 1, 2, 3, 0, 0, 0, 1, 1, 2};
 __device__ constexpr int PB[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
 
+// Distances inside the barrier range are formed in FP64 (positions and poses
+// are FP32 inputs): in FP32 the subtraction of ~1 m coordinates leaves an
+// absolute error of ~1e-7 m, a relative error of ~1e-3 at d = 1e-4 m.  Shapes
+// are ranked in FP32; the winner's distance is recomputed in FP64 when the
+// FP32 value is below kFp64Range d1.
+static constexpr float kFp64Range = 2.0f;
+
+// FP64 signed distance of x to shape k (plane: local z; capsule: distance to
+// the axis segment minus the radius) (src/mfem/ipc/distance.py:4-57)
+__device__ double shape_distance64(const mfem_solver& S, int k, const float* x) {
+  const float* xf = S.shape_xf + 7 * k;
+  double R[9];
+  quat_to_mat(xf + 3, R);
+  const double dw[3] = {(double)x[0] - (double)xf[0], (double)x[1] - (double)xf[1], (double)x[2] - (double)xf[2]};
+  double xl[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) xl[i] = R[i] * dw[0] + R[3 + i] * dw[1] + R[6 + i] * dw[2];
+  if (S.shape_type[k] == MFEM_SHAPE_PLANE) return xl[2];
+  const double rad = S.shape_scale[3 * k], hh = S.shape_scale[3 * k + 1];
+  const double az = fmin(fmax(xl[2], -hh), hh);
+  const double df[3] = {xl[0], xl[1], xl[2] - az};
+  return sqrt(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]) - rad;
+}
+
 // minimum signed distance over the rigid shapes, world gradient, shape id
-// (src/mfem/refinement/contact.py:62-164, src/mfem/ipc/distance.py:4-57)
-__device__ float shape_distance(const mfem_solver& S, const float* x, float* grad, int* sid) {
+// (src/mfem/refinement/contact.py:62-164, src/mfem/ipc/distance.py:4-57);
+// d64 (may be null) receives the FP64 distance of the nearest shape (the FP32
+// value when it is outside kFp64Range d1)
+__device__ float shape_distance(const mfem_solver& S, const float* x, float* grad, int* sid,
+                                double* d64 = nullptr) {
   float dm = 1.0e8f;
   grad[0] = grad[1] = grad[2] = 0.f;
   *sid = -1;
@@ -106,6 +133,7 @@ __device__ float shape_distance(const mfem_solver& S, const float* x, float* gra
       *sid = k;
     }
   }
+  if (d64) *d64 = (*sid >= 0 && dm < kFp64Range * S.d1) ? shape_distance64(S, *sid, x) : (double)dm;
   return dm;
 }
 
@@ -140,58 +168,67 @@ __device__ float friction_eval(const mfem_solver& S, int i, const float* x, floa
   return sc * friction_f0(y, ev);
 }
 
-// distance of triangle abc to the nearest capsule (minus its radius), the
-// barycentric weights of the closest triangle point and the contact normal
-// (capsule -> triangle; the face normal oriented away from the axis midpoint
-// when the axis touches or pierces the face) (contact.py:553-681)
+// distance of triangle abc to capsule k minus its radius, the barycentric
+// weights of the closest triangle point and the contact normal (capsule ->
+// triangle; the face normal oriented away from the axis midpoint when the
+// axis touches or pierces the face) (contact.py:553-681), in precision T
+template <typename T>
+__device__ T tri_capsule_one(const mfem_solver& S, int k, const T* a, const T* b, const T* c, T* bary, T* nrm) {
+  const float* xf = S.shape_xf + 7 * k;
+  T R[9];
+  quat_to_mat(xf + 3, R);
+  const T rad = S.shape_scale[3 * k], hh = S.shape_scale[3 * k + 1];
+  T p0[3], p1[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    p0[i] = (T)xf[i] - R[i * 3 + 2] * hh;
+    p1[i] = (T)xf[i] + R[i * 3 + 2] * hh;
+  }
+  T q[3];
+  bool hit;
+  const T dist = closest_point_triangle_segment(a, b, c, p0, p1, bary, q, &hit);
+  T p[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) p[i] = bary[0] * a[i] + bary[1] * b[i] + bary[2] * c[i];
+  if (!hit && dist > T(1e-8)) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) nrm[i] = (p[i] - q[i]) / dist;
+  } else {
+    const T e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const T e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    cross3t(e1, e2, nrm);
+    const T nl = sqrt(dot3t(nrm, nrm));
+    if (nl > T(1e-20)) {
+      nrm[0] /= nl; nrm[1] /= nl; nrm[2] /= nl;
+    } else {
+      nrm[0] = 0; nrm[1] = 0; nrm[2] = 1;
+    }
+    const T ax[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const T pp0[3] = {p[0] - p0[0], p[1] - p0[1], p[2] - p0[2]};
+    const T aa = dot3t(ax, ax);
+    const T t = dot3t(pp0, ax) / (aa > T(1e-20) ? aa : T(1e-20));
+    const T sgn = t < T(0.5) ? T(-1) : T(1);
+    if (sgn * dot3t(nrm, ax) < T(0)) {
+      nrm[0] = -nrm[0]; nrm[1] = -nrm[1]; nrm[2] = -nrm[2];
+    }
+  }
+  return dist - rad;
+}
+
+// nearest capsule of triangle abc: ranked in FP32, the winner recomputed in
+// FP64 inside kFp64Range d1 (weights, normal, distance); d64 (may be null) =
+// that distance in FP64
 __device__ float tri_capsule(const mfem_solver& S, const float* a, const float* b, const float* c, float* bary,
-                             float* nrm, int* sid) {
+                             float* nrm, int* sid, double* d64 = nullptr) {
   float dmin = 1.0e8f;
   bary[0] = bary[1] = bary[2] = 1.f / 3.f;
   nrm[0] = nrm[1] = nrm[2] = 0.f;
   *sid = -1;
   for (int k = 0; k < S.n_shapes; ++k) {
     if (S.shape_type[k] != MFEM_SHAPE_CAPSULE) continue;
-    const float* xf = S.shape_xf + 7 * k;
-    float R[9];
-    quat_to_mat(xf + 3, R);
-    const float rad = S.shape_scale[3 * k], hh = S.shape_scale[3 * k + 1];
-    float p0[3], p1[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      p0[i] = xf[i] - R[i * 3 + 2] * hh;
-      p1[i] = xf[i] + R[i * 3 + 2] * hh;
-    }
-    float w[3], q[3];
-    bool hit;
-    const float dist = closest_point_triangle_segment(a, b, c, p0, p1, w, q, &hit);
-    const float d = dist - rad;
+    float w[3], n[3];
+    const float d = tri_capsule_one<float>(S, k, a, b, c, w, n);
     if (d < dmin) {
-      float p[3], n[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) p[i] = w[0] * a[i] + w[1] * b[i] + w[2] * c[i];
-      if (!hit && dist > 1e-8f) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) n[i] = (p[i] - q[i]) / dist;
-      } else {
-        const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-        const float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-        cross3f(e1, e2, n);
-        const float nl = sqrtf(dot3f(n, n));
-        if (nl > 1e-20f) {
-          n[0] /= nl; n[1] /= nl; n[2] /= nl;
-        } else {
-          n[0] = 0.f; n[1] = 0.f; n[2] = 1.f;
-        }
-        const float ax[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
-        const float pp0[3] = {p[0] - p0[0], p[1] - p0[1], p[2] - p0[2]};
-        const float aa = dot3f(ax, ax);
-        const float t = dot3f(pp0, ax) / (aa > 1e-20f ? aa : 1e-20f);
-        const float sgn = t < 0.5f ? -1.f : 1.f;
-        if (sgn * dot3f(n, ax) < 0.f) {
-          n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2];
-        }
-      }
       dmin = d;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
@@ -201,6 +238,22 @@ __device__ float tri_capsule(const mfem_solver& S, const float* a, const float* 
       *sid = k;
     }
   }
+  double dd = dmin;
+  if (*sid >= 0 && dmin < kFp64Range * S.d1) {
+    double A[3], B[3], C[3], w[3], n[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      A[i] = a[i]; B[i] = b[i]; C[i] = c[i];
+    }
+    dd = tri_capsule_one<double>(S, *sid, A, B, C, w, n);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      bary[i] = (float)w[i];
+      nrm[i] = (float)n[i];
+    }
+    dmin = (float)dd;
+  }
+  if (d64) *d64 = dd;
   return dmin;
 }
 
@@ -214,6 +267,7 @@ __device__ __forceinline__ bool tri_contact_on(const mfem_solver& S) {
 
 // per surface triangle: nearest capsule, closest point, normal
 __global__ void __launch_bounds__(kBlock) k_tri_contact(mfem_solver S) {
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= S.n_tris) return;
   const int i0 = S.tris[3 * t], i1 = S.tris[3 * t + 1], i2 = S.tris[3 * t + 2];
@@ -232,6 +286,7 @@ __global__ void __launch_bounds__(kBlock) k_tri_contact(mfem_solver S) {
   S.tri_sid[t] = sid;
 }
 __global__ void __launch_bounds__(kBlock) k_begin_step(mfem_solver S, float dt) {
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= S.n_verts) return;
   float x[3], v[3];
@@ -245,13 +300,14 @@ __global__ void __launch_bounds__(kBlock) k_begin_step(mfem_solver S, float dt) 
   if (!S.friction_on) return;
   float grad[3];
   int sid;
-  float d = shape_distance(S, x, grad, &sid);
+  double d;
+  shape_distance(S, x, grad, &sid, &d);
   float lamn = 0.f, n[3] = {0.f, 0.f, 0.f};
   int s = -1;
   if (d < S.d1) {
-    float e, de, d2e;
-    barrier(d, S.d0, S.d1, S.qb, e, de, d2e);
-    lamn = S.kappa * fmaxf(-de, 0.f);
+    double e, de, d2e;
+    barrier64(d, S.d0, S.d1, e, de, d2e);
+    lamn = (float)((double)S.kappa * fmax(-de, 0.0));
     n[0] = grad[0]; n[1] = grad[1]; n[2] = grad[2];
     s = sid;
   }
@@ -263,6 +319,7 @@ __global__ void __launch_bounds__(kBlock) k_begin_step(mfem_solver S, float dt) 
 }
 
 __global__ void __launch_bounds__(kBlock) k_vertex(mfem_solver S, float dt) {
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= S.n_verts) return;
   const float idt2 = 1.f / (dt * dt);
@@ -281,15 +338,17 @@ __global__ void __launch_bounds__(kBlock) k_vertex(mfem_solver S, float dt) {
   }
   float grad[3] = {0.f, 0.f, 0.f}, eb = 0.f, gb[3] = {0.f, 0.f, 0.f}, H[9];
   int sid;
-  float d = S.n_shapes > 0 ? shape_distance(S, x, grad, &sid) : 1.0e8f;
+  double d = 1.0e8;
+  if (S.n_shapes > 0) shape_distance(S, x, grad, &sid, &d);
   float h2 = 0.f;
   if (d < S.d1) {
-    float e, de, d2e;
-    barrier(d, S.d0, S.d1, S.qb, e, de, d2e);
-    eb = S.kappa * e;
+    double e, de, d2e;
+    barrier64(d, S.d0, S.d1, e, de, d2e);
+    const double kap = S.kappa;
+    eb = (float)(kap * e);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) gb[k] = S.kappa * de * grad[k];
-    h2 = S.kappa * fmaxf(d2e, 0.f);
+    for (int k = 0; k < 3; ++k) gb[k] = (float)(kap * de) * grad[k];
+    h2 = (float)(kap * fmax(d2e, 0.0));
   }
   if (tri_contact_on(S)) {
     // triangle-capsule barrier energy and gradient of the incident
@@ -298,12 +357,13 @@ __global__ void __launch_bounds__(kBlock) k_vertex(mfem_solver S, float dt) {
       const int src = S.vtri_src[kk], t = src / 3, a = src - 3 * t;
       const float dt_ = S.tri_dist[t];
       if (!(dt_ < S.d1)) continue;
-      float e, de, d2e;
-      barrier(dt_, S.d0, S.d1, S.qb, e, de, d2e);
+      double e, de, d2e;
+      barrier64(dt_, S.d0, S.d1, e, de, d2e);
       const float wa = S.tri_bary[3 * t + a];
-      eb += wa * S.kappa * e;
+      eb += wa * (float)((double)S.kappa * e);
+      const float kde = (float)((double)S.kappa * de);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) gb[k] += wa * S.kappa * de * S.tri_nrm[3 * t + k];
+      for (int k = 0; k < 3; ++k) gb[k] += wa * kde * S.tri_nrm[3 * t + k];
     }
   }
   float gd[3], hd[9];
@@ -320,7 +380,7 @@ __global__ void __launch_bounds__(kBlock) k_vertex(mfem_solver S, float dt) {
   }
 #pragma unroll
   for (int q = 0; q < 9; ++q) S.Hv[9 * i + q] = H[q];
-  S.dist[i] = d;
+  S.dist[i] = (float)d;
   S.eB[i] = eb;
   S.eD[i] = ed;
 }
@@ -378,6 +438,7 @@ __global__ void __launch_bounds__(kBlock) k_init_stretch(mfem_solver S) {
 }
 
 __global__ void __launch_bounds__(kBlock, 2) k_element(mfem_solver S) {
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   // per-tet outputs leave through wave_store; lanes past the last tet compute
   // the last tet again and store nothing
   __shared__ float slabs[kWaves][kSlabFloats];
@@ -513,6 +574,7 @@ __device__ __forceinline__ int xcd_block() {
 }
 
 __global__ void __launch_bounds__(kBlock) k_assemble(mfem_solver S) {
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   int l = xcd_block() * kBlock + threadIdx.x;
   if (l >= S.n_slots * 64) return;
   float acc[9];
@@ -541,9 +603,9 @@ __global__ void __launch_bounds__(kBlock) k_assemble(mfem_solver S) {
       const int src = S.tcon_src[k], t = src / 9, a = (src / 3) % 3, bb = src % 3;
       const float d = S.tri_dist[t];
       if (!(d < S.d1)) continue;
-      float e, de, d2e;
-      barrier(d, S.d0, S.d1, S.qb, e, de, d2e);
-      const float h = S.tri_bary[3 * t + a] * S.tri_bary[3 * t + bb] * S.kappa * fmaxf(d2e, 0.f);
+      double e, de, d2e;
+      barrier64(d, S.d0, S.d1, e, de, d2e);
+      const float h = S.tri_bary[3 * t + a] * S.tri_bary[3 * t + bb] * (float)((double)S.kappa * fmax(d2e, 0.0));
       const float* n = S.tri_nrm + 3 * t;
 #pragma unroll
       for (int r = 0; r < 3; ++r)
@@ -557,6 +619,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble(mfem_solver S) {
 
 // g = (H_l G)^T g_l - g_x - grad B - grad D; diagonal += (M + Q)/dt^2 + H_B + H_D
 __global__ void __launch_bounds__(kBlock) k_rhs(mfem_solver S, float dt) {
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   int i = xcd_block() * kBlock + threadIdx.x;
   if (i >= S.n_verts) return;
   float g[3];
@@ -2108,6 +2171,7 @@ __device__ __forceinline__ uint16_t f2bf16(float f) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_ws_features(mfem_solver S, float dt, int newton_iter) {
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= S.n_verts) return;
   const int c = i >> 6, lane = i & 63;
@@ -2202,6 +2266,7 @@ __global__ void __launch_bounds__(kBlock) k_ws_features(mfem_solver S, float dt,
 }
 
 __global__ void __launch_bounds__(kBlock) k_ws_apply(mfem_solver S) {
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= S.n_verts) return;
   const float* a = S.ws_aux + 10 * i;
@@ -2215,6 +2280,7 @@ __global__ void __launch_bounds__(kBlock) k_ws_apply(mfem_solver S) {
 // local solve (kernels.py:252-283)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_local(mfem_solver S) {
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= S.n_tets) return;
   int4 tv = reinterpret_cast<const int4*>(S.tets)[e];
@@ -2280,11 +2346,12 @@ __device__ double vertex_merit(const mfem_solver& S, int i, float a, float dt) {
   if (S.n_shapes > 0) {
     float grad[3];
     int sid;
-    float d = shape_distance(S, x, grad, &sid);
+    double d;
+    shape_distance(S, x, grad, &sid, &d);
     if (d < S.d1) {
-      float e, de, d2e;
-      barrier(d, S.d0, S.d1, S.qb, e, de, d2e);
-      f += (double)(S.kappa * e);
+      double e, de, d2e;
+      barrier64(d, S.d0, S.d1, e, de, d2e);
+      f += (double)S.kappa * e;
     }
     f += (double)friction_eval(S, i, x, dt, nullptr, nullptr);
   }
@@ -2329,12 +2396,13 @@ __device__ double tri_merit(const mfem_solver& S, int t, float a) {
   }
   float w[3], n[3];
   int sid;
-  const float d = tri_capsule(S, v[0], v[1], v[2], w, n, &sid);
+  double d;
+  tri_capsule(S, v[0], v[1], v[2], w, n, &sid, &d);
   if (!(d < S.d1)) return 0.0;
-  float e, de, d2e;
-  barrier(d, S.d0, S.d1, S.qb, e, de, d2e);
+  double e, de, d2e;
+  barrier64(d, S.d0, S.d1, e, de, d2e);
   const float* wl = S.tri_bary + 3 * t;
-  return (double)(S.kappa * e) * ((double)wl[0] + (double)wl[1] + (double)wl[2]);
+  return (double)S.kappa * e * ((double)wl[0] + (double)wl[1] + (double)wl[2]);
 }
 
 // (g_x + grad B + grad D) . dx for vertex i: the vertex part of the slope
@@ -2460,6 +2528,7 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
 }
 
 __global__ void __launch_bounds__(kBlock) k_plain_update(mfem_solver S, int newton_iter) {
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < S.n_verts)
 #pragma unroll
@@ -2476,6 +2545,7 @@ __global__ void __launch_bounds__(kBlock) k_plain_update(mfem_solver S, int newt
 }
 
 __global__ void __launch_bounds__(kBlock) k_end_step(mfem_solver S, float dt) {
+  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= S.n_verts) return;
 #pragma unroll
@@ -2833,9 +2903,52 @@ __global__ void __launch_bounds__(kBlock) k_vertex_distance(mfem_solver S, float
   const float x[3] = {S.x[3 * i], S.x[3 * i + 1], S.x[3 * i + 2]};
   float grad[3];
   int sid = -1;
-  const float d = S.n_shapes > 0 ? shape_distance(S, x, grad, &sid) : 1.0e8f;
-  vdist[i] = d;
+  double d = 1.0e8;
+  if (S.n_shapes > 0) shape_distance(S, x, grad, &sid, &d);
+  vdist[i] = (float)d;
   vsid[i] = sid;
+}
+
+// world pose of each shape: body_q[shape_body[s]] * local[s] for a shape on a
+// body, local[s] for a static one (shape_body < 0); poses are p(3), q(4) xyzw
+__global__ void k_pose_shapes(const float* body_q, int n_body, const int32_t* shape_body, const float* local,
+                              int n_shapes, float* shape_xf) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_shapes) return;
+  const float* l = local + 7 * s;
+  const int b = shape_body[s];
+  float o[7];
+  if (b < 0 || b >= n_body) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) o[k] = l[k];
+  } else {
+    const float* t = body_q + 7 * b;
+    float R[9];
+    quat_to_mat(t + 3, R);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) o[i] = t[i] + R[3 * i] * l[0] + R[3 * i + 1] * l[1] + R[3 * i + 2] * l[2];
+    // Hamilton product q_body * q_local (apply the local rotation first)
+    const float x1 = t[3], y1 = t[4], z1 = t[5], w1 = t[6];
+    const float x2 = l[3], y2 = l[4], z2 = l[5], w2 = l[6];
+    o[3] = w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2;
+    o[4] = w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2;
+    o[5] = w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2;
+    o[6] = w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2;
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) shape_xf[7 * s + k] = o[k];
+}
+
+int mfem_pose_shapes(const float* body_q, int n_body, const int32_t* shape_body, const float* shape_local,
+                     int n_shapes, float* shape_xf, void* stream) {
+  if (n_shapes < 0 || n_body < 0) return fail(MFEM_EINVAL, "mfem_pose_shapes: negative count");
+  if (n_shapes == 0) return MFEM_OK;
+  if (!shape_body || !shape_local || !shape_xf || (n_body > 0 && !body_q))
+    return fail(MFEM_EINVAL, "mfem_pose_shapes: null pointer");
+  hipLaunchKernelGGL(k_pose_shapes, dim3((n_shapes + 63) / 64), dim3(64), 0, (hipStream_t)stream, body_q, n_body,
+                     shape_body, shape_local, n_shapes, shape_xf);
+  HIP_TRY(hipGetLastError());
+  return MFEM_OK;
 }
 
 int mfem_contact_distance(const mfem_solver* S, float* vdist, int32_t* vsid, void* stream) {

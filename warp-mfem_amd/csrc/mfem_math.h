@@ -430,11 +430,31 @@ MFEM_DEV float friction_f1_over_y(float y, float ev) {
   if (y >= ev) return 1.f / fmaxf(y, 1e-12f);
   return -y / (ev * ev) + 2.f / ev;
 }
-MFEM_DEV void quat_to_mat(const float* q, float* R) {
-  float x = q[0], y = q[1], z = q[2], w = q[3];
-  R[0] = 1.f - 2.f * (y * y + z * z); R[1] = 2.f * (x * y - z * w); R[2] = 2.f * (x * z + y * w);
-  R[3] = 2.f * (x * y + z * w); R[4] = 1.f - 2.f * (x * x + z * z); R[5] = 2.f * (y * z - x * w);
-  R[6] = 2.f * (x * z - y * w); R[7] = 2.f * (y * z + x * w); R[8] = 1.f - 2.f * (x * x + y * y);
+template <typename T, typename Q>
+MFEM_DEV void quat_to_mat(const Q* q, T* R) {
+  const T x = q[0], y = q[1], z = q[2], w = q[3], one = 1, two = 2;
+  R[0] = one - two * (y * y + z * z); R[1] = two * (x * y - z * w); R[2] = two * (x * z + y * w);
+  R[3] = two * (x * y + z * w); R[4] = one - two * (x * x + z * z); R[5] = two * (y * z - x * w);
+  R[6] = two * (x * z - y * w); R[7] = two * (y * z + x * w); R[8] = one - two * (x * x + y * y);
+}
+
+// barrier(d) in FP64 (distances inside the barrier range): the log branch on
+// (d0, d1) and, for d <= d0, its quadratic extension from d0 with the
+// coefficients formed here in FP64
+MFEM_DEV void barrier64(double d, double d0, double d1, double& e, double& de, double& d2e) {
+  e = 0.0; de = 0.0; d2e = 0.0;
+  const double x = d > d0 ? d : d0;
+  // (no extension without a positive d0, as the host's coefficients)
+  if (!(x < d1) || !(x > 0.0)) return;
+  const double l = log(x / d1), dd = x - d1;
+  e = -dd * dd * l;
+  de = -(2.0 * dd * l + dd * dd / x);
+  d2e = -(2.0 * l + 4.0 * dd / x - dd * dd / (x * x));
+  if (d <= d0) {
+    const double t = d - d0;
+    e += de * t + 0.5 * d2e * t * t;
+    de += d2e * t;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -448,70 +468,87 @@ MFEM_DEV void cross3f(const float* a, const float* b, float* c) {
   c[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+template <typename T>
+MFEM_DEV T dot3t(const T* a, const T* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+template <typename T>
+MFEM_DEV void cross3t(const T* a, const T* b, T* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+template <typename T>
+MFEM_DEV T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T>
+MFEM_DEV T tmax(T a, T b) { return a > b ? a : b; }
+
+// (templated on the scalar type: FP32 everywhere, FP64 for contact distances
+// inside the barrier range)
 // barycentric weights of the point of triangle abc closest to x (Voronoi
 // regions of the vertices, then the edges, then the face; Ericson,
 // Real-Time Collision Detection 5.1.5)
-MFEM_DEV void closest_point_triangle_bary(const float* x, const float* a, const float* b, const float* c, float* w) {
-  float ab[3], ac[3], ax[3], bx[3], cx[3];
+template <typename T>
+MFEM_DEV void closest_point_triangle_bary(const T* x, const T* a, const T* b, const T* c, T* w) {
+  T ab[3], ac[3], ax[3], bx[3], cx[3];
   for (int i = 0; i < 3; ++i) {
     ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i]; ax[i] = x[i] - a[i]; bx[i] = x[i] - b[i]; cx[i] = x[i] - c[i];
   }
-  const float d1 = dot3f(ab, ax), d2 = dot3f(ac, ax);
-  if (d1 <= 0.f && d2 <= 0.f) { w[0] = 1.f; w[1] = 0.f; w[2] = 0.f; return; }
-  const float d3 = dot3f(ab, bx), d4 = dot3f(ac, bx);
-  if (d3 >= 0.f && d4 <= d3) { w[0] = 0.f; w[1] = 1.f; w[2] = 0.f; return; }
-  const float d5 = dot3f(ab, cx), d6 = dot3f(ac, cx);
-  if (d6 >= 0.f && d5 <= d6) { w[0] = 0.f; w[1] = 0.f; w[2] = 1.f; return; }
-  const float vc = d1 * d4 - d3 * d2;
-  if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) {
-    const float v = d1 / (d1 - d3);
-    w[0] = 1.f - v; w[1] = v; w[2] = 0.f;
+  const T d1 = dot3t(ab, ax), d2 = dot3t(ac, ax);
+  if (d1 <= T(0) && d2 <= T(0)) { w[0] = T(1); w[1] = T(0); w[2] = T(0); return; }
+  const T d3 = dot3t(ab, bx), d4 = dot3t(ac, bx);
+  if (d3 >= T(0) && d4 <= d3) { w[0] = T(0); w[1] = T(1); w[2] = T(0); return; }
+  const T d5 = dot3t(ab, cx), d6 = dot3t(ac, cx);
+  if (d6 >= T(0) && d5 <= d6) { w[0] = T(0); w[1] = T(0); w[2] = T(1); return; }
+  const T vc = d1 * d4 - d3 * d2;
+  if (vc <= T(0) && d1 >= T(0) && d3 <= T(0)) {
+    const T v = d1 / (d1 - d3);
+    w[0] = T(1) - v; w[1] = v; w[2] = T(0);
     return;
   }
-  const float vb = d5 * d2 - d1 * d6;
-  if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) {
-    const float t = d2 / (d2 - d6);
-    w[0] = 1.f - t; w[1] = 0.f; w[2] = t;
+  const T vb = d5 * d2 - d1 * d6;
+  if (vb <= T(0) && d2 >= T(0) && d6 <= T(0)) {
+    const T t = d2 / (d2 - d6);
+    w[0] = T(1) - t; w[1] = T(0); w[2] = t;
     return;
   }
-  const float va = d3 * d6 - d5 * d4;
-  if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
-    const float t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
-    w[0] = 0.f; w[1] = 1.f - t; w[2] = t;
+  const T va = d3 * d6 - d5 * d4;
+  if (va <= T(0) && (d4 - d3) >= T(0) && (d5 - d6) >= T(0)) {
+    const T t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    w[0] = T(0); w[1] = T(1) - t; w[2] = t;
     return;
   }
-  const float tot = va + vb + vc;
-  if (tot <= 0.f) { w[0] = w[1] = w[2] = 1.f / 3.f; return; }
-  const float v = vb / tot, t = vc / tot;
-  w[0] = 1.f - v - t; w[1] = v; w[2] = t;
+  const T tot = va + vb + vc;
+  if (tot <= T(0)) { w[0] = w[1] = w[2] = T(1) / T(3); return; }
+  const T v = vb / tot, t = vc / tot;
+  w[0] = T(1) - v - t; w[1] = v; w[2] = t;
 }
 
 // closest points of segments p1q1 and p2q2 (Ericson 5.1.9); returns the
 // parameter on the first segment
-MFEM_DEV float segment_segment(const float* p1, const float* q1, const float* p2, const float* q2, float* c1,
-                               float* c2) {
-  const float d1[3] = {q1[0] - p1[0], q1[1] - p1[1], q1[2] - p1[2]};
-  const float d2[3] = {q2[0] - p2[0], q2[1] - p2[1], q2[2] - p2[2]};
-  const float r[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
-  const float a = dot3f(d1, d1), e = dot3f(d2, d2), f = dot3f(d2, r);
-  const float eps = 1e-12f;
-  float s, t;
+template <typename T>
+MFEM_DEV T segment_segment(const T* p1, const T* q1, const T* p2, const T* q2, T* c1,
+                               T* c2) {
+  const T d1[3] = {q1[0] - p1[0], q1[1] - p1[1], q1[2] - p1[2]};
+  const T d2[3] = {q2[0] - p2[0], q2[1] - p2[1], q2[2] - p2[2]};
+  const T r[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
+  const T a = dot3t(d1, d1), e = dot3t(d2, d2), f = dot3t(d2, r);
+  const T eps = T(1e-12);
+  T s, t;
   if (a <= eps && e <= eps) {
-    s = 0.f; t = 0.f;
+    s = T(0); t = T(0);
   } else if (a <= eps) {
-    s = 0.f; t = fminf(fmaxf(f / e, 0.f), 1.f);
+    s = T(0); t = tmin(tmax(f / e, T(0)), T(1));
   } else {
-    const float c = dot3f(d1, r);
+    const T c = dot3t(d1, r);
     if (e <= eps) {
-      t = 0.f; s = fminf(fmaxf(-c / a, 0.f), 1.f);
+      t = T(0); s = tmin(tmax(-c / a, T(0)), T(1));
     } else {
-      const float b = dot3f(d1, d2), den = a * e - b * b;
-      s = den != 0.f ? fminf(fmaxf((b * f - c * e) / den, 0.f), 1.f) : 0.f;
+      const T b = dot3t(d1, d2), den = a * e - b * b;
+      s = den != T(0) ? tmin(tmax((b * f - c * e) / den, T(0)), T(1)) : T(0);
       t = (b * s + f) / e;
-      if (t < 0.f) {
-        t = 0.f; s = fminf(fmaxf(-c / a, 0.f), 1.f);
-      } else if (t > 1.f) {
-        t = 1.f; s = fminf(fmaxf((b - c) / a, 0.f), 1.f);
+      if (t < T(0)) {
+        t = T(0); s = tmin(tmax(-c / a, T(0)), T(1));
+      } else if (t > T(1)) {
+        t = T(1); s = tmin(tmax((b - c) / a, T(0)), T(1));
       }
     }
   }
@@ -524,65 +561,67 @@ MFEM_DEV float segment_segment(const float* p1, const float* q1, const float* p2
 
 // does segment p0p1 pierce triangle abc (Moller-Trumbore with the parameter
 // limited to [0, 1])?  w = barycentric weights of the piercing point
-MFEM_DEV bool segment_triangle_intersect(const float* p0, const float* p1, const float* a, const float* b,
-                                         const float* c, float* w) {
-  float dir[3], e1[3], e2[3], pv[3], tv[3], qv[3];
+template <typename T>
+MFEM_DEV bool segment_triangle_intersect(const T* p0, const T* p1, const T* a, const T* b,
+                                         const T* c, T* w) {
+  T dir[3], e1[3], e2[3], pv[3], tv[3], qv[3];
   for (int i = 0; i < 3; ++i) {
     dir[i] = p1[i] - p0[i]; e1[i] = b[i] - a[i]; e2[i] = c[i] - a[i]; tv[i] = p0[i] - a[i];
   }
-  cross3f(dir, e2, pv);
-  const float det = dot3f(e1, pv);
-  const float scale = sqrtf(dot3f(dir, dir)) * sqrtf(dot3f(e1, e1)) * sqrtf(dot3f(e2, e2));
-  if (fabsf(det) <= 1e-7f * scale + 1e-30f) return false;
-  const float inv = 1.f / det;
-  const float u = dot3f(tv, pv) * inv;
-  if (u < 0.f || u > 1.f) return false;
-  cross3f(tv, e1, qv);
-  const float v = dot3f(dir, qv) * inv;
-  if (v < 0.f || u + v > 1.f) return false;
-  const float t = dot3f(e2, qv) * inv;
-  if (t < 0.f || t > 1.f) return false;
-  w[0] = 1.f - u - v; w[1] = u; w[2] = v;
+  cross3t(dir, e2, pv);
+  const T det = dot3t(e1, pv);
+  const T scale = sqrt(dot3t(dir, dir)) * sqrt(dot3t(e1, e1)) * sqrt(dot3t(e2, e2));
+  if (fabs(det) <= T(1e-7) * scale + T(1e-30)) return false;
+  const T inv = T(1) / det;
+  const T u = dot3t(tv, pv) * inv;
+  if (u < T(0) || u > T(1)) return false;
+  cross3t(tv, e1, qv);
+  const T v = dot3t(dir, qv) * inv;
+  if (v < T(0) || u + v > T(1)) return false;
+  const T t = dot3t(e2, qv) * inv;
+  if (t < T(0) || t > T(1)) return false;
+  w[0] = T(1) - u - v; w[1] = u; w[2] = v;
   return true;
 }
 
 // distance from triangle abc to segment p0p1: the two endpoints against the
 // face, the three edges against the segment, then the pierce test (distance
 // 0).  bary = weights of the triangle point, q = the segment point.
-MFEM_DEV float closest_point_triangle_segment(const float* a, const float* b, const float* c, const float* p0,
-                                              const float* p1, float* bary, float* q, bool* hit) {
-  float w[3], p[3];
+template <typename T>
+MFEM_DEV T closest_point_triangle_segment(const T* a, const T* b, const T* c, const T* p0,
+                                              const T* p1, T* bary, T* q, bool* hit) {
+  T w[3], p[3];
   closest_point_triangle_bary(p0, a, b, c, bary);
   for (int i = 0; i < 3; ++i) {
     p[i] = bary[0] * a[i] + bary[1] * b[i] + bary[2] * c[i] - p0[i];
     q[i] = p0[i];
   }
-  float best = sqrtf(dot3f(p, p));
+  T best = sqrt(dot3t(p, p));
   closest_point_triangle_bary(p1, a, b, c, w);
   for (int i = 0; i < 3; ++i) p[i] = w[0] * a[i] + w[1] * b[i] + w[2] * c[i] - p1[i];
-  float d = sqrtf(dot3f(p, p));
+  T d = sqrt(dot3t(p, p));
   if (d < best) {
     best = d;
     for (int i = 0; i < 3; ++i) { bary[i] = w[i]; q[i] = p1[i]; }
   }
-  const float* E0[3] = {a, b, c};
-  const float* E1[3] = {b, c, a};
+  const T* E0[3] = {a, b, c};
+  const T* E1[3] = {b, c, a};
   for (int e = 0; e < 3; ++e) {
-    float c1[3], c2[3];
-    const float s = segment_segment(E0[e], E1[e], p0, p1, c1, c2);
-    const float df[3] = {c1[0] - c2[0], c1[1] - c2[1], c1[2] - c2[2]};
-    d = sqrtf(dot3f(df, df));
+    T c1[3], c2[3];
+    const T s = segment_segment(E0[e], E1[e], p0, p1, c1, c2);
+    const T df[3] = {c1[0] - c2[0], c1[1] - c2[1], c1[2] - c2[2]};
+    d = sqrt(dot3t(df, df));
     if (d < best) {
       best = d;
-      if (e == 0) { bary[0] = 1.f - s; bary[1] = s; bary[2] = 0.f; }
-      else if (e == 1) { bary[0] = 0.f; bary[1] = 1.f - s; bary[2] = s; }
-      else { bary[0] = s; bary[1] = 0.f; bary[2] = 1.f - s; }
+      if (e == 0) { bary[0] = T(1) - s; bary[1] = s; bary[2] = T(0); }
+      else if (e == 1) { bary[0] = T(0); bary[1] = T(1) - s; bary[2] = s; }
+      else { bary[0] = s; bary[1] = T(0); bary[2] = T(1) - s; }
       for (int i = 0; i < 3; ++i) q[i] = c2[i];
     }
   }
   *hit = segment_triangle_intersect(p0, p1, a, b, c, w);
   if (*hit) {
-    best = 0.f;
+    best = T(0);
     for (int i = 0; i < 3; ++i) bary[i] = w[i];
     for (int i = 0; i < 3; ++i) q[i] = w[0] * a[i] + w[1] * b[i] + w[2] * c[i];
   }

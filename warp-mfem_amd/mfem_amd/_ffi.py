@@ -57,7 +57,7 @@ ENTRY_POINTS = [
     "mfem_refine_workspace_bytes", "mfem_refine_select", "mfem_refine_count", "mfem_refine_apply",
     "mfem_refine_last_error", "mfem_pcg_variant_count", "mfem_pcg_variant_info", "mfem_line_search_max_blocks",
     "mfem_contact_distance", "mfem_pattern_workspace_bytes", "mfem_pattern_count", "mfem_pattern_fill",
-    "mfem_pattern_last_error",
+    "mfem_pattern_last_error", "mfem_pose_shapes",
 ]
 
 
@@ -129,6 +129,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mfem_pcg_variant_count.argtypes = []
     lib.mfem_line_search_max_blocks.argtypes = []
     lib.mfem_pcg_variant_info.argtypes = [P, i, vp]
+    lib.mfem_pose_shapes.argtypes = [vp, i, vp, vp, i, vp, vp]
     _lib = lib
     return lib
 

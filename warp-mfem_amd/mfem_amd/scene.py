@@ -42,15 +42,48 @@ SHAPE_CAPSULE = 1
 @dataclass
 class Shape:
     """Rigid collider: a plane (local +Z normal through the origin) or a
-    capsule (local Z axis, scale = (radius, half_height)), posed in the world
-    by ``pos`` and the quaternion ``quat`` = (x, y, z, w)
-    (src/mfem/ipc/distance.py:4-57)."""
+    capsule (local Z axis, scale = (radius, half_height))
+    (src/mfem/ipc/distance.py:4-57).  ``pos`` and the quaternion ``quat`` =
+    (x, y, z, w) are the shape's transform relative to its rigid body
+    ``body`` (newton's shape_transform / shape_body), or its world pose when
+    ``body`` is -1 (a static shape); the world pose is
+    body_q[body] * (pos, quat) (src/mfem/refinement/contact.py:74-82)."""
 
     type: int
     pos: np.ndarray = field(default_factory=lambda: np.zeros(3))
     quat: np.ndarray = field(default_factory=lambda: np.array([0.0, 0.0, 0.0, 1.0]))
     scale: np.ndarray = field(default_factory=lambda: np.zeros(3))
     mu: float = 0.0
+    body: int = -1
+
+
+def quat_mul(q1, q2) -> np.ndarray:
+    """Hamilton product q1 * q2 of (x, y, z, w) quaternions (q2 applied first)."""
+    x1, y1, z1, w1 = q1
+    x2, y2, z2, w2 = q2
+    return np.array([w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2, w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
+                     w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2, w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2])
+
+
+def quat_rotate(q, v) -> np.ndarray:
+    """Rotation of v by the unit quaternion q = (x, y, z, w)."""
+    u, w = np.asarray(q[:3], float), float(q[3])
+    v = np.asarray(v, float)
+    return v + 2.0 * np.cross(u, np.cross(u, v) + w * v)
+
+
+def shape_world_poses(model, body_q=None) -> np.ndarray:
+    """(n_shapes, 7) world poses p, q of the model's shapes for the rigid
+    body poses ``body_q`` (n_bodies, 7) (default: the model's body_q)."""
+    bq = getattr(model, "body_q", None) if body_q is None else body_q
+    out = []
+    for sh in model.shapes:
+        p, q = np.asarray(sh.pos, float), np.asarray(sh.quat, float)
+        if sh.body >= 0:
+            t = np.asarray(bq, float).reshape(-1, 7)[sh.body]
+            p, q = t[:3] + quat_rotate(t[3:], p), quat_mul(t[3:], q)
+        out.append(np.concatenate([p, q]))
+    return np.array(out).reshape(-1, 7)
 
 
 @dataclass
@@ -67,6 +100,7 @@ class Model:
     gravity: np.ndarray
     params: dict = field(default_factory=dict)
     name: str = ""
+    body_q: np.ndarray | None = None  # (n_bodies, 7) initial rigid body poses p, q (xyzw)
 
     @property
     def particle_count(self) -> int:
@@ -257,11 +291,12 @@ def capsule_poke_scene(cells=(6, 6, 2), energy="arap") -> Model:
     pinned = np.nonzero(p[:, 2] < 1e-9)[0]
     # capsule axis along world x (local Z rotated 90 degrees about y), its
     # surface 2 mm above the slab's top face
-    cap = Shape(SHAPE_CAPSULE, pos=np.array([0.15, 0.15, 0.1 + 0.012]),
-                quat=np.array([0.0, np.sin(np.pi / 4), 0.0, np.cos(np.pi / 4)]),
-                scale=np.array([0.01, 0.08, 0.0]), mu=0.2)
+    # the capsule is rigid body 0 (its shape at the body's origin), posed by
+    # State.body_q each step
+    cap = Shape(SHAPE_CAPSULE, scale=np.array([0.01, 0.08, 0.0]), mu=0.2, body=0)
     m = build_model(p, t, _materials(mu, lam, len(t)), 1000.0, pinned=pinned, shapes=[cap], gravity=(0, 0, -9.81),
                     name="capsule_poke")
+    m.body_q = np.array([[0.15, 0.15, 0.1 + 0.012, 0.0, np.sin(np.pi / 4), 0.0, np.cos(np.pi / 4)]])
     m.params.update(energy=energy, iterations=6, cg_max_iterations=2000, cg_tolerance=1e-6,
                     line_search=True, line_search_max_iterations=20, contact_d0=1e-3, contact_d1=5e-3,
                     contact_stiffness=50.0, friction_mu=[0.2], friction_eps_v=1e-3, attachment_stiffness=10.0,

@@ -32,7 +32,7 @@ class State:
 
     particle_q: torch.Tensor   # (V, 3) float32, device
     particle_qd: torch.Tensor  # (V, 3) float32, device
-    body_q: torch.Tensor | None = None  # (n_shapes, 7) world poses p, q(xyzw), or None = model poses
+    body_q: torch.Tensor | None = None  # (n_bodies, 7) rigid body poses p, q(xyzw), or None = keep the poses
 
     def assign(self, other: "State") -> None:
         self.particle_q.copy_(other.particle_q)
@@ -162,7 +162,22 @@ class RefinementSolver:
         b["rest"] = f32(pad(np.asarray(model.particle_q).reshape(-1, 3), Vc))  # rest positions (refinement)
         b["tris"] = i32(pad(tris, max(Fc, 1), np.int32))
         b["shape_type"] = i32([sh.type for sh in shapes] or [0])
-        b["shape_xf"] = f32(np.array([np.concatenate([sh.pos, sh.quat]) for sh in shapes]) if shapes else np.zeros((1, 7)))
+        # shape poses relative to their rigid bodies (newton's shape_transform
+        # / shape_body; body -1 = static, the pose is the world pose), and the
+        # world poses for the model's initial body poses
+        shape_body = [int(getattr(sh, "body", -1)) for sh in shapes]
+        self._n_rigid = max(shape_body, default=-1) + 1
+        model_bq = getattr(model, "body_q", None)
+        if self._n_rigid > 0 and (model_bq is None or np.asarray(model_bq).reshape(-1, 7).shape[0] < self._n_rigid):
+            raise ValueError("the model's shapes are attached to rigid bodies but model.body_q lacks their poses")
+        b["shape_body"] = i32(shape_body or [-1])
+        b["shape_local"] = f32(np.array([np.concatenate([sh.pos, sh.quat]) for sh in shapes]) if shapes
+                               else np.zeros((1, 7)))
+        world = np.zeros((1, 7))
+        if shapes:
+            from .scene import shape_world_poses
+            world = shape_world_poses(model)
+        b["shape_xf"] = f32(world)
         b["shape_scale"] = f32(np.array([sh.scale for sh in shapes]) if shapes else np.zeros((1, 3)))
         b["shape_mu"] = f32(mu if n_shapes else np.zeros(1))
         b["s"] = zf(Tc, 6)
@@ -263,6 +278,9 @@ class RefinementSolver:
             if hasattr(S, name):
                 setattr(S, name, t.data_ptr())
         self._S = S
+        # device error bits as of the last step that has completed, copied
+        # into pinned host memory at the end of every step (no host sync)
+        self._status_mirror = torch.zeros(1, dtype=torch.int32, pin_memory=self.device.type == "cuda")
         self._pattern_builder = None
         self._set_topology(nv, nt, len(tris))
         self.warm_start_net = None
@@ -453,12 +471,19 @@ class RefinementSolver:
         self._bound = (q, qd)
 
     def _set_body_q(self, body_q) -> None:
-        if body_q is None:
+        """World poses of the shapes for this step's rigid body poses:
+        body_q[shape_body] * shape_transform, static shapes unchanged
+        (src/mfem/refinement/contact.py:74-82)."""
+        if body_q is None or self._S.n_shapes == 0:
             return
-        bq = torch.as_tensor(body_q, dtype=torch.float32, device=self.device).reshape(-1, 7)
-        if bq.shape[0] != self._S.n_shapes:
-            raise ValueError("body_q must hold one (p, q) pose per shape")
-        self.buffers["shape_xf"].copy_(bq)
+        bq = torch.as_tensor(body_q, dtype=torch.float32, device=self.device).reshape(-1, 7).contiguous()
+        if bq.shape[0] < self._n_rigid:
+            raise ValueError(f"body_q holds {bq.shape[0]} poses but the shapes use {self._n_rigid} rigid bodies")
+        b = self.buffers
+        self._body_q = bq  # (kept alive until the launch has read it)
+        _ffi.call("mfem_pose_shapes", ctypes.c_void_p(bq.data_ptr()), ctypes.c_int(bq.shape[0]),
+                  ctypes.c_void_p(b["shape_body"].data_ptr()), ctypes.c_void_p(b["shape_local"].data_ptr()),
+                  ctypes.c_int(self._S.n_shapes), ctypes.c_void_p(b["shape_xf"].data_ptr()), self._stream())
 
     def set_warm_start(self, net) -> None:
         """Attach a trained ``learned.WarmStartNet`` (or None for plain PCG).
@@ -482,6 +507,7 @@ class RefinementSolver:
         predictor, ``iterations`` Newton iterations (update, assemble, PCG,
         local solve, line search) and the velocity update, all on the device
         without a host synchronisation."""
+        self._check_device_status()
         self._step += 1
         if state_out is not state_in:
             state_out.assign(state_in)
@@ -490,11 +516,26 @@ class RefinementSolver:
         if self._refine_enabled:
             self._refine_pass(state_out)
         self._call("mfem_step", ctypes.c_float(dt), self._stream())
+        self._mirror_status()
+
+    def _check_device_status(self) -> None:
+        """Raises when an earlier step ended with a device error (a grid
+        barrier timed out): every kernel of a step then does nothing, so the
+        state stops advancing until ``reset_status``.  Reads the pinned
+        mirror of the last completed step: no host synchronisation."""
+        if int(self._status_mirror[0]) != 0:
+            raise _ffi.MfemError("a grid barrier timed out in an earlier step (device error bits "
+                                 f"{int(self._status_mirror[0])}); the state was left untouched since; "
+                                 "call reset_status() to continue")
+
+    def _mirror_status(self) -> None:
+        self._status_mirror.copy_(self.buffers["status"][:1], non_blocking=True)
 
     def step_instrumented(self, state_in: State, state_out: State, dt: float, timers: dict) -> None:
         """Same kernels and order as ``step`` (mfem_step), issued stage by stage
         so that torch.cuda.Events on the launching stream bracket each phase.
         ``timers[phase]`` collects (start, end) event pairs."""
+        self._check_device_status()
         self._step += 1
         if state_out is not state_in:
             state_out.assign(state_in)
@@ -521,6 +562,7 @@ class RefinementSolver:
             timed("local_solve", self.local_solve)
             timed("line_search", self.line_search, dt, it)
         timed("end_step", self.end_step, dt)
+        self._mirror_status()
 
     # stage entry points (used by tests and profiling) ------------------------
     def bind(self, state: State) -> None:
@@ -728,8 +770,9 @@ class RefinementSolver:
         return st[:, : self.iterations].max(axis=0)
 
     def status(self) -> int:
-        """Device error bits (bit 0: a grid barrier timed out; the persistent
-        kernels then do nothing until ``reset_status``)."""
+        """Device error bits (bit 0: a grid barrier timed out; every kernel of
+        a step then returns at once, leaving x, v, s and lambda untouched,
+        until ``reset_status``; the next ``step`` raises)."""
         torch.cuda.synchronize(self.device)
         return int(self.buffers["status"][0].item())
 
@@ -741,6 +784,7 @@ class RefinementSolver:
         self.buffers["flags"].zero_()
         self.buffers["red"].zero_()
         torch.cuda.synchronize(self.device)
+        self._status_mirror.zero_()
 
     def system_csr(self):
         """Assembled H as a scipy.sparse.bsr_matrix (host copy)."""
