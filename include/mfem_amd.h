@@ -311,6 +311,17 @@ int mfem_scan_i32(const int32_t* in, int32_t* out, int n, int inclusive, int32_t
                   size_t workspace_bytes, void* stream);
 size_t mfem_scan_workspace_bytes(int n);
 
+/* mfem_scan_i32 without a workspace argument and without any allocation (no
+   counterpart in the reference): for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  Its tile sums live in a fixed library-held scratch
+   array (one per GPU, ~256 KiB, held until the process ends), so n is at
+   most MFEM_SCAN_NOALLOC_MAX (larger n: MFEM_ECAPACITY, nothing launched).
+   Calls that overlap on one GPU (two streams at once) race on that scratch;
+   calls in one stream order are safe. */
+#define MFEM_SCAN_NOALLOC_MAX (1 << 26)
+int mfem_scan_i32_noalloc(const int32_t* in, int32_t* out, int n, int inclusive, int32_t* total, void* stream);
+
 /* Spatial-hash broad phase: all point-triangle pairs (points[i], tri) whose
    boxes inflated by d_hat / 2 overlap, the point not a vertex of the
    triangle, as (vertex id, tri index) sorted by query order then tri index;

@@ -11,6 +11,8 @@
 // corner of the intersection of the two boxes, so each pair appears once.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <cstdint>
 #include <string>
 
@@ -98,7 +100,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int32_t* in, 
   if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) *total_out = run;
 }
 
-size_t scan_ws_bytes(long n) {
+constexpr size_t scan_ws_bytes(long n) {
   size_t b = 0;
   long m = n;
   while (m > kScanTile) {
@@ -130,6 +132,41 @@ int scan_i32(const int32_t* in, int32_t* out, long n, int inclusive, int32_t* to
                      (int32_t*)nullptr);
   return launched();
 }
+
+// ---------------------------------------------------------------------------
+// scratch of the allocation-free scan (mfem_scan_i32_noalloc): the tile sums
+// of every recursion level for inputs of up to MFEM_SCAN_NOALLOC_MAX values,
+// 65536 sums at the first level and 64 at the second (256-byte aligned
+// levels, as scan_i32 lays them out).  One copy per GPU, held until the
+// process ends; every call on a GPU uses that GPU's copy, so calls that
+// overlap on one GPU (two streams at once) race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = 262144 + 256;
+static_assert(MFEM_SCAN_NOALLOC_MAX / kScanTile == 65536, "first level of tile sums");
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of this GPU's scratch (the array's host-side name is no
+// device address): looked up once per device, with that device current
+int noalloc_scratch(char** ws) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(MFEM_EHIP, "mfem_scan_i32_noalloc: no current device");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(MFEM_EHIP, (std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)).c_str());
+    g_scratch_ptr[dev] = ptr;
+  }
+  *ws = static_cast<char*>(g_scratch_ptr[dev]);
+  return MFEM_OK;
+}
+static_assert(scan_ws_bytes(MFEM_SCAN_NOALLOC_MAX) == kNoAllocScratchBytes, "scratch = the scan's tile sums at the maximum");
 
 // ---------------------------------------------------------------------------
 // geometry (src/mfem/collision.py:220-322)
@@ -569,6 +606,23 @@ extern "C" {
 const char* mfem_collide_last_error(void) { return g_err.c_str(); }
 
 size_t mfem_scan_workspace_bytes(int n) { return scan_ws_bytes(n) + 256; }
+
+int mfem_scan_i32_noalloc(const int32_t* in, int32_t* out, int n, int inclusive, int32_t* total, void* stream) {
+  if (n < 0 || (n > 0 && (!in || !out))) return fail(MFEM_EINVAL, "bad scan arguments");
+  if (n > MFEM_SCAN_NOALLOC_MAX)
+    return fail(MFEM_ECAPACITY, ("mfem_scan_i32_noalloc: n = " + std::to_string(n) + " exceeds the maximum of " +
+                                 std::to_string(MFEM_SCAN_NOALLOC_MAX) + " values")
+                                    .c_str());
+  if (n == 0) {
+    if (total) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kThreads), 0, (hipStream_t)stream, total, 1L, 0);
+    return launched();
+  }
+  char* ws = nullptr;
+  if (int rc = noalloc_scratch(&ws)) return rc;
+  // every tile sum the scan reads is written by the same call first: nothing
+  // in the scratch needs clearing
+  return scan_i32(in, out, n, inclusive, total, ws, kNoAllocScratchBytes, (hipStream_t)stream);
+}
 
 int mfem_scan_i32(const int32_t* in, int32_t* out, int n, int inclusive, int32_t* total, void* workspace,
                   size_t workspace_bytes, void* stream) {

@@ -45,6 +45,7 @@ MFEM_RED_MAX_BLOCKS = 2048
 SHAPE_PLANE, SHAPE_CAPSULE = 0, 1
 PCG_PIPELINED, PCG_STREAM_ROWS, PCG_LEARNED_GUESS = 1, 2, 4
 MFEM_MLP_NPARAMS = 6288
+MFEM_SCAN_NOALLOC_MAX = 1 << 26
 
 ENTRY_POINTS = [
     "mfem_solver_sizeof", "mfem_solver_offsetof", "mfem_version", "mfem_wall_clock_khz", "mfem_last_error",
@@ -57,7 +58,7 @@ ENTRY_POINTS = [
     "mfem_refine_workspace_bytes", "mfem_refine_select", "mfem_refine_count", "mfem_refine_apply",
     "mfem_refine_last_error", "mfem_pcg_variant_count", "mfem_pcg_variant_info", "mfem_line_search_max_blocks",
     "mfem_contact_distance", "mfem_pattern_workspace_bytes", "mfem_pattern_count", "mfem_pattern_fill",
-    "mfem_pattern_last_error", "mfem_pose_shapes",
+    "mfem_pattern_last_error", "mfem_pose_shapes", "mfem_scan_i32_noalloc",
 ]
 
 
@@ -113,6 +114,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mfem_scan_i32.argtypes = [vp, vp, i, i, vp, vp, sz, vp]
     lib.mfem_scan_workspace_bytes.argtypes = [i]
     lib.mfem_scan_workspace_bytes.restype = sz
+    lib.mfem_scan_i32_noalloc.argtypes = [vp, vp, i, i, vp, vp]
     lib.mfem_broadphase_pt.argtypes = [vp, vp, i, vp, i, f, f, i, vp, i, vp, vp, vp, sz, vp]
     lib.mfem_broadphase_ee.argtypes = [vp, vp, i, f, f, i, vp, i, vp, vp, vp, sz, vp]
     lib.mfem_broadphase_workspace_bytes.argtypes = [i, i, i]
