@@ -73,6 +73,13 @@ def main():
                          "(even; 0 = block Jacobi as the reference; default: the autotuner picks 0, 2 or 4)")
     ap.add_argument("--local-omega", type=float, default=0.5)
     ap.add_argument("--no-learned", action="store_true", help="time plain PCG only")
+    ap.add_argument("--coarse-steps", type=int, default=3, help="plain steps logged for the coarse-weight network")
+    ap.add_argument("--coarse-epochs", type=int, default=10)
+    ap.add_argument("--coarse-lr", type=float, default=2e-2)
+    ap.add_argument("--coarse-omega", type=float, default=0.0,
+                    help="uniform smoothing weight of the coarse basis when the network is off")
+    ap.add_argument("--uniform-omega", type=float, default=0.75,
+                    help="the hand-tuned uniform weight of the ablation that replaces the network")
     ap.add_argument("--dump-iters", default=None,
                     help="write the CG iteration count of every PCG launch, in launch order, to this JSON file "
                          "(to divide per-dispatch hardware counters by iterations)")
@@ -114,6 +121,7 @@ def main():
     solver = RefinementSolver(m, m.params["iterations"], max_tets=m.tet_count, enable_refinement=False,
                               device="cuda:0", pcg_pipelined=args.pcg == "pipelined",
                               pcg_resident_rows=not args.stream_rows, **kw)
+    T_tets = m.tet_count
     state = model_state(m, "cuda:0")
     out = state.clone()
     print(f"[bench] {wl}: {m.tet_count} tets, {m.particle_count} verts, warmup {args.warmup}", flush=True)
@@ -128,13 +136,39 @@ def main():
         raise SystemExit("device error during warmup")
 
     from mfem_amd import _ffi
+    from mfem_amd.coarse import CoarseWeightNet, collect_coarse
     from mfem_amd.learned import WarmStartNet, collect_solves
-
 
     learned = not args.no_learned
     train_info = None
+    coarse_info = None
     net = None
+    cnet = None
     if learned:
+        # the coarse-weight network (smoothing of the PCG's workgroup-local
+        # rigid-mode basis): systems logged from plain steps, trained on the
+        # GPU, kept only if whole steps take fewer CG iterations than with
+        # the network off (w = 0)
+        t0 = time.perf_counter()
+        csol = collect_coarse(solver, state, dt, steps=args.coarse_steps)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        cnet = CoarseWeightNet("cuda:0", seed=1)
+        chist = cnet.fit(csol, solver.pattern, epochs=args.coarse_epochs, lr=args.coarse_lr, seed=0)
+        del csol
+        cval = cnet.validate_cg(solver, state, dt, steps=2)
+        cw = cnet_weights_summary(solver, cnet, state, dt)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        coarse_info = {"systems": args.coarse_steps * solver.iterations, "epochs": args.coarse_epochs,
+                       "objective": "log(1 - (z.He)^2 / ((e.He)(z.Hz))), z = M(w)^-1 H e, e = dx* - (x~ - x): "
+                                    "energy-norm reduction of one damped preconditioned Richardson step",
+                       "collect_s": t1 - t0, "train_s": t2 - t1, "loss_network_off": chist["off"],
+                       "loss_per_epoch": [round(v, 5) for v in chist["loss"]], "best_epoch": chist["best_epoch"],
+                       "loss_uniform_weight": chist.get("uniform"), "validation": cval, "weights": cw}
+        print(f"[bench] coarse-weight network: loss {chist['loss'][0]:.5g} -> {min(chist['loss']):.5g} (kept epoch "
+              f"{chist['best_epoch']}); validation CG iterations {cval['cg_iters']}, kept {cval['kept']}", flush=True)
+        # the warm start's network (Galerkin start candidate)
         t0 = time.perf_counter()
         solves = collect_solves(solver, state, dt, steps=args.train_steps)
         torch.cuda.synchronize()
@@ -142,7 +176,11 @@ def main():
         net = WarmStartNet("cuda:0")
         hist = net.fit_solves(solves, solver, args.objective, epochs=args.epochs, lr=1e-4, seed=0)
         del solves
-        val = net.validate_cg(solver, state, dt, steps=2)
+        solver.set_coarse_net(cnet)
+        # its launches (features, MLP, blend) cost ~5 % of a step: kept only
+        # if it saves at least that share of the CG iterations
+        val = net.validate_cg(solver, state, dt, steps=2, margin=0.05)
+        solver.set_coarse_net(None)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         train_info = {"solves": args.train_steps * solver.iterations,
@@ -151,32 +189,49 @@ def main():
                       "train_s": t2 - t1, "loss_prev_newton_step": hist["base"],
                       "loss_per_epoch": [round(v, 4) for v in hist["loss"]], "best_epoch": hist["best_epoch"],
                       "validation": val}
-        print(f"[bench] trained in {t2 - t1:.2f} s, objective {hist['loss'][0]:.4g} -> {min(hist['loss']):.4g} "
-              f"(kept epoch {hist['best_epoch']}; previous Newton step alone: {hist['base']:.4g}); validation CG "
-              f"iterations {val['cg_iters']}, kept {val['kept']}", flush=True)
+        print(f"[bench] warm start trained in {t2 - t1:.2f} s, objective {hist['loss'][0]:.4g} -> "
+              f"{min(hist['loss']):.4g} (kept epoch {hist['best_epoch']}); validation CG iterations "
+              f"{val['cg_iters']}, kept {val['kept']}", flush=True)
+        if val["kept"] != "trained":
+            net = None  # a zero correction: no warm-start network launches (the Galerkin start without it)
 
     khz = solver.lib.mfem_wall_clock_khz()
 
-    tune = None
-    if not args.no_autotune:
-        # whole steps with the warm start and preconditioner the timed leg uses
-        solver.set_warm_start(net)
+    def configure(ws_net, coarse_net, coarse_on, omega):
+        if ws_net is not None:
+            solver.set_warm_start(ws_net)
+        else:
+            solver.set_galerkin_start(learned)  # the Galerkin start without the learned guess
         solver.set_local_preconditioner(0, args.local_omega)
-        tune = solver.autotune(state, dt, local_sweeps=_sweep_options(args))
-        c = tune["chosen"]
-        args.local_sweeps = c["local_sweeps"]
-        print(f"[bench] autotuned PCG: variant {c['variant']} ({c['chunks_per_wave']} chunk(s)/wave, "
-              f"{c['resident_slots']} resident slots, {c['mode']}), {c['local_sweeps']} local sweeps, "
-              f"{c['ms_per_step']:.2f} ms/step, {c['us_per_iter']:.2f} us/iter", flush=True)
-    if args.local_sweeps is None:
-        args.local_sweeps = 0
+        solver.set_coarse_net(None)
+        solver.set_coarse_correction(coarse_on, omega=omega)
+        if coarse_net is not None:
+            solver.set_coarse_net(coarse_net)
+        if not coarse_on:
+            solver._S.pcg_flags |= _ffi.PCG_PIPELINED
 
-    def timed_leg(s0, use_net, sweeps=0):
+    tune = None
+    if learned and not args.no_autotune:
+        configure(net, cnet if coarse_info["validation"]["kept"] == "trained" else None, True, args.coarse_omega)
+        tune = solver.autotune(state, dt)
+        c = tune["chosen"]
+        print(f"[bench] autotuned PCG: variant {c['variant']} ({c['chunks_per_wave']} chunk(s)/wave, "
+              f"{c['resident_slots']} resident slots, {c['mode']}), {c['ms_per_step']:.2f} ms/step, "
+              f"{c['us_per_iter']:.2f} us/iter", flush=True)
+    tuned_variant = tune["chosen"]["variant"] if tune else -1
+
+    def timed_leg(s0, ws_net, coarse_net, coarse_on, omega=0.0, ws=True):
         st = s0.clone()
         o = st.clone()
         solver.restore(snap0)
-        solver.set_warm_start(use_net)
-        solver.set_local_preconditioner(sweeps, args.local_omega)
+        if ws:
+            configure(ws_net, coarse_net, coarse_on, omega)
+        else:
+            solver.set_warm_start(None)
+            solver.set_coarse_net(None)
+            solver.set_coarse_correction(False)
+            solver._S.pcg_flags |= _ffi.PCG_PIPELINED
+        solver._S.pcg_variant = tuned_variant if coarse_on else -1
         timers: dict = {}
         hist = []
         torch.cuda.synchronize()
@@ -196,7 +251,7 @@ def main():
 
     s0 = state.clone()
     snap0 = solver.snapshot()  # every leg starts from the same solver state
-    wall_p, st_p, timers_p, end_p = timed_leg(s0, None)
+    wall_p, st_p, timers_p, end_p = timed_leg(s0, None, None, False, ws=False)
     if args.dump_iters:
         launch_iters += st_p[:, :, 0].astype(int).reshape(-1).tolist()
         os.makedirs(os.path.dirname(os.path.abspath(args.dump_iters)), exist_ok=True)
@@ -204,18 +259,26 @@ def main():
             json.dump({"kernel": "k_pcg", "iters_per_launch": launch_iters, "tets": m.tet_count,
                        "verts": m.particle_count, "nnz_blocks": int(len(solver.pattern.cols))}, f)
     print(f"[bench] plain: {args.steps} steps in {wall_p:.3f} s", flush=True)
+    abl = {}
     if learned:
-        wall, st, timers, end_l = timed_leg(s0, net, args.local_sweeps)
-        print(f"[bench] learned + local preconditioner: {args.steps} steps in {wall:.3f} s", flush=True)
+        cn = cnet if coarse_info["validation"]["kept"] == "trained" else None
+        wall, st, timers, end_l = timed_leg(s0, net, cn, True, args.coarse_omega)
+        print(f"[bench] learned (warm start + learned coarse correction): {args.steps} steps in {wall:.3f} s",
+              flush=True)
         pos_diff = float((end_l.particle_q - end_p.particle_q).abs().max().item())
-        # ablations: the same with the network's correction off (zero output
-        # layer: the previous Newton step, same fallback), and the learned
-        # start with the reference's block-Jacobi preconditioner
-        base_net = WarmStartNet("cuda:0", weights=np.zeros(net.w.numel(), np.float32))
-        wall_b, st_b, _, _ = timed_leg(s0, base_net, args.local_sweeps)
-        print(f"[bench] previous Newton step only: {args.steps} steps in {wall_b:.3f} s", flush=True)
-        wall_j, st_j, _, _ = timed_leg(s0, net, 0)
-        print(f"[bench] learned, block Jacobi: {args.steps} steps in {wall_j:.3f} s", flush=True)
+        # ablations, from the same state and solver state: the networks off
+        # (Galerkin start without the learned guess, unsmoothed rigid modes,
+        # w = 0); a hand-tuned uniform weight in place of the network; the
+        # coarse correction off (block Jacobi under the Galerkin start)
+        for name, kwl in (("networks_off", dict(ws_net=None, coarse_net=None, coarse_on=True, omega=0.0)),
+                          ("uniform_weight", dict(ws_net=net, coarse_net=None, coarse_on=True,
+                                                  omega=args.uniform_omega)),
+                          ("block_jacobi", dict(ws_net=net, coarse_net=None, coarse_on=False))):
+            w_, s_, _, _ = timed_leg(s0, **kwl)
+            abl[name] = {"value": T_tets * args.steps / w_, "ms_per_step": w_ * 1e3 / args.steps,
+                         "cg_iters_per_step": float(s_[:, :, 0].sum(axis=1).mean())}
+            print(f"[bench] ablation {name}: {args.steps} steps in {w_:.3f} s, "
+                  f"{abl[name]['cg_iters_per_step']:.0f} CG iterations per step", flush=True)
     else:
         wall, st, timers = wall_p, st_p, timers_p
 
@@ -227,6 +290,7 @@ def main():
     relres = st[:, :, 1]
     phase_ms = {k: sum(a.elapsed_time(b) for a, b in v) for k, v in timers.items()}
     pcg_ms = np.array([a.elapsed_time(b) for a, b in timers["global_solve"]])
+    pcg_ms_p = np.array([a.elapsed_time(b) for a, b in timers_p["global_solve"]])
     nnz = int(len(solver.pattern.cols))
     V, T = m.particle_count, m.tet_count
     bytes_per_iter = 40.0 * nnz + 24.0 * V
@@ -252,10 +316,10 @@ def main():
         "newton_iterations": solver.iterations,
         "dt": dt,
         "preconditioner": {"plain": "block_jacobi",
-                           "value": f"block_jacobi + {args.local_sweeps} workgroup-local sweeps (omega "
-                                    f"{args.local_omega})" if args.local_sweeps else "block_jacobi"},
-        "pcg": tune["chosen"]["mode"] if tune else args.pcg,
-        "learned_warm_start": learned,
+                           "value": "block_jacobi + workgroup-local rigid-mode correction, learned smoothing weights"
+                           if learned else "block_jacobi"},
+        "pcg": tune["chosen"]["mode"] if tune else "pipelined",
+        "learned_warm_start": bool(learned and net is not None),
         "reductions": "f64",
         "plain_value": value_plain,
         "plain_ms_per_step": wall_p * 1e3 / args.steps,
@@ -269,12 +333,13 @@ def main():
         "ls_trials_last_step": st[-1, :, 2].astype(int).tolist(),
         "pcg_phase_us_per_iter": pcg_phase_us,
         "phase_ms_per_step": {k: v / args.steps for k, v in phase_ms.items()},
+        "plain_us_per_cg_iter": float(np.sum(pcg_ms_p) * 1e3 / max(1.0, float(st_p[:, :, 0].sum()))),
     }
     if tune is not None:
         config["pcg_autotune"] = {
             "chosen": tune["chosen"]["variant"],
             "table": [{k: r[k] for k in ("variant", "chunks_per_wave", "resident_slots", "mode", "workgroups",
-                                         "local_sweeps", "ms_per_step", "us_per_iter", "cg_iters")}
+                                         "ms_per_step", "us_per_iter", "cg_iters")}
                       for r in tune["table"]]}
     if learned:
         config["pcg_start_frac"] = {"x_tilde_minus_x": float((st[:, :, 14] == 0).mean()),
@@ -283,12 +348,11 @@ def main():
         config["start_residual_ratio_median"] = float(np.median(st[:, :, 15]))
         config["max_abs_position_diff_vs_plain"] = pos_diff
         config["training"] = train_info
-        config["ablation_prev_newton_step_only"] = {
-            "value": T * args.steps / wall_b, "ms_per_step": wall_b * 1e3 / args.steps,
-            "cg_iters_per_step": float(st_b[:, :, 0].sum(axis=1).mean())}
-        config["ablation_block_jacobi"] = {
-            "value": T * args.steps / wall_j, "ms_per_step": wall_j * 1e3 / args.steps,
-            "cg_iters_per_step": float(st_j[:, :, 0].sum(axis=1).mean())}
+        config["coarse_weight_network"] = coarse_info
+        config["ablations"] = abl
+        # what the networks add over the same method without them
+        config["learned_correction_gain"] = value / abl["networks_off"]["value"]
+        config["learned_vs_uniform_weight"] = value / abl["uniform_weight"]["value"]
     result = {
         "metric": "tet-steps/s at CG rel-res 1e-6, learned-precond vs plain PCG, 1 MI355X",
         "value": value,
@@ -305,22 +369,26 @@ def main():
         "config": config,
         "roofline": {
             "kernel": "k_pcg",
-            "bound": "hbm",
+            "bound": "latency",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "bytes_per_cg_iter": bytes_per_iter,
-            "traffic_note": "traffic = measured HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, "
-                            "profiles/hbm_traffic_octopus_204k.json); below the algorithmic bytes because the "
-                            "matrix rows stay in VGPRs for the whole solve: an iteration moves only the published "
-                            "vector (staged per workgroup in LDS)",
+            "traffic_note": "achieved = algorithmic bytes (40 B per nonzero block + 24 B per vertex per CG iteration) "
+                            "over the launch's event time; traffic = measured HBM bytes per launch (FETCH_SIZE x 2.0 "
+                            "+ WRITE_SIZE x 1.0, profiles/hbm_traffic_octopus_204k.json), below the algorithmic bytes "
+                            "because the matrix rows stay in VGPRs for the whole solve: an iteration moves only the "
+                            "published vector (staged per workgroup in LDS); the kernel is bound by grid-reduction "
+                            "and memory latency, not HBM bandwidth",
+            "hbm_frac_measured": (float(traffic / (np.mean(pcg_ms[active]) * 1e-3) / 1e9 / HBM_PEAK_GBS)
+                                  if traffic else None),
             "us_per_cg_iter": float(np.sum(pcg_ms[active]) * 1e3 / max(1.0, np.sum(iters_flat[active]))),
         },
     }
     if learned:
-        result["mlp"] = mlp_mfma(solver, net, V)
+        result["mlp"] = mlp_mfma(solver, cnet, V)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(m, dt)
     print(json.dumps(result), flush=True)
@@ -581,12 +649,35 @@ def _sweep_options(args):
     return (0, 2, 4) if args.local_sweeps is None else (args.local_sweeps,)
 
 
-def mlp_mfma(solver, net, V):
-    """Event-timed forward pass of the warm-start MLP over the body's
-    features (the launch the solver makes once per Newton iteration)."""
+def cnet_weights_summary(solver, cnet, state, dt):
+    """Statistics of the network's smoothing weights on the first Newton
+    system of a step from ``state`` (the solver state is restored)."""
     import torch
 
-    X = solver.buffers["ws_x"]
+    snap = solver.snapshot()
+    try:
+        st = state.clone()
+        solver.bind(st)
+        solver.begin_step(dt)
+        solver.update_system(dt)
+        solver.assemble(dt)
+        solver.set_coarse_net(cnet)
+        solver.coarse_weights_from_net(dt)
+        torch.cuda.synchronize()
+        w = solver.buffers["coarse_w_net"][: solver.active_particle_count].double()
+        q = torch.quantile(w, torch.tensor([0.05, 0.5, 0.95], dtype=w.dtype, device=w.device)).tolist()
+        return {"mean": float(w.mean()), "std": float(w.std()), "p05": q[0], "median": q[1], "p95": q[2]}
+    finally:
+        solver.set_coarse_net(None)
+        solver.restore(snap)
+
+
+def mlp_mfma(solver, net, V):
+    """Event-timed forward pass of the per-vertex MLP over the body's
+    features (the coarse-weight network's launch of every step)."""
+    import torch
+
+    X = solver.buffers["cw_x"][:V] if "cw_x" in solver.buffers else solver.buffers["ws_x"][:V]
     for _ in range(3):
         net.forward(X)
     n = 20

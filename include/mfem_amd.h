@@ -55,7 +55,9 @@ extern "C" {
 /* pcg_flags bits */
 #define MFEM_PCG_PIPELINED 1   /* one grid reduction per CG iteration (pipelined PCG) */
 #define MFEM_PCG_STREAM_ROWS 2 /* stream matrix rows from HBM instead of keeping them in VGPRs */
-#define MFEM_PCG_LEARNED_GUESS 4 /* start from the lowest-energy combination of x~ - x, the learned guess, the last three Newton steps and the previous step's same iteration (needs dx_hist) */
+#define MFEM_PCG_LEARNED_GUESS 4 /* start from the lowest-energy combination of x~ - x, the learned guess, the last three Newton steps and the previous step's same iteration (needs dx_hist; with guess null the learned guess is left out and no network runs) */
+#define MFEM_PCG_COARSE 8 /* block Jacobi plus a workgroup-local rigid-mode correction (pipelined, resident rows;
+                             see coarse_w): M^-1 = D^-1 + P (P^T H_aa P)^+ P^T per 256-row workgroup */
 
 /* status[] words */
 #define MFEM_STATUS_ERRBITS 0 /* bit 0: grid barrier timed out; every kernel of a step (the persistent
@@ -217,6 +219,28 @@ typedef struct mfem_solver {
      counterpart in the reference); null = none */
   double* dx_same;
   int32_t dx_same_iters;
+  /* Workgroup-local coarse correction (pcg_flags & MFEM_PCG_COARSE; no
+     counterpart in the reference, whose CG uses block Jacobi): each PCG
+     workgroup's 256 rows form one aggregate a, with the 6 rigid modes of its
+     vertices about their centroid as basis P_a (3 translations, 3 rotations
+     scaled by the aggregate's radius), smoothed row by row as
+     P'_i = P_i - w_i D_i^-1 (H_aa P)_i (H_aa: the matrix restricted to the
+     aggregate).  The preconditioner is
+     z = D^-1 r + P' (P'^T H_aa P')^+ P'^T r on each aggregate: symmetric
+     positive definite for any weights, no communication beyond the
+     workgroup.  w_i = coarse_w[i] (e.g. a network's output) or
+     coarse_omega for every vertex when coarse_w is null. */
+  float* coarse_w; /* [n_verts] or null (written by mfem_coarse_weights) */
+  float coarse_omega;
+  /* the network that gives coarse_w (no counterpart in the reference): at
+     the first Newton iteration of a step, when cw_wbf is set and
+     MFEM_PCG_COARSE is on, mfem_step evaluates mfem_coarse_features into
+     cw_x, the per-vertex MLP (same architecture as the warm start's, its own
+     weights) into cw_y, and coarse_w = clamp(cw_y[i][0], 0, 2) */
+  uint16_t* cw_x;          /* [n_verts][16] BF16 features */
+  float* cw_y;             /* [n_verts][3] network outputs */
+  const float* cw_w;       /* [MFEM_MLP_NPARAMS] FP32 master weights */
+  const uint16_t* cw_wbf;  /* [MFEM_MLP_NPARAMS] BF16 copy */
 } mfem_solver;
 
 /* ABI self-description (checked against the Python mirror by the tests) */
@@ -275,6 +299,18 @@ int mfem_ws_features(const mfem_solver* S, float dt, int newton_iter, void* stre
 /* guess = c3 + s (y0 c1 + y1 c2 + y2 c3) from the network outputs ws_y:
    the previous Newton step plus a learned correction. */
 int mfem_ws_apply(const mfem_solver* S, void* stream);
+
+/* Features of the coarse-weight network (no counterpart in the reference),
+   from the assembled system, per vertex i of aggregate a (the 256-row block
+   of PCG workgroup a), BF16 into S->cw_x[i][0..15]: 1, the share of the
+   row's off-diagonal block norms that couple outside a, the mass term's
+   share of tr D_i, the contact Hessian's share of tr D_i, |x_i - centroid|
+   over a's RMS radius, off-diagonal over diagonal block norms, pinned,
+   log10(tr D_i / mass term) / 4, blocks in the row / 16, blocks outside a /
+   16, contact active (distance < d1), zeros. */
+int mfem_coarse_features(const mfem_solver* S, float dt, void* stream);
+/* coarse_w[i] = clamp(cw_y[i][0], 0, 2) for the active vertices. */
+int mfem_coarse_weights(const mfem_solver* S, void* stream);
 
 /* Per-vertex MLP (16 -> 64 -> 64 -> 3, GELU, BF16 MFMA with FP32 accumulate):
    y[n][0..2] for the BF16 inputs x[n][16]. */
@@ -430,11 +466,12 @@ const char* mfem_refine_last_error(void);
 /* Persistent-PCG variants for an autotuner, no counterpart in the reference:
    info[0..4] = chunks per wave, resident row slots (0 = streamed rows),
    mode (0 standard, 1 pipelined, 2 standard re-reading the preconditioner
-   and right-hand side to run four waves per SIMD), workgroups for S,
-   applicable to S (0/1:
-   co-resident, rows fit the slots, a batch's bodies aligned to its
-   workgroups).  Reads only n_chunks, max_width, n_bodies and body_wg_align
-   of S (a host-only struct may ask before any buffer exists).
+   and right-hand side to run four waves per SIMD, 3 pipelined with the
+   workgroup-local coarse correction), workgroups for S, applicable to S
+   (0/1: co-resident, rows fit the slots, a batch's bodies aligned to its
+   workgroups, mode 3 exactly when pcg_flags has MFEM_PCG_COARSE).  Reads
+   only n_chunks, max_width, n_bodies, body_wg_align, local_sweeps and
+   pcg_flags of S (a host-only struct may ask before any buffer exists).
    Returns MFEM_EINVAL when k >= mfem_pcg_variant_count(). */
 int mfem_pcg_variant_count(void);
 /* Co-resident workgroups of the persistent line-search kernel (the bound

@@ -1072,6 +1072,51 @@ __device__ __forceinline__ void block_jacobi(const mfem_solver& S, int row, floa
   }
 }
 
+// Sums over the workgroup of NV per-lane values, in one fixed order, so that
+// every thread gets the same numbers: wave sums through DPP, then the four
+// wave sums from LDS.  buf: [kWaves][24] doubles of LDS that no other thread
+// reads until every thread has passed this call's barrier (callers alternate
+// two buffers).  Every thread of the workgroup calls it together.
+template <int NV>
+__device__ __forceinline__ void wg_sum_post(const double (&v)[NV], double (*buf)[24]) {
+  static_assert(NV <= 24, "at most 24 values");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const double w = wave_sum(v[i]);
+    if (lane == 0) buf[wave][i] = w;
+  }
+}
+template <int NV>
+__device__ __forceinline__ void wg_sum_get(double (&v)[NV], double (*buf)[24]) {
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = (buf[0][i] + buf[1][i]) + (buf[2][i] + buf[3][i]);
+}
+// (the two halves back to back; work placed between them overlaps the
+// other waves' arrival at the barrier)
+template <int NV>
+__device__ __forceinline__ void wg_sum(double (&v)[NV], double (*buf)[24]) {
+  wg_sum_post(v, buf);
+  wg_sum_get(v, buf);
+}
+
+// rigid modes of a vertex at e (position about the aggregate's centroid over
+// its radius): the 3 x 6 block [I | e_k x e], columns 3 + k = rotation about
+// axis k
+__device__ __forceinline__ void rigid_basis(float e0, float e1, float e2, float (&P)[3][6]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) P[a][j] = (j == a) ? 1.f : 0.f;
+  P[0][4] = e2;
+  P[0][5] = -e1;
+  P[1][3] = -e2;
+  P[1][5] = e0;
+  P[2][3] = e1;
+  P[2][4] = -e0;
+}
+
 // LOWREG (standard mode, streamed rows, one chunk per wave): the
 // preconditioner blocks and the right-hand side are re-read from memory where
 // used instead of held in registers and the phase clock is off, so that four
@@ -1082,6 +1127,10 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
   static_assert(!LOWREG || (CPW == 1 && RW == 0 && MODE == 0), "LOWREG: standard, streamed, one chunk per wave");
   constexpr bool PIPE = MODE != 0;
   constexpr bool PUB32 = MODE == 2;  // pipelined: publish the SpMV input in FP32
+  // pipelined, resident rows, block Jacobi plus the workgroup-local
+  // rigid-mode correction (the workgroup's 256 rows are one aggregate)
+  constexpr bool COARSE = MODE == 3;
+  static_assert(!COARSE || (CPW == 1 && RW > 0), "the coarse correction needs one chunk per wave and resident rows");
   // pipelined with resident rows: the FP64 SpMV input is staged in LDS, and
   // every FP64 publication carries its publication count in the two lowest
   // mantissa bits (a relative change below 1e-15)
@@ -1217,6 +1266,204 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
   // per-wave transpose slice for the coalesced publication of m
   __shared__ double tr[(PIPE && !PUB32) ? kWaves * 192 : 1];
   double* trw = tr + ((PIPE && !PUB32) ? (threadIdx.x >> 6) * 192 : 0);
+  // Workgroup-local coarse correction, set up once per launch (before the
+  // halo set-up, which rewrites the resident columns and reuses hv):
+  //   P_i   = rigid modes of row i about the aggregate's centroid,
+  //   P'_i  = P_i - w_i D_i^-1 (H_aa P)_i  (H_aa: columns of this workgroup),
+  //   Ac    = P'^T H_aa P' (6 x 6, FP64), cinv = its pseudo-inverse (pivoted
+  //           Cholesky, pivots <= 1e-9 of the diagonal dropped),
+  //   M r   = D^-1 r + P' cinv P'^T r.
+  // Pc holds this row's P' for the whole solve; the coarse coefficients
+  // cinv P'^T v of w, r and s are wave-uniform 6-vectors (ccw, ccr, ccs).
+  float Pc[3][6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) Pc[a][j] = 0.f;
+  double Pd[3][6];  // (Pc in FP64 for the solve: no conversion per use)
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) Pd[a][j] = 0.0;
+  __shared__ double cinv[COARSE ? 21 : 1];
+  __shared__ double cred[COARSE ? 2 : 1][kWaves][24];
+  int cpar = 0;  // which cred buffer the next workgroup sum uses
+  double ccw[6], ccr[6], ccs[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) ccw[j] = ccr[j] = ccs[j] = 0.0;
+  if constexpr (COARSE) {
+    const int row = chunk[0] * 64 + lane;
+    const bool lv = live[0];
+    const int li = (int)threadIdx.x;  // this row's place in the aggregate (one chunk per wave)
+    float xi[3] = {0.f, 0.f, 0.f};
+    if (lv) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) xi[a] = S.x[3 * row + a];
+    }
+    double m4[4] = {xi[0], xi[1], xi[2], lv ? 1.0 : 0.0};
+    wg_sum<4>(m4, cred[cpar]);
+    cpar ^= 1;
+    const double cnt = m4[3];
+    double d[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) d[a] = (lv && cnt > 0.0) ? (double)xi[a] - m4[a] / cnt : 0.0;
+    double r2[1] = {d[0] * d[0] + d[1] * d[1] + d[2] * d[2]};
+    wg_sum<1>(r2, cred[cpar]);
+    cpar ^= 1;
+    const double rad = (cnt > 0.0 && r2[0] > 0.0) ? sqrt(r2[0] / cnt) : 1.0;
+    const float e0 = (float)(d[0] / rad), e1 = (float)(d[1] / rad), e2 = (float)(d[2] / rad);
+    if (lv) rigid_basis(e0, e1, e2, Pc);
+    // the aggregate's positions, then its smoothed basis, staged in hv
+    float4* els = reinterpret_cast<float4*>(hv);
+    float* pls = reinterpret_cast<float*>(hv) + 4 * kBlock;
+    static_assert(4 * kBlock + 18 * kBlock <= 2 * kHaloLines * 8, "coarse set-up scratch fits the halo buffer");
+    els[li] = make_float4(e0, e1, e2, lv ? 1.f : 0.f);
+    // resident slot k's column inside the aggregate, or -1
+    const int cs0 = own[0] ? S.chunk_ptr[chunk[0]] : 0;
+    const int cwd = own[0] ? S.chunk_ptr[chunk[0] + 1] - cs0 : 0;
+    int lpos[RW];
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int c = rb.col[k];
+      lpos[k] = (lv && k < cwd && (c >> 8) == (int)blockIdx.x) ? (c & 255) : -1;
+    }
+    __syncthreads();
+    const float om = lv ? (S.coarse_w ? S.coarse_w[row] : S.coarse_omega) : 0.f;
+    if (om != 0.f) {
+      float HP[3][6];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) HP[a][j] = 0.f;
+#pragma unroll
+      for (int k = 0; k < RW; ++k) {
+        if (lpos[k] < 0) continue;
+        const float4 e = els[lpos[k]];
+        float Pj[3][6];
+        rigid_basis(e.x, e.y, e.z, Pj);
+        const float* v = rb.val[k];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int j = 0; j < 6; ++j)
+            HP[a][j] += e.w * (v[3 * a] * Pj[0][j] + v[3 * a + 1] * Pj[1][j] + v[3 * a + 2] * Pj[2][j]);
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+          Pc[a][j] -= om * (M[0][3 * a] * HP[0][j] + M[0][3 * a + 1] * HP[1][j] + M[0][3 * a + 2] * HP[2][j]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) pls[li * 18 + 6 * a + j] = Pc[a][j];
+    __syncthreads();
+    float HQ[3][6];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) HQ[a][j] = 0.f;
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      if (lpos[k] < 0) continue;
+      const float* q = pls + lpos[k] * 18;
+      const float* v = rb.val[k];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) HQ[a][j] += v[3 * a] * q[j] + v[3 * a + 1] * q[6 + j] + v[3 * a + 2] * q[12 + j];
+    }
+    double ac[21];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = i; j < 6; ++j)
+        ac[sym21(i, j)] = (double)Pc[0][i] * HQ[0][j] + (double)Pc[1][i] * HQ[1][j] + (double)Pc[2][i] * HQ[2][j];
+    wg_sum<21>(ac, cred[cpar]);
+    cpar ^= 1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) Pd[a][j] = (double)Pc[a][j];
+    if (threadIdx.x == 0) {
+      double A[6][6], L[6][6], Li[6][6];
+      bool kept[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+          A[i][j] = ac[sym21(i, j)];
+          L[i][j] = 0.0;
+        }
+      for (int j = 0; j < 6; ++j) {
+        double dj = A[j][j];
+        for (int q = 0; q < j; ++q) dj -= L[j][q] * L[j][q];
+        kept[j] = A[j][j] > 0.0 && dj > 1e-9 * A[j][j];
+        L[j][j] = kept[j] ? sqrt(dj) : 1.0;
+        for (int i = j + 1; i < 6; ++i) {
+          double v = A[j][i];
+          for (int q = 0; q < j; ++q) v -= L[i][q] * L[j][q];
+          L[i][j] = kept[j] ? v / L[j][j] : 0.0;
+        }
+      }
+      for (int i = 0; i < 6; ++i)  // a dropped row takes no part either
+        if (!kept[i])
+          for (int q = 0; q < i; ++q) L[i][q] = 0.0;
+      for (int c = 0; c < 6; ++c)
+        for (int i = 0; i < 6; ++i) {
+          double v = (i == c) ? 1.0 : 0.0;
+          for (int q = 0; q < i; ++q) v -= L[i][q] * Li[q][c];
+          Li[i][c] = v / L[i][i];
+        }
+      for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 6; ++j) {
+          double v = 0.0;
+          if (kept[i] && kept[j])
+            for (int q = 0; q < 6; ++q) v += kept[q] ? Li[q][i] * Li[q][j] : 0.0;
+          cinv[sym21(i, j)] = v;
+        }
+    }
+    __syncthreads();
+  }
+  // coarse coefficients cinv P'^T v of a per-row vector v (all threads
+  // together), and M v = D^-1 v + P' y with such coefficients y
+  // (in two halves: coarse_post sends this lane's share P'^T v, coarse_get
+  // waits for the workgroup and forms cinv P'^T v)
+  auto coarse_post = [&](const double (&v)[3]) {
+    double t[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) t[j] = Pd[0][j] * v[0] + Pd[1][j] * v[1] + Pd[2][j] * v[2];
+    wg_sum_post<6>(t, cred[cpar]);
+  };
+  auto coarse_get = [&](double (&y)[6]) {
+    double t[6];
+    wg_sum_get<6>(t, cred[cpar]);
+    cpar ^= 1;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc += cinv[sym21(i, j)] * t[j];
+      y[i] = acc;
+    }
+  };
+  auto coarse_coef = [&](const double (&v)[3], double (&y)[6]) {
+    coarse_post(v);
+    coarse_get(y);
+  };
+  auto apply_Mc = [&](const double (&v)[3], const double (&y)[6], double (&out)[3]) {
+    apply_M<CPW, double>(M, 0, v, out);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      out[a] += Pd[a][0] * y[0] + Pd[a][1] * y[1] + Pd[a][2] * y[2] + Pd[a][3] * y[3] + Pd[a][4] * y[4] +
+                Pd[a][5] * y[5];
+  };
+  (void)coarse_coef;
+  (void)coarse_post;
+  (void)coarse_get;
+  (void)apply_Mc;
+
   if constexpr (HALO) {
     uint32_t* bm = reinterpret_cast<uint32_t*>(hv);
     const long long n_lines = (long long)nrow_pad * 3 / 8;
@@ -1559,11 +1806,13 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         for (int a = 0; a < 3; ++a) w[k][a] = own[k] ? (double)y[k][a] : 0.0;
     } else if constexpr (PIPE) {
       double* ub = pxbuf(pubx);
+      if constexpr (COARSE) coarse_coef(r[0], ccr);
 #pragma unroll
       for (int k = 0; k < CPW; ++k) {
         if (!own[k]) continue;
         const int row = chunk[k] * 64 + lane;
-        apply_M<CPW, double>(M, k, r[k], u[k]);
+        if constexpr (COARSE) apply_Mc(r[k], ccr, u[k]);
+        else apply_M<CPW, double>(M, k, r[k], u[k]);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           pub_store(&ub[3 * row + a], tagx(u[k][a], pubx));
@@ -1575,6 +1824,11 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       if (!grid_reduce(gs, red2)) return false;
       rr = red2[0];
       spmv_resident64(ub, w);
+      if constexpr (COARSE) {
+        coarse_coef(w[0], ccw);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) ccs[j] = 0.0;
+      }
     } else {
       float* zb = pzbuf(pub);
       precondition();
@@ -1612,7 +1866,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
 #pragma unroll
         for (int a = 0; a < 3; ++a) r[k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
     }
-    if ((S.pcg_flags & MFEM_PCG_LEARNED_GUESS) && S.guess && S.dx_hist) {
+    if ((S.pcg_flags & MFEM_PCG_LEARNED_GUESS) && S.dx_hist) {
       // Learned warm start, Galerkin form.  Six candidate starts c_0..c_5:
       // x~ - x (the plain guess), the learned guess, the previous Newton step,
       // the two before it, and this Newton iteration's solution in the
@@ -1634,7 +1888,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         const int row = chunk[k] * 64 + lane;
         if (!live[k]) return 0.0;
         if (i == 0) return x[k][a];
-        if (i == 1) return (double)S.guess[3 * row + a];
+        if (i == 1) return S.guess ? (double)S.guess[3 * row + a] : 0.0;  // (no network: dropped as empty)
         if (i == 2) return S.dx[3 * row + a];
         if (i == 3) return h0[3 * row + a];
         if (i == 4) return h1[3 * row + a];
@@ -1850,7 +2104,8 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
           if (!own[k] || pre) continue;  // (uniform over the wave)
           const int row = chunk[k] * 64 + lane;
           double m[3];
-          apply_M<CPW, double>(M, k, w[k], m);
+          if constexpr (COARSE) apply_Mc(w[k], ccw, m);
+          else apply_M<CPW, double>(M, k, w[k], m);
           if constexpr (PUB32) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) pub_store(&mb32[3 * row + a], (float)m[a]);
@@ -1873,6 +2128,9 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
           if constexpr (PUB32) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) ul[a] = u[k][a];
+          } else if constexpr (COARSE) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) ul[a] = u[k][a];  // (M r, carried)
           } else {
             apply_M<CPW, double>(M, k, r[k], ul);
           }
@@ -1975,7 +2233,70 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         // the next iteration's carried scalars, computed here so that the
         // division overlaps the vector updates
         const double ig_next = 1.0 / gamma, P_next = gamma * alpha;
-        if constexpr (!PUB32) {
+        if constexpr (COARSE) {
+          // as below, with the coarse coefficients: those of the new w from
+          // one workgroup sum, those of s and r by the same recurrences as
+          // the vectors (linear: c(s') = c(w) + beta c(s), c(r') = c(r) -
+          // alpha c(s')).  The new w's share of the sum goes out first, the
+          // other recurrences and dot products run while the other waves
+          // arrive, then m = M w is published.  Every wave takes part.
+          double* mb = pxbuf(pubx);
+          double g2 = 0.0, d2 = 0.0, r2 = 0.0;
+          double wo[3], mn[3], ul[3], cwn[6], csn[6], crn[6];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            wo[a] = w[0][a];
+            zd[0][a] = n[0][a] + beta * zd[0][a];
+            w[0][a] -= alpha * zd[0][a];
+          }
+          coarse_post(w[0]);
+          // u = M r of the current r is carried (set by the restart and by
+          // the previous update): one application of M fewer per iteration
+#pragma unroll
+          for (int j = 0; j < 6; ++j) {
+            csn[j] = ccw[j] + beta * ccs[j];
+            crn[j] = ccr[j] - alpha * csn[j];
+          }
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            sd[0][a] = wo[a] + beta * sd[0][a];
+            pd[0][a] = u[0][a] + beta * pd[0][a];
+            x[0][a] += alpha * pd[0][a];
+            r[0][a] -= alpha * sd[0][a];
+          }
+          apply_Mc(r[0], crn, ul);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) u[0][a] = ul[a];
+          if (own[0]) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              g2 += r[0][a] * ul[a];
+              d2 += w[0][a] * ul[a];
+              r2 += r[0][a] * r[0][a];
+            }
+          }
+          coarse_get(cwn);
+          if (own[0]) {
+            apply_Mc(w[0], cwn, mn);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) trw[3 * lane + a] = mn[a];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int j = 0; j < 3; ++j) pub_store(&mb[(size_t)chunk[0] * 192 + lane + 64 * j], tagx(trw[lane + 64 * j], pubx));
+            __builtin_amdgcn_wave_barrier();
+          }
+#pragma unroll
+          for (int j = 0; j < 6; ++j) {
+            ccw[j] = cwn[j];
+            ccs[j] = csn[j];
+            ccr[j] = crn[j];
+          }
+          ++pubx;
+          pre = true;
+          gamma_pre = g2;
+          delta_pre = d2;
+          rr_pre = r2;
+        } else if constexpr (!PUB32) {
           // w first; the next m = M w goes out at once, and the remaining
           // recurrences and the next dot products run while it drains
           double* mb = pxbuf(pubx);
@@ -2274,6 +2595,82 @@ __global__ void __launch_bounds__(kBlock) k_ws_apply(mfem_solver S) {
 #pragma unroll
   for (int d = 0; d < 3; ++d)
     S.guess[3 * i + d] = a[9] * ((1.f + y[2]) * a[6 + d] + y[0] * a[d] + y[1] * a[3 + d]);
+}
+
+// ---------------------------------------------------------------------------
+// features and weights of the coarse-weight network (one workgroup per
+// aggregate: rows [256 b, 256 b + 256), the PCG workgroup's rows)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_coarse_features(mfem_solver S, float dt) {
+  __shared__ double red[2][kWaves][24];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const bool lv = i < S.n_verts;
+  float xi[3] = {0.f, 0.f, 0.f};
+  if (lv) load3(S.x, i, xi);
+  double m4[4] = {xi[0], xi[1], xi[2], lv ? 1.0 : 0.0};
+  wg_sum<4>(m4, red[0]);
+  const double cnt = m4[3];
+  double d[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) d[a] = (lv && cnt > 0.0) ? (double)xi[a] - m4[a] / cnt : 0.0;
+  double r2[1] = {d[0] * d[0] + d[1] * d[1] + d[2] * d[2]};
+  wg_sum<1>(r2, red[1]);
+  if (!lv) return;  // (the network runs over the active vertices only)
+  const double rad = (cnt > 0.0 && r2[0] > 0.0) ? sqrt(r2[0] / cnt) : 1.0;
+  const int dl = S.diag_lane[i];
+  float D[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) D[q] = S.sell_val[val_index(dl, q)];
+  const float trD = D[0] + D[4] + D[8];
+  float nD = 0.f;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) nD += D[q] * D[q];
+  nD = sqrtf(nD);
+  const float im = S.inv_mass[i];
+  const float md = (im == 0.f ? 1.f + S.attach_k : 1.f / im) / (dt * dt);
+  const float trHv = S.Hv[9 * i] + S.Hv[9 * i + 4] + S.Hv[9 * i + 8];
+  const int c = i >> 6, lane = i & 63;
+  float off_all = 0.f, off_out = 0.f;
+  int nblk = 0, nout = 0;
+  for (int sl = S.chunk_ptr[c]; sl < S.chunk_ptr[c + 1]; ++sl) {
+    const int col = S.sell_col[sl * 64 + lane];
+    const float* vb = S.sell_val + (size_t)sl * 576 + lane;
+    float nb = 0.f;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) nb += vb[64 * q] * vb[64 * q];
+    if (!(nb > 0.f) || col == i) continue;
+    nb = sqrtf(nb);
+    const bool out = (col >> 8) != (i >> 8);
+    off_all += nb;
+    ++nblk;
+    if (out) {
+      off_out += nb;
+      ++nout;
+    }
+  }
+  const float tiny = 1e-30f;
+  float f[16];
+  f[0] = 1.f;
+  f[1] = off_out / fmaxf(off_all, tiny);
+  f[2] = 3.f * md / fmaxf(trD, tiny);
+  f[3] = trHv / fmaxf(trD, tiny);
+  f[4] = (float)(sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) / rad);
+  f[5] = off_all / fmaxf(nD, tiny);
+  f[6] = im == 0.f ? 1.f : 0.f;
+  f[7] = 0.25f * log10f(fmaxf(trD, tiny) / fmaxf(3.f * md, tiny));
+  f[8] = (float)(nblk + 1) / 16.f;
+  f[9] = (float)nout / 16.f;
+  f[10] = (S.n_shapes > 0 && S.dist[i] < S.d1) ? 1.f : 0.f;
+#pragma unroll
+  for (int k = 11; k < 16; ++k) f[k] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) S.cw_x[16 * (size_t)i + k] = f2bf16(f[k]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_coarse_weights(mfem_solver S) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= S.n_verts) return;
+  S.coarse_w[i] = fminf(fmaxf(S.cw_y[3 * i], 0.f), 2.f);
 }
 
 // ---------------------------------------------------------------------------
@@ -2585,7 +2982,7 @@ __global__ void __launch_bounds__(kBlock) k_merit(mfem_solver S, float dt, float
 namespace {
 
 struct PcgVariant {
-  int cpw, rw, mode;  // mode 0 standard, 1 pipelined, 2 standard with four waves per SIMD
+  int cpw, rw, mode;  // mode 0 standard, 1 pipelined, 2 standard with four waves per SIMD, 3 pipelined with the coarse correction
   const void* fn;
 };
 #define PCG_V(C, R, P) {C, R, P, reinterpret_cast<const void*>(&k_pcg<C, R, P>)}
@@ -2603,6 +3000,9 @@ const PcgVariant kPcgVariants[] = {
     // rows of at most 15 blocks (valence <= 14 in a tetrahedral mesh, the
     // common case): no padding slot in every row's SpMV
     PCG_V(1, 15, 1),
+    // block Jacobi plus the workgroup-local rigid-mode correction
+    // (MFEM_PCG_COARSE): pipelined, resident rows
+    PCG_V(1, 16, 3), PCG_V(1, 15, 3),
 };
 #undef PCG_V
 constexpr int kNumPcgVariants = sizeof(kPcgVariants) / sizeof(kPcgVariants[0]);
@@ -2686,7 +3086,7 @@ int launched() {
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
   X(lag_sid) X(G) X(Hl) X(gl) X(Hsinv) X(gs) X(psi) X(Kb) X(fe) X(gx) X(gB) X(gD) X(Hv) X(dist) X(eB)      \
-  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega) X(dx_hist) X(dx_same) X(dx_same_iters)
+  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega) X(dx_hist) X(dx_same) X(dx_same_iters) X(coarse_w) X(coarse_omega) X(cw_x) X(cw_y) X(cw_w) X(cw_wbf)
 
 extern "C" {
 
@@ -2774,8 +3174,10 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
     }
     const PcgVariant& v = kPcgVariants[k];
     const bool want = (S->pcg_flags & MFEM_PCG_PIPELINED) != 0;
-    if ((v.mode == 1) != want) continue;
-    if (v.mode == 1 && S->local_sweeps > 0) continue;  // the local preconditioner is standard-mode only
+    const bool coarse = (S->pcg_flags & MFEM_PCG_COARSE) != 0;
+    if ((v.mode == 3) != coarse) continue;  // the coarse correction: its own variants, and only they
+    if (!coarse && (v.mode == 1) != want) continue;
+    if ((v.mode == 1 || v.mode == 3) && S->local_sweeps > 0) continue;  // the local preconditioner is standard-mode only
     if (S->n_bodies > 1 && S->body_wg_align % v.cpw != 0) continue;  // a batch: body-aligned workgroups
     if (v.rw > 0 && ((S->pcg_flags & MFEM_PCG_STREAM_ROWS) || S->max_width > v.rw)) continue;
     long waves = (S->n_chunks + v.cpw - 1) / v.cpw;
@@ -2836,11 +3238,19 @@ int mfem_step(const mfem_solver* S, float dt, void* stream) {
   for (int it = 0; it < S->iterations; ++it) {
     if (int rc = mfem_update_system(S, dt, stream)) return rc;
     if (int rc = mfem_assemble(S, dt, stream)) return rc;
-    if ((S->pcg_flags & MFEM_PCG_LEARNED_GUESS) && S->guess) {
+    if ((S->pcg_flags & MFEM_PCG_LEARNED_GUESS) && S->guess && S->mlp_wbf && S->mlp_w) {
       if (int rc = mfem_ws_features(S, dt, it, stream)) return rc;
       if (int rc = mfem_mlp_forward(S->ws_x, S->n_verts, S->mlp_wbf, S->mlp_w, S->ws_y, stream))
         return fail(rc, std::string("mfem_mlp_forward: ") + mfem_mlp_last_error());
       if (int rc = mfem_ws_apply(S, stream)) return rc;
+    }
+    if (it == 0 && (S->pcg_flags & MFEM_PCG_COARSE) && S->cw_wbf && S->cw_w && S->coarse_w) {
+      // the coarse-weight network, once per step (its features come from
+      // the first Newton iteration's system)
+      if (int rc = mfem_coarse_features(S, dt, stream)) return rc;
+      if (int rc = mfem_mlp_forward(S->cw_x, S->n_verts, S->cw_wbf, S->cw_w, S->cw_y, stream))
+        return fail(rc, std::string("mfem_mlp_forward: ") + mfem_mlp_last_error());
+      if (int rc = mfem_coarse_weights(S, stream)) return rc;
     }
     if (int rc = mfem_global_solve(S, it, stream)) return rc;
     if (int rc = mfem_local_solve(S, stream)) return rc;
@@ -2869,7 +3279,9 @@ int mfem_pcg_variant_info(const mfem_solver* S, int k, int32_t* info) {
   info[2] = v.mode;
   info[3] = (int32_t)nb;
   info[4] = (nb <= ci->max_blocks_pcg[k] && nb <= MFEM_RED_MAX_BLOCKS && (v.rw == 0 || S->max_width <= v.rw) &&
-             (S->n_bodies <= 1 || S->body_wg_align % v.cpw == 0) && !(v.mode == 1 && S->local_sweeps > 0))
+             (S->n_bodies <= 1 || S->body_wg_align % v.cpw == 0) &&
+             !((v.mode == 1 || v.mode == 3) && S->local_sweeps > 0) &&
+             (v.mode == 3) == ((S->pcg_flags & MFEM_PCG_COARSE) != 0))
                 ? 1
                 : 0;
   return MFEM_OK;
@@ -2880,6 +3292,21 @@ int mfem_ws_features(const mfem_solver* S, float dt, int newton_iter, void* stre
   if (!S->ws_x || !S->ws_aux) return fail(MFEM_EINVAL, "warm-start buffers not set");
   hipLaunchKernelGGL(k_ws_features, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, (hipStream_t)stream, *S, dt,
                      newton_iter);
+  return launched();
+}
+
+int mfem_coarse_features(const mfem_solver* S, float dt, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  if (!S->cw_x) return fail(MFEM_EINVAL, "mfem_coarse_features: cw_x is null");
+  if (!(dt > 0.f)) return fail(MFEM_EINVAL, "dt must be > 0");
+  hipLaunchKernelGGL(k_coarse_features, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, (hipStream_t)stream, *S, dt);
+  return launched();
+}
+
+int mfem_coarse_weights(const mfem_solver* S, void* stream) {
+  if (int rc = check_solver(S)) return rc;
+  if (!S->cw_y || !S->coarse_w) return fail(MFEM_EINVAL, "mfem_coarse_weights: cw_y / coarse_w is null");
+  hipLaunchKernelGGL(k_coarse_weights, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, (hipStream_t)stream, *S);
   return launched();
 }
 

@@ -31,7 +31,9 @@ FIELDS = [
     "tcon_ptr", "tcon_src", "vtri_ptr", "vtri_src", "tri_dist", "tri_bary", "tri_nrm", "tri_sid",
 )] + [("n_bodies", c_i32), ("ls_blocks", c_i32)] + [(n, c_ptr) for n in (
     "body_wg", "body_vert", "body_tet", "body_tri", "body_ls_wg", "body_epoch", "tet_energy",
-)] + [("body_wg_align", c_i32), ("local_sweeps", c_i32), ("local_omega", c_f32), ("dx_hist", c_ptr), ("dx_same", c_ptr), ("dx_same_iters", c_i32)]
+)] + [("body_wg_align", c_i32), ("local_sweeps", c_i32), ("local_omega", c_f32), ("dx_hist", c_ptr), ("dx_same", c_ptr), ("dx_same_iters", c_i32),
+   ("coarse_w", c_ptr), ("coarse_omega", c_f32), ("cw_x", c_ptr), ("cw_y", c_ptr), ("cw_w", c_ptr),
+   ("cw_wbf", c_ptr)]
 
 
 class MfemSolverStruct(ctypes.Structure):
@@ -43,7 +45,7 @@ MFEM_STATS_PER_ITER = 16
 MFEM_MAX_STATS_ITERS = 64
 MFEM_RED_MAX_BLOCKS = 2048
 SHAPE_PLANE, SHAPE_CAPSULE = 0, 1
-PCG_PIPELINED, PCG_STREAM_ROWS, PCG_LEARNED_GUESS = 1, 2, 4
+PCG_PIPELINED, PCG_STREAM_ROWS, PCG_LEARNED_GUESS, PCG_COARSE = 1, 2, 4, 8
 MFEM_MLP_NPARAMS = 6288
 MFEM_SCAN_NOALLOC_MAX = 1 << 26
 
@@ -58,7 +60,8 @@ ENTRY_POINTS = [
     "mfem_refine_workspace_bytes", "mfem_refine_select", "mfem_refine_count", "mfem_refine_apply",
     "mfem_refine_last_error", "mfem_pcg_variant_count", "mfem_pcg_variant_info", "mfem_line_search_max_blocks",
     "mfem_contact_distance", "mfem_pattern_workspace_bytes", "mfem_pattern_count", "mfem_pattern_fill",
-    "mfem_pattern_last_error", "mfem_pose_shapes", "mfem_scan_i32_noalloc",
+    "mfem_pattern_last_error", "mfem_pose_shapes", "mfem_scan_i32_noalloc", "mfem_coarse_features",
+    "mfem_coarse_weights",
 ]
 
 
@@ -101,6 +104,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mfem_pattern_last_error.restype = ctypes.c_char_p
     lib.mfem_ws_features.argtypes = [P, ctypes.c_float, ctypes.c_int, vp]
     lib.mfem_ws_apply.argtypes = [P, vp]
+    lib.mfem_coarse_features.argtypes = [P, ctypes.c_float, vp]
+    lib.mfem_coarse_weights.argtypes = [P, vp]
     lib.mfem_mlp_forward.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
     lib.mfem_mlp_train_step.argtypes = [vp, vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp, vp, vp,
                                         ctypes.c_float, ctypes.c_int, vp]

@@ -217,13 +217,15 @@ class WarmStartNet:
         """fit_solves on the initial-residual objective log |g - H dx0|^2."""
         return self.fit_solves(solves, solver, "residual", epochs, lr, seed)
 
-    def validate_cg(self, solver, state, dt: float, steps: int = 2) -> dict:
+    def validate_cg(self, solver, state, dt: float, steps: int = 2, margin: float = 0.0) -> dict:
         """Model selection on the quantity the guess is for: whole steps from
         ``state`` (the solver's own state restored before each run) with
         these weights and with the network's correction off (zero output
         layer: the previous Newton step, the same fallback); keeps the
-        weights with fewer CG iterations (zeroing the output layer when the
-        correction does not help).  Returns both counts and the choice."""
+        weights only if they take fewer than (1 - margin) times the CG
+        iterations of the correction off (zeroing the output layer
+        otherwise: margin covers the cost of the network's own launches).
+        Returns both counts and the choice."""
         snap = solver.snapshot()
         counts = {}
         trained = self.w.clone()
@@ -245,10 +247,10 @@ class WarmStartNet:
         finally:
             solver.restore(snap)
             solver.set_warm_start(None)
-        keep = "trained" if counts["trained"] < counts["prev_newton_step"] else "prev_newton_step"
+        keep = "trained" if counts["trained"] < (1.0 - margin) * counts["prev_newton_step"] else "prev_newton_step"
         self.w.copy_(trained if keep == "trained" else zero)
         self.refresh()
-        return {"cg_iters": counts, "kept": keep, "steps": steps}
+        return {"cg_iters": counts, "kept": keep, "steps": steps, "margin": margin}
 
     def loss(self, X, aux) -> float:
         out = torch.zeros(1, dtype=torch.float32, device=self.device)

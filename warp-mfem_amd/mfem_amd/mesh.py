@@ -96,6 +96,36 @@ def implicit_tet_mesh(inside, lo, hi, h: float):
     return lattice_tet_mesh(mask, lo, h)
 
 
+def morton_order(points: np.ndarray, bits: int = 10) -> np.ndarray:
+    """Permutation sorting points along a Z-order (Morton) curve of a
+    2^bits lattice over their bounding cube (ties keep their order)."""
+    p = np.asarray(points, dtype=np.float64)
+    lo = p.min(axis=0)
+    ext = float((p.max(axis=0) - lo).max()) or 1.0
+    q = np.minimum(((p - lo) / ext * (1 << bits)).astype(np.int64), (1 << bits) - 1)
+    code = np.zeros(len(p), dtype=np.int64)
+    for b in range(bits):
+        for k in range(3):
+            code |= ((q[:, k] >> b) & 1) << (3 * b + k)
+    return np.argsort(code, kind="stable")
+
+
+def spatial_order(particles: np.ndarray, tets: np.ndarray):
+    """Renumbers a tet mesh for locality: vertices along a Morton curve, tets
+    by their lowest new vertex (corner order, hence orientation, kept).
+    Consecutive blocks of vertices are then compact in space, which the
+    solver's workgroup-local coarse correction uses as aggregates (each PCG
+    workgroup's 256 rows).  Returns (particles, tets, perm) with
+    new particles = old particles[perm]."""
+    particles = np.asarray(particles)
+    perm = morton_order(particles)
+    inv = np.empty(len(perm), dtype=np.int64)
+    inv[perm] = np.arange(len(perm))
+    t = inv[np.asarray(tets, dtype=np.int64)]
+    t = t[np.argsort(t.min(axis=1), kind="stable")]
+    return particles[perm], t.astype(np.int32), perm
+
+
 def rest_pose(particles: np.ndarray, tets: np.ndarray):
     """DmInv (T,3,3) and signed rest volume (T,) of each tet."""
     x = particles[tets]  # (T,4,3)

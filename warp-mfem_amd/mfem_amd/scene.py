@@ -205,6 +205,7 @@ def octopus_scene(h: float = 7.3e-3, seed: int = 7, clearance: float = 2.0e-3, d
     lo = np.array([-0.72, -0.72, -0.001])
     hi = np.array([0.72, 0.72, 0.24])
     p, t = _mesh.implicit_tet_mesh(octopus_inside, lo, hi, h)
+    p, t, _ = _mesh.spatial_order(p, t)  # compact 256-vertex blocks (the PCG's local aggregates)
     p[:, 2] += clearance - p[:, 2].min()
     cen = p[t].mean(axis=1)
     g = gaussian_field(cen, 0.05, seed)
@@ -260,6 +261,7 @@ def soft_body_batch(n_bodies: int = 64, seed: int = 5, tets_min: float = 2.0e3, 
         else:
             cells = np.maximum(np.round(dims / h).astype(int), 1)
             p, t = _mesh.box_tet_mesh(-0.5 * dims, 0.5 * dims, cells)
+        p, t, _ = _mesh.spatial_order(p, t)
         energy = "neohookean" if rng.integers(2) else "arap"
         E = float(np.exp(rng.uniform(np.log(1.0e4), np.log(1.0e6))))
         mu, lam = _mesh.lame(E, 0.45)

@@ -251,6 +251,11 @@ class RefinementSolver:
         S.pcg_variant = -1  # chosen from pcg_flags until autotune() picks one
         self.set_local_preconditioner(int(kwargs.get("pcg_local_sweeps", 0)),
                                       float(kwargs.get("pcg_local_omega", 0.5)), S)
+        # workgroup-local rigid-mode correction on top of block Jacobi (no
+        # counterpart in the reference): off unless asked for
+        S.coarse_omega = float(kwargs.get("pcg_coarse_omega", 0.0))
+        if kwargs.get("pcg_coarse", False):
+            S.pcg_flags |= _ffi.PCG_COARSE | _ffi.PCG_PIPELINED
         S.energy = _ENERGIES[energy]
         S.iterations = self.iterations
         S.dx_same_iters = self.iterations
@@ -285,6 +290,8 @@ class RefinementSolver:
         self._set_topology(nv, nt, len(tris))
         self.warm_start_net = None
         self.set_warm_start(kwargs.get("warm_start"))
+        self.coarse_net = None
+        self._coarse_w = None
 
         # AdditionalState.from_model: rest stretch from the model's positions
         x0 = b["rest"].clone()
@@ -485,12 +492,27 @@ class RefinementSolver:
                   ctypes.c_void_p(b["shape_body"].data_ptr()), ctypes.c_void_p(b["shape_local"].data_ptr()),
                   ctypes.c_int(self._S.n_shapes), ctypes.c_void_p(b["shape_xf"].data_ptr()), self._stream())
 
+    def set_galerkin_start(self, on: bool = True) -> None:
+        """The Galerkin start without a network: the lowest quadratic-model
+        energy over x~ - x, the last three Newton steps and the same Newton
+        iteration's solution one step earlier (no learned guess, no MLP
+        launch).  A warm-start network attached later replaces it."""
+        self.warm_start_net = None
+        S = self._S
+        S.mlp_w = S.mlp_wbf = None
+        S.guess = None
+        if on:
+            S.pcg_flags |= _ffi.PCG_LEARNED_GUESS
+        else:
+            S.pcg_flags &= ~_ffi.PCG_LEARNED_GUESS
+
     def set_warm_start(self, net) -> None:
         """Attach a trained ``learned.WarmStartNet`` (or None for plain PCG).
         With a network, every PCG solve first evaluates it and starts from the
         lowest quadratic-model energy over the span of x~ - x, the learned
         guess and the last three Newton steps (a Galerkin start)."""
         self.warm_start_net = net
+        self._S.guess = self.buffers["guess"].data_ptr()
         if net is None:
             self._S.pcg_flags &= ~_ffi.PCG_LEARNED_GUESS
             self._S.mlp_w = self._S.mlp_wbf = None
@@ -558,6 +580,8 @@ class RefinementSolver:
             timed("assemble", self.assemble, dt)
             if self.warm_start_net is not None:
                 timed("warm_start", self.warm_start, dt, it)
+            if it == 0 and self.coarse_net is not None and (self._S.pcg_flags & _ffi.PCG_COARSE):
+                timed("coarse_net", self.coarse_weights_from_net, dt)
             timed("global_solve", self.global_solve, it)
             timed("local_solve", self.local_solve)
             timed("line_search", self.line_search, dt, it)
@@ -657,8 +681,82 @@ class RefinementSolver:
         for k in range(n):
             _ffi.call("mfem_pcg_variant_info", ctypes.byref(self._S), ctypes.c_int(k), info)
             out.append(dict(variant=k, chunks_per_wave=info[0], resident_slots=info[1],
-                            mode=("standard", "pipelined", "standard_lowreg")[info[2]], workgroups=info[3], applicable=bool(info[4])))
+                            mode=("standard", "pipelined", "standard_lowreg", "pipelined_coarse")[info[2]],
+                            workgroups=info[3], applicable=bool(info[4])))
         return out
+
+    def set_coarse_correction(self, on: bool, omega: float | None = None, weights: torch.Tensor | None = None) -> None:
+        """Block Jacobi plus the workgroup-local rigid-mode correction (on) or
+        block Jacobi alone (off).  Each 256-row PCG workgroup is one
+        aggregate whose rigid modes, smoothed per vertex i by
+        P'_i = P_i - w_i D_i^-1 (H_aa P)_i, span a local coarse space solved
+        exactly: M^-1 = D^-1 + P' (P'^T H_aa P')^+ P'^T.  w_i = weights[i]
+        (a device float32 tensor of n_verts, e.g. a network's output) or the
+        uniform ``omega``.  The choice of PCG variant restarts."""
+        S = self._S
+        if on:
+            S.pcg_flags |= _ffi.PCG_COARSE | _ffi.PCG_PIPELINED
+        else:
+            S.pcg_flags &= ~_ffi.PCG_COARSE
+        if omega is not None:
+            S.coarse_omega = float(omega)
+        if weights is None:
+            self._coarse_w = None
+            S.coarse_w = self.buffers["coarse_w_net"].data_ptr() if self.coarse_net is not None else None
+        else:
+            if weights.dtype != torch.float32 or weights.device != self.device or weights.numel() < S.n_verts:
+                raise ValueError("coarse weights must be a float32 tensor of n_verts on the solver's device")
+            self._coarse_w = weights
+            S.coarse_w = weights.data_ptr()
+        S.pcg_variant = -1
+
+    def _ensure_coarse_buffers(self) -> None:
+        b, dev, Vc = self.buffers, self.device, self.max_particles
+        if "cw_x" not in b:
+            b["cw_x"] = torch.zeros(Vc, 16, dtype=torch.int16, device=dev)
+            b["cw_y"] = torch.zeros(Vc, 3, dtype=torch.float32, device=dev)
+            b["coarse_w_net"] = torch.zeros(Vc, dtype=torch.float32, device=dev)
+            self._S.cw_x = b["cw_x"].data_ptr()
+            self._S.cw_y = b["cw_y"].data_ptr()
+
+    def coarse_features(self, dt: float) -> None:
+        """The coarse-weight network's features of the assembled system into
+        buffers['cw_x']."""
+        self._ensure_coarse_buffers()
+        self._call("mfem_coarse_features", ctypes.c_float(dt), self._stream())
+
+    def set_coarse_net(self, net) -> None:
+        """Attach a ``coarse.CoarseWeightNet`` (or None): with a network, the
+        coarse correction is on and its smoothing weights are the network's,
+        evaluated by every step at its first Newton iteration; None leaves
+        the correction's switch as it is with the uniform weight."""
+        S = self._S
+        self.coarse_net = net
+        if net is None:
+            S.cw_w = S.cw_wbf = None
+            if self._coarse_w is None:
+                S.coarse_w = None
+            return
+        if net.device != self.device:
+            raise ValueError("the coarse-weight network must live on the solver's device")
+        self._ensure_coarse_buffers()
+        S.cw_w = net.w.data_ptr()
+        S.cw_wbf = net.wbf.data_ptr()
+        S.coarse_w = self.buffers["coarse_w_net"].data_ptr()
+        S.pcg_flags |= _ffi.PCG_COARSE | _ffi.PCG_PIPELINED
+
+    def coarse_weights_from_net(self, dt: float) -> None:
+        """Features, network and weights (what mfem_step does at the first
+        Newton iteration of a step with a coarse-weight network)."""
+        net = self.coarse_net
+        self.coarse_features(dt)
+        b = self.buffers
+        rc = self.lib.mfem_mlp_forward(ctypes.c_void_p(b["cw_x"].data_ptr()), ctypes.c_int(self.active_particle_count),
+                                       ctypes.c_void_p(net.wbf.data_ptr()), ctypes.c_void_p(net.w.data_ptr()),
+                                       ctypes.c_void_p(b["cw_y"].data_ptr()), self._stream())
+        if rc != 0:
+            raise RuntimeError(f"mfem_mlp_forward failed ({rc}): {self.lib.mfem_mlp_last_error().decode()}")
+        self._call("mfem_coarse_weights", self._stream())
 
     def set_local_preconditioner(self, sweeps: int, omega: float = 0.5, _S=None) -> None:
         """Preconditioner of the standard PCG: sweeps = 0 is the reference's
