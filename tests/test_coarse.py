@@ -89,3 +89,79 @@ def test_coarse_cuts_octopus_cg_iterations(gpu):
         err = np.abs(res[label][1] - xb).max() / np.abs(xb).max()
         print(label, "positions vs block Jacobi", err)
         assert err <= 1e-4
+
+
+def _bf16(bits: torch.Tensor) -> np.ndarray:
+    return (bits.to(torch.int32) << 16).view(torch.float32).cpu().numpy() if bits.dtype == torch.int32 else \
+        bits.view(torch.bfloat16).float().cpu().numpy()
+
+
+def test_coarse_features_match_restatement(gpu):
+    """The coarse-weight network's features against a numpy restatement from
+    the assembled system (BF16 rounding: 1/128 relative)."""
+    m = scene.drop_box_scene(cells=(10, 6, 6))
+    solver, state = _make(m, gpu, pcg_coarse=True)
+    dt = 0.01
+    solver.bind(state)
+    solver.begin_step(dt)
+    solver.update_system(dt)
+    solver.assemble(dt)
+    solver.coarse_features(dt)
+    torch.cuda.synchronize()
+    n = m.particle_count
+    F = _bf16(solver.buffers["cw_x"][:n])
+    H = solver.system_csr().tobsr(blocksize=(3, 3))
+    x = state.particle_q[:n].double().cpu().numpy()
+    Hv = solver.buffers["Hv"][:n].double().cpu().numpy().reshape(n, 3, 3)
+    dist = solver.buffers["dist"][:n].double().cpu().numpy()
+    im = solver.buffers["inv_mass"][:n].double().cpu().numpy()
+    md = np.where(im == 0, 1.0 + 10.0, 1.0 / np.where(im == 0, 1.0, im)) / dt ** 2
+    agg = np.arange(n) // 256
+    ref = np.zeros((n, 16))
+    for a in range(agg.max() + 1):
+        idx = np.nonzero(agg == a)[0]
+        c = x[idx].mean(axis=0)
+        rad = np.sqrt(((x[idx] - c) ** 2).sum() / len(idx))
+        for i in idx:
+            cols = H.indices[H.indptr[i]:H.indptr[i + 1]]
+            blks = H.data[H.indptr[i]:H.indptr[i + 1]]
+            nb = np.sqrt((blks ** 2).sum(axis=(1, 2)))
+            D = blks[cols == i][0]
+            off = (cols != i) & (nb > 0)
+            out = off & ((cols // 256) != (i // 256))
+            trD = np.trace(D)
+            ref[i, :11] = [1.0, nb[out].sum() / nb[off].sum(), 3 * md[i] / trD, np.trace(Hv[i]) / trD,
+                           np.linalg.norm(x[i] - c) / rad, nb[off].sum() / np.linalg.norm(D), float(im[i] == 0),
+                           0.25 * np.log10(trD / (3 * md[i])), (off.sum() + 1) / 16, out.sum() / 16,
+                           float(dist[i] < m.params["contact_d1"])]
+    assert ref[:, 3].max() > 0 and ref[:, 10].max() == 1.0  # contact present
+    np.testing.assert_allclose(F, ref, rtol=1.0 / 64, atol=1e-3)
+
+
+def test_coarse_net_weights_in_a_step(gpu):
+    """With a coarse-weight network attached, a step's weights are
+    clamp(network output 0, 0, 2) on its first system's features, and the
+    step still meets rel-res 1e-6 and the oracle."""
+    from mfem_amd.coarse import CoarseWeightNet
+    from mfem_amd.learned import O_B3, O_W3
+
+    m = scene.drop_box_scene()
+    solver, state = _make(m, gpu)
+    w = CoarseWeightNet(gpu).weights()
+    w[O_W3:O_B3] = np.random.default_rng(0).normal(0, 0.5, O_B3 - O_W3)
+    w[O_B3] = 0.7
+    net = CoarseWeightNet(gpu, weights=w)
+    solver.set_coarse_net(net)
+    out = state.clone()
+    solver.step(state, out, None, None, 0.01)
+    torch.cuda.synchronize()
+    n = m.particle_count
+    y = net.forward(solver.buffers["cw_x"][:n])[:, 0].clamp(0.0, 2.0)
+    got = solver.buffers["coarse_w_net"][:n]
+    assert torch.equal(got, y)
+    assert float(got.std()) > 0.01 and float(got.max()) <= 2.0
+    assert np.all(solver.last_stats()[:, 1] <= 1e-6)
+    o = Oracle(m, 64)
+    o.step(0.01)
+    err = np.abs(out.particle_q[:n].double().cpu().numpy() - o.x).max() / np.abs(o.x).max()
+    assert err <= 1e-4
