@@ -1,5 +1,6 @@
 """Benchmark: tet-steps/s of the per-timestep soft-body solve on one MI355X,
-learned CG warm start vs plain block-Jacobi PCG.
+learned preconditioning (network-weighted coarse correction on top of block
+Jacobi, with a learned Galerkin warm start) vs plain block-Jacobi PCG.
 
 Workload (default): the ~2e5-tet octopus-like body (ellipsoid mantle + 8
 tapered arms, Neo-Hookean with a log-normal stiffness field) on a ground
@@ -7,21 +8,24 @@ plane with IPC barrier (dhat = 1e-3) and friction (mu = 0.3), dt = 1/100,
 12 Newton iterations per step, block-Jacobi PCG to relative residual 1e-6
 (FP64 residual), line search on.  Synthetic mesh, generated in-process.
 
-Order: W warmup steps (plain PCG); ``--train-steps`` plain steps whose
-solves are logged (features, candidates, right-hand side, matrix, PCG
-solution); training of the warm-start MLP on the GPU on the energy-norm error
-of its guess (``--objective``; BF16 MFMA forward/backward, SpMV, Adam) and
-model selection by CG iterations over two validation steps (the trained
-correction or none); the autotuner's choice of PCG variant and workgroup-local
-preconditioner sweeps (0, 2 or 4) on whole steps; then K timed steps with plain
-PCG (the reference's start and block Jacobi), K with the learned warm start
-(the Galerkin start over x~ - x, the learned guess and the last three Newton
-steps) and the tuned preconditioner (the value), and as ablations K with the
-network's correction switched off (previous Newton step as the guess) and K
-with the learned start under block Jacobi, all from the same state and
-solver state.  One "step" = RefinementSolver.step on the whole body.
-value = tets * K / wall time of the learned leg (inputs resident in HBM);
-the plain leg's rate is reported beside it (config.plain_value).
+Order: W warmup steps (plain PCG); ``--coarse-steps`` plain steps whose
+Newton systems are logged, the coarse-weight network (per-vertex smoothing
+weights of the PCG's workgroup-local rigid-mode correction) trained on them on
+the GPU (BF16 MFMA forward/backward, Adam) on the energy-norm error left by
+five preconditioned CG iterations, and kept only if two validation steps take
+fewer CG iterations than with the network off; ``--train-steps`` plain steps
+whose solves are logged for the warm-start MLP (a Galerkin-start candidate),
+trained the same way and kept only with a 5 % margin; plain steps up to
+``--start-step`` (default 50 of the 100-step scene); the autotuner's choice of
+PCG variant on whole steps; then K timed steps with plain PCG (the reference's
+start x~ - x and block Jacobi), K with the learned path (Galerkin start +
+block Jacobi + the network-weighted rigid-mode correction: the value), and as
+ablations K with the networks off (unsmoothed rigid modes, no learned guess),
+K with a hand-tuned uniform weight in place of the network, and K with block
+Jacobi alone under the Galerkin start, all from the same state and solver
+state.  One "step" = RefinementSolver.step on the whole body.  value = tets *
+K / wall time of the learned leg (inputs resident in HBM); the plain leg's
+rate is reported beside it (config.plain_value).
 
 The roofline object is for the dominant kernel, the persistent PCG (k_pcg):
 algorithmic bytes per CG iteration = 40 B per nonzero 3x3 block (36 B values
@@ -29,12 +33,15 @@ algorithmic bytes per CG iteration = 40 B per nonzero 3x3 block (36 B values
 times the iterations each launch ran, divided by the launch's duration from
 HIP events on the launching stream.  The mlp object is the MFMA figure of the
 network's forward kernel: 2 * (16*64 + 64*64 + 64*3) FLOP per vertex over its
-event-timed duration, against the dense BF16 peak.  cpu_baseline = the FP64 C
-oracle (oracle/mfem_oracle.c), one step of the same scene from its initial
-state, one thread.
+event-timed duration, against the dense BF16 peak.  cpu_baseline = the C
+oracle (oracle/mfem_oracle.c, -O3 -march=x86-64-v3, OpenMP over vertices and
+tets), one step from the state the timed legs start from, FP32 and FP64, on
+all host threads the process may use and on one; value = the faster
+all-threads leg.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                       [--train-steps T] [--epochs E] [--no-learned]
+                       [--start-step S] [--train-steps T] [--epochs E]
+                       [--no-learned] [--workload octopus|beam|batch]
 """
 import argparse
 import json
@@ -60,8 +67,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="octopus", choices=["octopus", "beam", "batch"],
                     help="octopus: the ~2e5-tet body (default); beam: the 1,920-tet cantilever; batch: 64 seeded "
-                         "soft bodies (~9.4e5 tets) in one batched solve")
+                         "soft bodies (~1.2e6 tets) in one batched solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-no-single-thread", action="store_true",
+                    help="time the CPU baseline on all threads only (skip the one-thread legs)")
     ap.add_argument("--pcg", default="standard", choices=["standard", "pipelined"])
     ap.add_argument("--stream-rows", action="store_true", help="stream matrix rows instead of keeping them in VGPRs")
     ap.add_argument("--train-steps", type=int, default=6, help="plain steps logged as training solves")
@@ -73,6 +82,9 @@ def main():
                          "(even; 0 = block Jacobi as the reference; default: the autotuner picks 0, 2 or 4)")
     ap.add_argument("--local-omega", type=float, default=0.5)
     ap.add_argument("--no-learned", action="store_true", help="time plain PCG only")
+    ap.add_argument("--start-step", type=int, default=50,
+                    help="step of the 100-step scene the timed legs start at: after warmup and the logged training "
+                         "steps, plain steps advance the body to it (the settled, contact-dense phase)")
     ap.add_argument("--coarse-steps", type=int, default=3, help="plain steps logged for the coarse-weight network")
     ap.add_argument("--coarse-epochs", type=int, default=10)
     ap.add_argument("--coarse-lr", type=float, default=2e-2)
@@ -161,8 +173,8 @@ def main():
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         coarse_info = {"systems": args.coarse_steps * solver.iterations, "epochs": args.coarse_epochs,
-                       "objective": "log(1 - (z.He)^2 / ((e.He)(z.Hz))), z = M(w)^-1 H e, e = dx* - (x~ - x): "
-                                    "energy-norm reduction of one damped preconditioned Richardson step",
+                       "objective": "log(e_5^T H e_5 / e^T H e): energy-norm error left by 5 CG iterations "
+                                    "preconditioned by M(w) on H y = H e, e = dx* - (x~ - x)",
                        "collect_s": t1 - t0, "train_s": t2 - t1, "loss_network_off": chist["off"],
                        "loss_per_epoch": [round(v, 5) for v in chist["loss"]], "best_epoch": chist["best_epoch"],
                        "loss_uniform_weight": chist.get("uniform"), "validation": cval, "weights": cw}
@@ -194,6 +206,22 @@ def main():
               f"{val['cg_iters']}, kept {val['kept']}", flush=True)
         if val["kept"] != "trained":
             net = None  # a zero correction: no warm-start network launches (the Galerkin start without it)
+
+    # plain steps up to the timed window (the networks were trained on the
+    # early steps: the window measures how they carry to later ones)
+    steps_done = args.warmup + ((args.coarse_steps + args.train_steps) if learned else 0)
+    solver.set_warm_start(None)
+    solver.set_coarse_net(None)
+    solver.set_coarse_correction(False)
+    solver._S.pcg_flags |= _ffi.PCG_PIPELINED
+    while steps_done < args.start_step:
+        solver.step(state, out, None, None, dt)
+        state, out = out, state
+        steps_done += 1
+    torch.cuda.synchronize()
+    if solver.status() != 0:
+        raise SystemExit("device error while advancing to the timed window")
+    print(f"[bench] timed window: steps {steps_done} to {steps_done + args.steps - 1}", flush=True)
 
     khz = solver.lib.mfem_wall_clock_khz()
 
@@ -315,6 +343,7 @@ def main():
         "verts": V,
         "newton_iterations": solver.iterations,
         "dt": dt,
+        "timed_steps": [steps_done, steps_done + args.steps - 1],
         "preconditioner": {"plain": "block_jacobi",
                            "value": "block_jacobi + workgroup-local rigid-mode correction, learned smoothing weights"
                            if learned else "block_jacobi"},
@@ -390,7 +419,7 @@ def main():
     if learned:
         result["mlp"] = mlp_mfma(solver, cnet, V)
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(m, dt)
+        result["cpu_baseline"] = cpu_baseline(m, dt, s0, snap0, single_thread=not args.cpu_no_single_thread)
     print(json.dumps(result), flush=True)
 
 
@@ -616,7 +645,7 @@ def run_batch(args):
     if learned:
         result["mlp"] = mlp_mfma(solver, net, solver.active_particle_count)
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline_bodies(models, dt)
+        result["cpu_baseline"] = cpu_baseline_bodies(models, dt, single_thread=not args.cpu_no_single_thread)
     print(json.dumps(result), flush=True)
 
 
@@ -625,24 +654,52 @@ def _young(m):
     return mu * (3 * lam + 2 * mu) / (lam + mu)
 
 
-def cpu_baseline_bodies(models, dt, budget_tets: int = 150_000):
-    """FP64 C oracle, one step of the first bodies of the batch (up to
-    ``budget_tets`` tets) from their initial states, one thread."""
+def cpu_baseline_bodies(models, dt, single_thread: bool = True, budget_tets: int = 150_000):
+    """The C oracle (-O3 -march=x86-64-v3) on the batch, one step of every
+    body from its initial state, bodies spread over all host threads the
+    process may use (one single-threaded oracle per body and thread: the
+    bodies are independent, ctypes drops the GIL), FP32 and FP64; and one
+    thread on the first bodies up to ``budget_tets`` tets.  ``value`` is the
+    faster all-threads rate."""
+    from concurrent.futures import ThreadPoolExecutor
+
     from oracle import Oracle
 
-    tets, sec, n = 0, 0.0, 0
-    for m in models:
-        if n and tets + m.tet_count > budget_tets:
-            break
-        o = Oracle(m, 64)
-        t0 = time.perf_counter()
+    threads, model = _cpu_info()
+
+    def one(m, bits):
+        o = Oracle(m, bits, threads=1)  # set in the worker thread: it owns its OpenMP settings
         o.step(dt)
-        sec += time.perf_counter() - t0
-        tets += m.tet_count
-        n += 1
-    return {"value": tets / sec, "unit": "tet-steps/s", "cores": 1, "kind": "port",
-            "sample": f"1 step (12 Newton iterations) of the first {n} bodies ({tets} tets) from their initial "
-                      f"states, one body at a time, FP64 C oracle, 1 thread, {sec:.1f} s"}
+        return m.tet_count
+
+    def run(bits, bodies, nthreads):
+        t0 = time.perf_counter()
+        if nthreads == 1:
+            tets = sum(one(m, bits) for m in bodies)
+        else:
+            # largest bodies first so the pool's tail is short
+            order = sorted(bodies, key=lambda m: -m.tet_count)
+            with ThreadPoolExecutor(max_workers=nthreads) as ex:
+                tets = sum(ex.map(lambda m: one(m, bits), order))
+        sec = time.perf_counter() - t0
+        return {"bits": bits, "threads": nthreads, "bodies": len(bodies), "tets": tets, "value": tets / sec,
+                "s": sec}
+
+    legs = [run(32, models, threads), run(64, models, threads)]
+    best = max(legs, key=lambda r: r["value"])
+    if single_thread:
+        sample, tets = [], 0
+        for m in models:
+            if sample and tets + m.tet_count > budget_tets:
+                break
+            sample.append(m)
+            tets += m.tet_count
+        legs += [run(32, sample, 1), run(64, sample, 1)]
+    return {"value": best["value"], "unit": "tet-steps/s", "cores": threads, "kind": "port", "cpu": model,
+            "sample": f"1 step (12 Newton iterations) of all {len(models)} bodies ({best['tets']} tets) from their "
+                      f"initial states, one body per thread on {threads} threads, C oracle -O3 "
+                      f"-march=x86-64-v3, FP{best['bits']}, {best['s']:.1f} s",
+            "legs": legs}
 
 
 def _sweep_options(args):
@@ -695,22 +752,84 @@ def mlp_mfma(solver, net, V):
             "unit": "TFLOP/s", "frac": tflops / BF16_PEAK_TFLOPS, "us_per_launch": us, "flop_per_launch": flop}
 
 
-def cpu_baseline(m, dt):
+def _cpu_info():
+    """Host threads this process may use (affinity mask, capped by
+    OMP_NUM_THREADS when that is set) and the CPU model name."""
+    n = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        n = min(n, int(os.environ["OMP_NUM_THREADS"]))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, n), model
+
+
+def cpu_baseline(m, dt, state=None, snap=None, steps: int = 1, single_thread: bool = True):
+    """The C oracle (oracle/mfem_oracle.c) built -O3 -march=x86-64-v3 with
+    OpenMP, timed on ``steps`` steps from the state the timed GPU legs start
+    from (positions, velocities, stretch and multipliers copied from the
+    device), in FP32 and FP64 on all host threads this process may use, and
+    at one thread.  ``value`` is the faster all-threads leg."""
     from oracle import Oracle
 
-    o = Oracle(m, 64)
-    t0 = time.perf_counter()
-    o.step(dt)
-    sec = time.perf_counter() - t0
+    threads, model = _cpu_info()
+
+    def run(bits, nthreads):
+        o = Oracle(m, bits, threads=nthreads)
+        if state is not None:
+            n = m.particle_count
+            o.buf("x")[:] = state.particle_q[:n].double().cpu().numpy().ravel()
+            o.buf("v")[:] = state.particle_qd[:n].double().cpu().numpy().ravel()
+            o.buf("s")[:] = snap["s"][: m.tet_count].double().cpu().numpy().ravel()
+            o.buf("lam")[:] = snap["lam"][: m.tet_count].double().cpu().numpy().ravel()
+        t0 = time.perf_counter()
+        cg = 0
+        for _ in range(steps):
+            o.step(dt)
+            cg += int(o.stats()[:, 0].sum())
+        sec = time.perf_counter() - t0
+        return {"bits": bits, "threads": o.threads, "value": m.tet_count * steps / sec, "s_per_step": sec / steps,
+                "cg_iters_per_step": cg / steps}
+
+    legs = [run(32, threads), run(64, threads)]
+    best = max(legs, key=lambda r: r["value"])
+    if single_thread:
+        legs += [run(32, 1), run(64, 1)]
     return {
-        "value": m.tet_count / sec,
+        "value": best["value"],
         "unit": "tet-steps/s",
-        "cores": 1,
+        "cores": best["threads"],
         "kind": "port",
-        "sample": f"1 step ({int(o.param('iterations'))} Newton iterations) of the same scene from its initial "
-                  f"state, FP64 C oracle, 1 thread, {sec:.1f} s",
+        "cpu": model,
+        "sample": f"{steps} step(s) ({int(m.params['iterations'])} Newton iterations, CG to rel-res 1e-6) from the "
+                  f"state the timed GPU legs start from, C oracle -O3 -march=x86-64-v3 + OpenMP, "
+                  f"FP{best['bits']}, {best['threads']} threads, {best['s_per_step'] * steps:.1f} s",
+        "legs": legs,
     }
+
+
+def _release():
+    """Frees the device memory held by torch's caching allocator (the
+    solvers are gone with main's frame) before the interpreter tears down
+    HIP, so that no device free runs after the runtime or a profiler's tool
+    has finalised."""
+    import gc
+
+    import torch
+
+    if torch.cuda.is_initialized():
+        torch.cuda.synchronize()
+        gc.collect()
+        torch.cuda.empty_cache()
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":
     main()
+    _release()

@@ -5,7 +5,10 @@ load this package, and only as the checker / CPU baseline: the product path
 (``mfem_amd``) never imports it.
 
 ``Oracle`` wraps ``liboracle64.so`` (FP64) or ``liboracle32.so`` (FP32), both
-built from ``mfem_oracle.c`` by ``oracle/Makefile``.  Each stage function
+built from ``mfem_oracle.c`` by ``oracle/Makefile``; ``Oracle(..., threads=n)``
+loads the OpenMP builds (``liboracle{64,32}_omp.so``) instead, which time the
+CPU baseline (their sums and scatters run in another order, so they are not
+the bit-reproducible checker).  Each stage function
 mirrors one phase of ``RefinementSolver.step``
 (src/mfem/refinement/solver.py:429-579 of the reference).
 """
@@ -18,7 +21,7 @@ import subprocess
 import numpy as np
 
 _DIR = os.path.dirname(os.path.abspath(__file__))
-_LIBS: dict[int, ctypes.CDLL] = {}
+_LIBS: dict[tuple, ctypes.CDLL] = {}
 
 
 def build() -> None:
@@ -26,10 +29,10 @@ def build() -> None:
     subprocess.run(["make", "-s", "-B", "-C", _DIR], check=True)
 
 
-def _lib(bits: int) -> ctypes.CDLL:
-    if bits in _LIBS:
-        return _LIBS[bits]
-    path = os.path.join(_DIR, f"liboracle{bits}.so")
+def _lib(bits: int, omp: bool = False) -> ctypes.CDLL:
+    if (bits, omp) in _LIBS:
+        return _LIBS[(bits, omp)]
+    path = os.path.join(_DIR, f"liboracle{bits}{'_omp' if omp else ''}.so")
     if not os.path.exists(path):
         build()
     lib = ctypes.CDLL(path)
@@ -60,7 +63,9 @@ def _lib(bits: int) -> ctypes.CDLL:
     lib.orc_spmv.argtypes = [P, P, P]
     lib.orc_spmv.restype = None
     lib.orc_real_size.restype = ctypes.c_int
-    _LIBS[bits] = lib
+    lib.orc_set_threads.argtypes = [ctypes.c_int]
+    lib.orc_set_threads.restype = ctypes.c_int
+    _LIBS[(bits, omp)] = lib
     return lib
 
 
@@ -221,9 +226,10 @@ class Oracle:
 
     INT_FIELDS = {"tets", "pin_idx", "tris", "shape_type", "rowptr", "col", "diag", "lag_sid"}
 
-    def __init__(self, scene, bits: int = 64, params: dict | None = None):
+    def __init__(self, scene, bits: int = 64, params: dict | None = None, threads: int | None = None):
         self.bits = bits
-        self.lib = _lib(bits)
+        self.lib = _lib(bits, omp=threads is not None)
+        self.threads = self.lib.orc_set_threads(threads) if threads is not None else 1
         self.dt = _real(bits)
         nv, nt = scene.particle_q.shape[0], scene.tet_indices.shape[0]
         ntris = 0 if scene.tri_indices is None else scene.tri_indices.shape[0]

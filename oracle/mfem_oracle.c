@@ -41,6 +41,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifndef REAL
 #define REAL double
@@ -908,6 +911,7 @@ EXPORT real orc_tri_barrier(const real* a, const real* b, const real* c, int nsh
 /* start of step (solver.py:499-520): x_tilde, friction anchor, lagged force */
 EXPORT void orc_begin_step(orc_ctx* C, real dt) {
   real g[3] = {C->p[P_GX], C->p[P_GY], C->p[P_GZ]};
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < C->nv; ++i)
     for (int k = 0; k < 3; ++k) {
       C->xin[3 * i + k] = C->x[3 * i + k];
@@ -918,6 +922,7 @@ EXPORT void orc_begin_step(orc_ctx* C, real dt) {
   for (int k = 0; k < C->nshapes; ++k) if (C->shape_mu[k] > mumax) mumax = C->shape_mu[k];
   if (C->nshapes == 0 || mumax <= 0) return;
   const real* qb = C->p + P_QB0;
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < C->nv; ++i) {
     real d, gr[3];
     int sid;
@@ -960,6 +965,7 @@ EXPORT void orc_update_system(orc_ctx* C, real dt) {
   real idt2 = 1 / (dt * dt);
   const real* qb = C->p + P_QB0;
   real k = C->p[P_KAPPA], d0 = C->p[P_D0], d1 = C->p[P_D1];
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < nv; ++i) {
     for (int a = 0; a < 3; ++a) C->gx[3 * i + a] = C->mass[i] * (C->x[3 * i + a] - C->xt[3 * i + a]) * idt2;
     real d, gr[3], hs[9];
@@ -1002,6 +1008,7 @@ EXPORT void orc_update_system(orc_ctx* C, real dt) {
     }
   }
   int energy = (int)C->p[P_ENERGY];
+#pragma omp parallel for schedule(static)
   for (int e = 0; e < nt; ++e) {
     const real* s = C->s + 6 * e;
     real mu = C->mat[3 * e], la = C->mat[3 * e + 1], vol = C->vol[e];
@@ -1021,6 +1028,9 @@ EXPORT void orc_assemble(orc_ctx* C, real dt) {
   memset(C->val, 0, sizeof(real) * 9 * C->nnzb);
   for (int i = 0; i < nv; ++i)
     for (int a = 0; a < 3; ++a) C->g[3 * i + a] = -C->gx[3 * i + a];
+  /* (built with OpenMP for the CPU baseline: tets in parallel, their
+     contributions added atomically; the serial build sums in tet order) */
+#pragma omp parallel for schedule(static)
   for (int e = 0; e < nt; ++e) {
     real* Hl = C->Hl + 36 * e;
     const real* Hs = C->Hs + 36 * e;
@@ -1050,11 +1060,17 @@ EXPORT void orc_assemble(orc_ctx* C, real dt) {
     }
     for (int va = 0; va < 4; ++va) {
       int i = C->tets[4 * e + va];
-      for (int r = 0; r < 3; ++r) C->g[3 * i + r] += fe[3 * va + r];
+      for (int r = 0; r < 3; ++r) {
+#pragma omp atomic
+        C->g[3 * i + r] += fe[3 * va + r];
+      }
       for (int vb = 0; vb < 4; ++vb) {
         real* blk = C->val + 9 * C->tet_slot[16 * e + 4 * va + vb];
         for (int r = 0; r < 3; ++r)
-          for (int c = 0; c < 3; ++c) blk[3 * r + c] += Ke[(3 * va + r) * 12 + 3 * vb + c];
+          for (int c = 0; c < 3; ++c) {
+#pragma omp atomic
+            blk[3 * r + c] += Ke[(3 * va + r) * 12 + 3 * vb + c];
+          }
       }
     }
   }
@@ -1099,6 +1115,7 @@ EXPORT void orc_assemble(orc_ctx* C, real dt) {
 }
 
 EXPORT void orc_spmv(orc_ctx* C, const real* x, real* y) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < C->nv; ++i) {
     real acc[3] = {0, 0, 0};
     for (int k = C->rowptr[i]; k < C->rowptr[i + 1]; ++k) {
@@ -1112,6 +1129,7 @@ EXPORT void orc_spmv(orc_ctx* C, const real* x, real* y) {
 
 static double vdot(const real* a, const real* b, int n) {
   double acc = 0;
+#pragma omp parallel for schedule(static) reduction(+ : acc)
   for (int i = 0; i < n; ++i) acc += (double)a[i] * (double)b[i];
   return acc;
 }
@@ -1145,11 +1163,14 @@ EXPORT int orc_global_solve(orc_ctx* C, double* relres) {
     double pq = vdot(C->pp, C->q, n);
     if (pq == 0) break;
     real alpha = (real)(rz / pq);
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < n; ++k) { C->dx[k] += alpha * C->pp[k]; C->r[k] -= alpha * C->q[k]; }
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nv; ++i) mm(C->Minv + 9 * i, C->r + 3 * i, C->z + 3 * i, 3, 3, 1);
     double rz2 = vdot(C->r, C->z, n);
     real beta = (real)(rz2 / rz);
     rz = rz2;
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < n; ++k) C->pp[k] = C->z[k] + beta * C->pp[k];
     rn = sqrt(vdot(C->r, C->r, n));
     ++it;
@@ -1163,6 +1184,7 @@ EXPORT int orc_global_solve(orc_ctx* C, double* relres) {
 
 /* local solve (kernels.py:252-283); tet_lambda <- new lambda (solver.py:537) */
 EXPORT void orc_local_solve(orc_ctx* C) {
+#pragma omp parallel for schedule(static)
   for (int e = 0; e < C->nt; ++e) {
     real t[6];
     const real* G = C->G + 72 * e;
@@ -1193,6 +1215,7 @@ EXPORT void orc_local_solve(orc_ctx* C) {
 EXPORT double orc_merit(orc_ctx* C, const real* x, const real* s, real dt) {
   double idt2 = 1.0 / ((double)dt * (double)dt);
   double kin = 0, att = 0, ela = 0, bar = 0, fri = 0;
+#pragma omp parallel for schedule(static) reduction(+ : kin)
   for (int i = 0; i < C->nv; ++i) {
     double d2 = 0;
     for (int a = 0; a < 3; ++a) { double d = (double)x[3 * i + a] - (double)C->xt[3 * i + a]; d2 += d * d; }
@@ -1205,6 +1228,7 @@ EXPORT double orc_merit(orc_ctx* C, const real* x, const real* s, real dt) {
     att += 0.5 * (double)C->p[P_ATTACH_K] * d2;
   }
   int energy = (int)C->p[P_ENERGY];
+#pragma omp parallel for schedule(static) reduction(+ : ela)
   for (int e = 0; e < C->nt; ++e) {
     const real* se = s + 6 * e;
     real mu = C->mat[3 * e], la = C->mat[3 * e + 1], vol = C->vol[e];
@@ -1218,6 +1242,7 @@ EXPORT double orc_merit(orc_ctx* C, const real* x, const real* s, real dt) {
     ela += psi - lc;
   }
   real kappa = C->p[P_KAPPA];
+#pragma omp parallel for schedule(static) reduction(+ : bar, fri)
   for (int i = 0; i < C->nv; ++i) {
     real d;
     vertex_distance(C, x + 3 * i, &d, NULL, NULL, NULL);
@@ -1231,6 +1256,7 @@ EXPORT double orc_merit(orc_ctx* C, const real* x, const real* s, real dt) {
     fri += E;
   }
   if (C->p[P_TRI_CONTACT] != 0 && C->ntris > 0 && C->nshapes > 0) {
+#pragma omp parallel for schedule(static) reduction(+ : bar)
     for (int t = 0; t < C->ntris; ++t) {
       const int* tr = C->tris + 3 * t;
       real dd, w[3], n[3];
@@ -1256,6 +1282,7 @@ EXPORT int orc_line_search(orc_ctx* C, real dt, int* accepted, double* alpha_out
   double gdx = 0, gds = 0;
   real* gxf = C->q; /* scratch: g_x + gB + gD - G^T lambda */
   for (int k = 0; k < 3 * nv; ++k) gxf[k] = C->gx[k] + C->gB[k] + C->gD[k];
+#pragma omp parallel for schedule(static)
   for (int e = 0; e < nt; ++e) {
     const real* G = C->G + 72 * e;
     for (int a = 0; a < 4; ++a) {
@@ -1263,11 +1290,13 @@ EXPORT int orc_line_search(orc_ctx* C, real dt, int* accepted, double* alpha_out
       for (int r = 0; r < 3; ++r) {
         real acc = 0;
         for (int k = 0; k < 6; ++k) acc += G[k * 12 + 3 * a + r] * C->lam[6 * e + k];
+#pragma omp atomic
         gxf[3 * i + r] -= acc;
       }
     }
   }
   gdx = vdot(gxf, C->dx, 3 * nv);
+#pragma omp parallel for schedule(static) reduction(+ : gds)
   for (int e = 0; e < nt; ++e)
     for (int k = 0; k < 6; ++k) gds += ((double)C->gs[6 * e + k] + (double)SYMD[k] * (double)C->lam[6 * e + k]) * (double)C->ds[6 * e + k];
   double c = C->p[P_LS_C], tau = C->p[P_LS_TAU], thr = C->p[P_LS_THR];
@@ -1335,3 +1364,14 @@ EXPORT void orc_destroy(orc_ctx* C) {
 }
 
 EXPORT int orc_real_size(void) { return (int)sizeof(real); }
+
+/* Thread count of the OpenMP build (the serial build reports 1). */
+EXPORT int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}

@@ -187,11 +187,10 @@ class CoarseWeightNet(WarmStartNet):
         return tot / max(len(systems), 1)
 
     def fit(self, solves: list, pattern, epochs: int = 10, lr: float = 3e-3, seed: int = 0) -> dict:
-        """One Adam step per logged system per epoch on the Richardson loss
-        (dL/dy from torch autograd on the GPU, the MLP's backward kernel for
-        the weights); keeps the best epoch by the mean loss, epoch 0 (w = 0)
-        included."""
-        n = int(solves[0]["X"].shape[0]) if solves else 0
+        """One Adam step per logged system per epoch on the five-step CG loss
+        (``CoarseSystem.loss``; dL/dy from torch autograd on the GPU, the
+        MLP's backward kernel for the weights); keeps the best epoch by the
+        mean loss, epoch 0 (w = 0) included."""
         bs = BlockSystem(pattern, int(solves[0]["n"]), self.device)
         systems = [CoarseSystem(bs, sv) for sv in solves]
         g = torch.Generator(device="cpu").manual_seed(seed)
@@ -213,7 +212,6 @@ class CoarseWeightNet(WarmStartNet):
                 best.copy_(self.w)
         self.w.copy_(best)
         self.refresh()
-        del n
         return hist
 
     def validate_cg(self, solver, state, dt: float, steps: int = 2) -> dict:

@@ -234,7 +234,7 @@ def drop_box_scene(cells=(4, 3, 3), size=(0.2, 0.15, 0.15), clearance=5.0e-4, E=
     return m
 
 
-def soft_body_batch(n_bodies: int = 64, seed: int = 5, tets_min: float = 2.0e3, tets_max: float = 5.0e4,
+def soft_body_batch(n_bodies: int = 64, seed: int = 360, tets_min: float = 2.0e3, tets_max: float = 5.0e4,
                     dhat: float = 1.0e-3, friction_mu: float = 0.3, iterations: int = 12,
                     spacing: float = 1.0) -> list:
     """``n_bodies`` independent soft bodies, each a box or an ellipsoid
@@ -244,7 +244,10 @@ def soft_body_batch(n_bodies: int = 64, seed: int = 5, tets_min: float = 2.0e3, 
     [1e4, 1e6] Pa, nu = 0.45, rho = 1000.  Each starts deformed (a random
     stretch / shear of up to 15 % from its rest shape), 2 mm above the plane
     z = 0 (IPC barrier dhat, friction mu) with a small random velocity, on an
-    8-wide grid ``spacing`` apart.  Solver parameters are shared (a batch
+    8-wide grid ``spacing`` apart.  The log-uniform draw of 64 counts in
+    [2e3, 5e4] averages ~0.95e6 tets; the default seed is one whose draw
+    totals ~1.2e6 (1,209,342 tets, 248,279 vertices, bodies of 2,052 to
+    46,872 tets).  Solver parameters are shared (a batch
     shares them): those of ``octopus_scene`` with the barrier stiffness of
     the heaviest-vertex body, so every body's contact holds."""
     rng = np.random.default_rng(seed)
