@@ -248,7 +248,13 @@ def main():
               f"{c['us_per_iter']:.2f} us/iter", flush=True)
     tuned_variant = tune["chosen"]["variant"] if tune else -1
 
-    def timed_leg(s0, ws_net, coarse_net, coarse_on, omega=0.0, ws=True):
+    def timed_leg(s0, ws_net, coarse_net, coarse_on, omega=0.0, ws=True, mode="eager"):
+        """K steps from s0 and the same solver state.  mode "eager": one
+        ``solver.step`` (one mfem_step, no host work between the kernels) per
+        step; "graph": replays of two HIP graphs of the step (a -> b, b -> a;
+        ``capture_step``); "instrumented": the same steps stage by stage with
+        events between the phases (each event record costs ~10 us of device
+        idle), for the phase and per-launch timings."""
         st = s0.clone()
         o = st.clone()
         solver.restore(snap0)
@@ -260,12 +266,22 @@ def main():
             solver.set_coarse_correction(False)
             solver._S.pcg_flags |= _ffi.PCG_PIPELINED
         solver._S.pcg_variant = tuned_variant if coarse_on else -1
+        graphs = None
+        if mode == "graph":
+            graphs = (solver.capture_step(st, o, dt), solver.capture_step(o, st, dt))
+            st.assign(s0)
+            solver.restore(snap0)
         timers: dict = {}
         hist = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(args.steps):
-            solver.step_instrumented(st, o, dt, timers)
+            if mode == "instrumented":
+                solver.step_instrumented(st, o, dt, timers)
+            elif mode == "graph":
+                graphs[k % 2].replay()
+            else:
+                solver.step(st, o, None, None, dt)
             hist.append(solver.buffers["stats"].clone())
             st, o = o, st
         torch.cuda.synchronize()
@@ -308,7 +324,29 @@ def main():
             print(f"[bench] ablation {name}: {args.steps} steps in {w_:.3f} s, "
                   f"{abl[name]['cg_iters_per_step']:.0f} CG iterations per step", flush=True)
     else:
-        wall, st, timers = wall_p, st_p, timers_p
+        wall, st = wall_p, st_p
+    # the plain and the value configuration replayed as HIP graphs; the
+    # value is the faster mode, compared with the plain leg in the same mode
+    wall_pg, st_pg, _, _ = timed_leg(s0, None, None, False, ws=False, mode="graph")
+    if learned:
+        wall_g, st_g, _, _ = timed_leg(s0, net, cn, True, args.coarse_omega, mode="graph")
+    else:
+        wall_g, st_g = wall_pg, st_pg
+    print(f"[bench] HIP graph replays: plain {wall_pg:.3f} s, value configuration {wall_g:.3f} s", flush=True)
+    eager = {"value_ms_per_step": wall * 1e3 / args.steps, "plain_ms_per_step": wall_p * 1e3 / args.steps}
+    graph = {"value_ms_per_step": wall_g * 1e3 / args.steps, "plain_ms_per_step": wall_pg * 1e3 / args.steps}
+    wall_e = wall
+    step_mode = "graph" if wall_g < wall else "eager"
+    if step_mode == "graph":
+        wall, wall_p, st = wall_g, wall_pg, st_g
+    # the same steps again, stage by stage with events: phase and PCG launch
+    # timings of the value leg and of the plain leg
+    if learned:
+        wall_i, st_i, timers, _ = timed_leg(s0, net, cn, True, args.coarse_omega, mode="instrumented")
+    else:
+        wall_i, st_i, timers, _ = timed_leg(s0, None, None, False, ws=False, mode="instrumented")
+    wall_pi, _, timers_p, _ = timed_leg(s0, None, None, False, ws=False, mode="instrumented")
+    same_iters = bool(np.array_equal(st_i[:, :, 0], st[:, :, 0]))
 
     ticks = st[:, :, 8:14].sum(axis=(0, 1))
     n_it = max(1.0, float(st[:, :, 0].sum()))
@@ -352,6 +390,11 @@ def main():
         "reductions": "f64",
         "plain_value": value_plain,
         "plain_ms_per_step": wall_p * 1e3 / args.steps,
+        "step_mode": step_mode,
+        "eager": eager,
+        "hip_graph": graph,
+        "instrumented_ms_per_step": {"value_leg": wall_i * 1e3 / args.steps, "plain_leg": wall_pi * 1e3 / args.steps,
+                                     "same_cg_iterations": same_iters},
         "learned_speedup": value / value_plain,
         "cg_iters_per_step": float(cg_iters.sum(axis=1).mean()),
         "plain_cg_iters_per_step": float(st_p[:, :, 0].sum(axis=1).mean()),
@@ -380,8 +423,10 @@ def main():
         config["coarse_weight_network"] = coarse_info
         config["ablations"] = abl
         # what the networks add over the same method without them
-        config["learned_correction_gain"] = value / abl["networks_off"]["value"]
-        config["learned_vs_uniform_weight"] = value / abl["uniform_weight"]["value"]
+        # (ablations run eagerly: compared with the eager value leg)
+        value_e = T * args.steps / wall_e
+        config["learned_correction_gain"] = value_e / abl["networks_off"]["value"]
+        config["learned_vs_uniform_weight"] = value_e / abl["uniform_weight"]["value"]
     result = {
         "metric": "tet-steps/s at CG rel-res 1e-6, learned-precond vs plain PCG, 1 MI355X",
         "value": value,
@@ -414,8 +459,13 @@ def main():
             "hbm_frac_measured": (float(traffic / (np.mean(pcg_ms[active]) * 1e-3) / 1e9 / HBM_PEAK_GBS)
                                   if traffic else None),
             "us_per_cg_iter": float(np.sum(pcg_ms[active]) * 1e3 / max(1.0, np.sum(iters_flat[active]))),
+            "us_per_launch": float(np.mean(pcg_ms[active]) * 1e3) if active.any() else None,
+            "cg_iters_per_launch": float(np.mean(iters_flat[active])) if active.any() else None,
         },
     }
+    result["roofline_assembly"] = assembly_roofline(phase_ms, args.steps * solver.iterations, T, V, nnz,
+                                                    args.workload == "octopus")
+    result["roofline_spmv"] = spmv_roofline(solver, V, nnz)
     if learned:
         result["mlp"] = mlp_mfma(solver, cnet, V)
     if not args.no_cpu:
@@ -499,7 +549,9 @@ def run_batch(args):
     if args.local_sweeps is None:
         args.local_sweeps = 0
 
-    def timed_leg(s0, use_net, sweeps=0):
+    def timed_leg(s0, use_net, sweeps=0, instrumented=False):
+        """K eager steps (``solver.step``), or with ``instrumented`` the same
+        steps stage by stage with events between the phases."""
         st, o = s0.clone(), s0.clone()
         solver.restore(snap0)
         solver.set_warm_start(use_net)
@@ -509,7 +561,10 @@ def run_batch(args):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(args.steps):
-            solver.step_instrumented(st, o, dt, timers)
+            if instrumented:
+                solver.step_instrumented(st, o, dt, timers)
+            else:
+                solver.step(st, o, None, None, dt)
             hist.append(body_stats().clone())
             st, o = o, st
         torch.cuda.synchronize()
@@ -554,6 +609,8 @@ def run_batch(args):
     wall_g = graph_leg(s0, net, args.local_sweeps)
     print(f"[bench] learned, HIP graph replays: {args.steps} steps in {wall_g:.3f} s", flush=True)
     wall = min(wall_e, wall_g)
+    # phase and PCG launch timings from an instrumented repeat of the value leg
+    _, _, timers, _ = timed_leg(s0, net, args.local_sweeps, instrumented=True)
 
     # per-body algorithmic bytes per CG iteration
     rp = solver.pattern.rowptr
@@ -727,6 +784,59 @@ def cnet_weights_summary(solver, cnet, state, dt):
     finally:
         solver.set_coarse_net(None)
         solver.restore(snap)
+
+
+FP32_VALU_PEAK_TFLOPS = 157.3  # published (packed FP32); 78.6 without packed instructions
+
+
+def assembly_roofline(phase_ms, n_newton, T, V, B, octopus):
+    """Element + assembly per Newton iteration: k_vertex + k_element (the
+    update_system phase) and k_assemble + k_rhs (the assemble phase), event
+    timed over the timed leg.  Algorithmic bytes = 84 T + 40 V + 36 B (tets,
+    DmInv, s, Lame pair per tet; x, x~, mass and g per vertex; 36 B per
+    nonzero block of H).  The VALU figure, when the counter summary of this
+    workload is present, is the measured FP32 FLOP count of those kernels
+    (64 lanes x (ADD + MUL + TRANS + 2 FMA) instructions) over the same time."""
+    us = (phase_ms["update_system"] + phase_ms["assemble"]) * 1e3 / max(1, n_newton)
+    algo = 84.0 * T + 40.0 * V + 36.0 * B
+    gbs = algo / (us * 1e-6) / 1e9
+    out = {"kernels": ["k_vertex", "k_element", "k_assemble", "k_rhs"], "bound": "valu", "achieved": gbs,
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "us_per_newton_iter": us,
+           "bytes_per_newton_iter": algo}
+    f = os.path.join(ROOT, "profiles", "valu_element_assembly_octopus_204k.json")
+    if octopus and os.path.exists(f):
+        with open(f) as fh:
+            prof = json.load(fh)
+        flop = float(prof["flop_per_newton_iter"])
+        tf = flop / (us * 1e-6) / 1e12
+        out.update({"valu_tflops": tf, "valu_peak_tflops": FP32_VALU_PEAK_TFLOPS,
+                    "valu_frac": tf / FP32_VALU_PEAK_TFLOPS, "flop_per_newton_iter": flop,
+                    "traffic": prof.get("hbm_bytes_per_newton_iter")})
+    return out
+
+
+def spmv_roofline(solver, V, B):
+    """The stand-alone SELL-64 SpMV (k_spmv, mfem_spmv) on the last assembled
+    system, event timed on the launching stream: algorithmic bytes = 40 B +
+    4 (V + 1) + 24 V (block values and column, row pointer, x and y)."""
+    import torch
+
+    x = torch.randn(solver.max_particles, 3, device=solver.device)
+    for _ in range(3):
+        solver.spmv(x)
+    n = 50
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(n):
+        solver.spmv(x)
+    b.record()
+    torch.cuda.synchronize()
+    us = a.elapsed_time(b) * 1e3 / n
+    algo = 40.0 * B + 4.0 * (V + 1) + 24.0 * V
+    gbs = algo / (us * 1e-6) / 1e9
+    return {"kernel": "k_spmv", "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS, "us_per_launch": us, "bytes_per_launch": algo}
 
 
 def mlp_mfma(solver, net, V):

@@ -174,3 +174,19 @@ def test_full_batch_one_step(gpu):
         errs.append(np.abs(xb - ref[i]).max() / np.abs(ref[i]).max())
     print("rel err vs FP64 oracle per body: max", max(errs), "median", float(np.median(errs)))
     assert max(errs) <= 1e-4, int(np.argmax(errs))
+
+
+def test_octopus_spmv_matches_csr(octopus):
+    """The stand-alone SpMV (mfem_spmv) on the octopus's last assembled
+    system against scipy's CSR product of the same matrix (FP64)."""
+    m, solver, state, snap, x_ref, dt = octopus
+    n = m.particle_count
+    rng = np.random.default_rng(5)
+    xv = rng.normal(size=(solver.max_particles, 3))
+    y = solver.spmv(torch.as_tensor(xv, dtype=torch.float32, device=state.particle_q.device))
+    y = y[:n].double().cpu().numpy().ravel()
+    H = solver.system_csr()
+    ref = H @ xv[:n].astype(np.float32).astype(np.float64).ravel()
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    print("octopus SpMV rel err vs CSR", err)
+    assert err <= 1e-5

@@ -2949,14 +2949,52 @@ __global__ void __launch_bounds__(kBlock) k_end_step(mfem_solver S, float dt) {
   for (int k = 0; k < 3; ++k) S.v[3 * i + k] = (S.x[3 * i + k] - S.x_start[3 * i + k]) / dt;
 }
 
+// Stand-alone SELL-64 SpMV y = H x: one workgroup per 64-row chunk, its four
+// waves splitting the chunk's slots (wave w takes slots w, w + 4, ...: the
+// loads of up to four slots per lane in flight at once, one dependent gather
+// round trip per group), partial rows summed through LDS, and the chunk's 192
+// outputs stored as three contiguous 256-byte runs.
 __global__ void __launch_bounds__(kBlock) k_spmv(mfem_solver S, const float* xin, float* yout) {
-  int row = blockIdx.x * kBlock + threadIdx.x;
-  if (row >= S.n_verts) return;
-  float y[3];
-  spmv_row_stream<float, float>(S, row >> 6, row & 63, xin, y, false);
-  yout[3 * row] = y[0];
-  yout[3 * row + 1] = y[1];
-  yout[3 * row + 2] = y[2];
+  __shared__ float part[kWaves][3][64];
+  const int c = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int s0 = S.chunk_ptr[c], s1 = S.chunk_ptr[c + 1];
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  constexpr int G = 4;
+  for (int sb = s0 + w; sb < s1; sb += kWaves * G) {
+    int col[G];
+    float vv[G][9], g[G][3];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      const int sl = sb + kWaves * k;
+      const bool ok = sl < s1;
+      col[k] = ok ? S.sell_col[sl * 64 + lane] : 0;
+      const float* vb = S.sell_val + (size_t)(ok ? sl : s0) * 576 + lane;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) vv[k][q] = ok ? vb[64 * q] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) g[k][a] = xin[3 * col[k] + a];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      a0 += vv[k][0] * g[k][0] + vv[k][1] * g[k][1] + vv[k][2] * g[k][2];
+      a1 += vv[k][3] * g[k][0] + vv[k][4] * g[k][1] + vv[k][5] * g[k][2];
+      a2 += vv[k][6] * g[k][0] + vv[k][7] * g[k][1] + vv[k][8] * g[k][2];
+    }
+  }
+  part[w][0][lane] = a0;
+  part[w][1][lane] = a1;
+  part[w][2][lane] = a2;
+  __syncthreads();
+  // thread t < 192 sums output t of the chunk (row t / 3, component t % 3)
+  const int t = threadIdx.x;
+  if (t < 192) {
+    const int r = t / 3, a = t - 3 * r;
+    const long row = (long)c * 64 + r;
+    if (row < S.n_verts)
+      yout[3 * row + a] = part[0][a][r] + part[1][a][r] + part[2][a][r] + part[3][a][r];
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_merit(mfem_solver S, float dt, float a, double* out) {
@@ -3320,7 +3358,7 @@ int mfem_ws_apply(const mfem_solver* S, void* stream) {
 int mfem_spmv(const mfem_solver* S, const float* x, float* y, void* stream) {
   if (int rc = check_solver(S)) return rc;
   if (!x || !y) return fail(MFEM_EINVAL, "null vector");
-  hipLaunchKernelGGL(k_spmv, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, (hipStream_t)stream, *S, x, y);
+  hipLaunchKernelGGL(k_spmv, dim3(S->n_chunks), dim3(kBlock), 0, (hipStream_t)stream, *S, x, y);
   return launched();
 }
 
