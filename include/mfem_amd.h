@@ -64,6 +64,10 @@ extern "C" {
                                  PCG and line search included) then returns at once, leaving the state
                                  untouched, until status and flags are cleared */
 #define MFEM_STATUS_EPOCH 1   /* grid-barrier epoch counter (internal) */
+/* error bits in status[MFEM_STATUS_ERRBITS] (any bit voids the step the same way) */
+#define MFEM_ERR_BARRIER 1 /* a grid barrier timed out */
+#define MFEM_ERR_CONTACT 2 /* self-contact candidates overflowed or a swept primitive spanned too many cells
+                              (sc_count[2], sc_count[3] hold the MFEM_BP_* bits) */
 
 /*
  * Solver description: sizes, parameters and device buffers.
@@ -241,6 +245,36 @@ typedef struct mfem_solver {
   float* cw_y;             /* [n_verts][3] network outputs */
   const float* cw_w;       /* [MFEM_MLP_NPARAMS] FP32 master weights */
   const uint16_t* cw_wbf;  /* [MFEM_MLP_NPARAMS] BF16 copy */
+  /* self-contact: the IPC barrier b(d) = -(d - sc_dhat)^2 ln(d / sc_dhat)
+     (src/mfem/collision.py:529-546) on the distance of every point-triangle
+     and edge-edge pair of surface primitives (closest points of
+     collision.py:220-322, evaluated in FP64) below sc_dhat, inside the Newton
+     solve: energy in the merit, gradient in g, the pair's Gauss-Newton
+     Hessian kappa max(b'', 0) grad d grad d^T on its vertices' diagonal
+     blocks, candidates from the spatial-hash broad phase on the boxes swept
+     by the Newton step, and the line search's first step cut by a
+     conservative time of impact (additive CCD) so that no pair reaches
+     zero distance.  The reference's solve leaves this hook commented out
+     (src/mfem/solver.py:443-447).  Single body only (n_bodies <= 1). */
+  int32_t self_contact;      /* 1: on */
+  int32_t n_sc_points;       /* surface vertices (query points of the point-triangle pairs) */
+  int32_t n_sc_edges;        /* surface edges */
+  int32_t sc_max_pairs;      /* capacity of each candidate list */
+  int32_t sc_max_span;       /* broad-phase cells per axis a swept primitive may span (1..8) */
+  float sc_dhat;             /* barrier activation distance */
+  float sc_kappa;            /* barrier stiffness */
+  float sc_cell;             /* broad-phase cell size */
+  float sc_ccd_s;            /* CCD: a pair keeps at least this fraction of its distance (0.1) */
+  const int32_t* sc_points;  /* [n_sc_points] vertex ids; the triangles are tris[] */
+  const int32_t* sc_edges;   /* [n_sc_edges][2] */
+  int32_t* sc_pt;            /* [sc_max_pairs][2] (vertex, triangle) candidates */
+  int32_t* sc_ee;            /* [sc_max_pairs][2] (edge, edge) candidates, e1 < e2 */
+  int32_t* sc_count;         /* [4] point-triangle and edge-edge counts, their broad-phase status (MFEM_BP_*) */
+  double* sc_g;              /* [n_verts][3] barrier gradient (FP64 sums) */
+  double* sc_h;              /* [n_verts][6] barrier Hessian diagonal blocks, sym6 order 00 11 22 01 02 12 */
+  float* sc_alpha;           /* [1] largest line-search step the CCD allows (<= 1) */
+  void* sc_ws;               /* broad-phase workspace */
+  int64_t sc_ws_bytes;
 } mfem_solver;
 
 /* ABI self-description (checked against the Python mirror by the tests) */
@@ -373,6 +407,32 @@ int mfem_broadphase_ee(const float* x, const int32_t* edges, int n_edges, float 
                        int32_t* pairs, int max_pairs, int32_t* n_pairs, int32_t* status, void* workspace,
                        size_t workspace_bytes, void* stream);
 size_t mfem_broadphase_workspace_bytes(int n_insert, int n_query, int max_span);
+
+/* The broad phase on swept primitives (kind 0: point-triangle, prims =
+   points, tris = triangles; kind 1: edge-edge, prims = edges, tris unused):
+   each vertex's box holds it at x and at x + dx (dx FP64, may be null for
+   the static boxes of mfem_broadphase_pt / _ee), so that the pairs cover
+   every step size in [0, 1] of the Newton step dx. */
+int mfem_broadphase_swept(int kind, const float* x, const double* dx, const int32_t* prims, int n_prims,
+                          const int32_t* tris, int n_tris, float d_hat, float cell, int max_span, int32_t* pairs,
+                          int max_pairs, int32_t* n_pairs, int32_t* status, void* workspace, size_t workspace_bytes,
+                          void* stream);
+
+/* Self-contact stages of mfem_step (mfem_selfcontact.hip; the fields
+   self_contact .. sc_ws_bytes of mfem_solver).  No counterpart in the
+   reference's live solve (the hook at src/mfem/solver.py:443-447 is
+   commented out); the distance primitives and barrier follow
+   src/mfem/collision.py:220-322, 529-546.
+   _candidates: point-triangle and edge-edge candidate lists from the
+   broad phase, on static boxes (swept = 0) or on the boxes swept by dx;
+   _eval: gradient and diagonal Hessian blocks into sc_g / sc_h;
+   _ccd: sc_alpha = the largest first line-search step (<= 1) that keeps
+   every candidate pair at a positive distance (additive CCD). */
+size_t mfem_self_contact_workspace_bytes(int n_points, int n_tris, int n_edges, int max_span);
+int mfem_self_contact_candidates(const mfem_solver* S, int swept, void* stream);
+int mfem_self_contact_eval(const mfem_solver* S, void* stream);
+int mfem_self_contact_ccd(const mfem_solver* S, void* stream);
+const char* mfem_self_contact_last_error(void);
 
 /* eval_soft_rigid_distance (src/mfem/collision.py:380-476): for rigid/soft
    candidate triangle pairs, per soft particle the minimum of the 15 feature

@@ -65,6 +65,17 @@ def _lib(bits: int, omp: bool = False) -> ctypes.CDLL:
     lib.orc_real_size.restype = ctypes.c_int
     lib.orc_set_threads.argtypes = [ctypes.c_int]
     lib.orc_set_threads.restype = ctypes.c_int
+    lib.orc_set_surface.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P]
+    lib.orc_set_surface.restype = None
+    lib.orc_sc_distance.argtypes = [ctypes.c_int, P, P, P]
+    lib.orc_sc_distance.restype = ctypes.c_double
+    lib.orc_sc_accd.argtypes = [ctypes.c_int, P, P, ctypes.c_double]
+    lib.orc_sc_accd.restype = ctypes.c_double
+    for name in ("orc_sc_energy", "orc_sc_gradient"):
+        getattr(lib, name).argtypes = [P, P]
+        getattr(lib, name).restype = ctypes.c_double
+    lib.orc_sc_toi.argtypes = [P, P, P]
+    lib.orc_sc_toi.restype = ctypes.c_double
     _LIBS[(bits, omp)] = lib
     return lib
 
@@ -76,7 +87,10 @@ _PARAM_NAMES = {
     "line_search_c": "ls_c", "line_search_threshold": "ls_thr", "attachment_stiffness": "attach_k",
     "contact_d0": "d0", "contact_d1": "d1", "contact_stiffness": "kappa", "friction_eps_v": "eps_v",
     "preconditioner_singular_threshold": "sing_thr",
+    "self_contact_dhat": "sc_dhat", "self_contact_kappa": "sc_kappa", "self_contact_ccd_s": "sc_ccd_s",
 }
+# solver keywords with no oracle counterpart (GPU data-structure sizes)
+_GPU_ONLY = {"self_contact_max_pairs", "self_contact_cell", "self_contact_max_span"}
 
 
 def _real(bits: int):
@@ -262,6 +276,19 @@ class Oracle:
         if nshapes:
             self.buf("shape_mu")[:] = mu
         op = dict(gx=float(scene.gravity[0]), gy=float(scene.gravity[1]), gz=float(scene.gravity[2]))
+        if p.get("self_contact", False):
+            # the solver's defaults: the vertex barrier's range and stiffness
+            p.setdefault("self_contact_dhat", p.get("contact_d1", 1.0e-1))
+            p.setdefault("self_contact_kappa", p.get("contact_stiffness", 1.0e7))
+            tri = np.asarray(scene.tri_indices, np.int32).reshape(-1, 3)
+            pts = np.unique(tri.reshape(-1)).astype(np.int32)
+            e = np.sort(np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]]), axis=1)
+            e = np.unique(e, axis=0).astype(np.int32)
+            self.lib.orc_set_surface(self.ctx, len(pts), pts.ctypes.data_as(ctypes.c_void_p), len(e),
+                                     np.ascontiguousarray(e).ctypes.data_as(ctypes.c_void_p))
+            self.surface_points, self.surface_edges = pts, e
+        for k in _GPU_ONLY:
+            p.pop(k, None)
         for k, v in p.items():
             if k == "energy":
                 v = {"arap": 0, "neohookean": 1}.get(v, v)
@@ -306,9 +333,11 @@ class Oracle:
         ptr = self.lib.orc_buf(self.ctx, name.encode(), ctypes.byref(n), ctypes.byref(k))
         if not ptr:
             raise KeyError(name)
+        kinds = {0: (self.dt, ctypes.c_double if self.bits == 64 else ctypes.c_float), 1: (np.int32, ctypes.c_int32),
+                 2: (np.float64, ctypes.c_double)}
+        npt, ctype = kinds[k.value]
         if n.value == 0:
-            return np.zeros(0, np.int32 if k.value else self.dt)
-        ctype = ctypes.c_int32 if k.value else (ctypes.c_double if self.bits == 64 else ctypes.c_float)
+            return np.zeros(0, npt)
         return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), shape=(n.value,))
 
     def param(self, name: str) -> float:
@@ -349,6 +378,29 @@ class Oracle:
 
     def step(self, dt):
         self.lib.orc_step(self.ctx, dt)
+
+    # self-contact (surface point-triangle / edge-edge barrier), all pairs
+    def _ptr(self, a):
+        a = np.ascontiguousarray(np.asarray(a, dtype=self.dt).reshape(-1))
+        return a, a.ctypes.data_as(ctypes.c_void_p)
+
+    def sc_energy(self, x=None) -> float:
+        """kappa sum b(d) over every surface pair below dhat at x (default: the state)."""
+        a, p = self._ptr(self.x if x is None else x)
+        return float(self.lib.orc_sc_energy(self.ctx, p))
+
+    def sc_gradient(self, x=None):
+        """(energy, gradient (V, 3), Gauss-Newton diagonal blocks (V, 6) in sym6 order)."""
+        a, p = self._ptr(self.x if x is None else x)
+        e = float(self.lib.orc_sc_gradient(self.ctx, p))
+        return e, self.buf("sc_g").reshape(-1, 3).copy(), self.buf("sc_h").reshape(-1, 6).copy()
+
+    def sc_toi(self, x, dx) -> float:
+        """Smallest additive-CCD time of impact (<= 1) over the pairs whose
+        boxes swept by dx overlap."""
+        a, p = self._ptr(x)
+        b, q = self._ptr(dx)
+        return float(self.lib.orc_sc_toi(self.ctx, p, q))
 
     def stats(self):
         n = int(self.param("iterations"))

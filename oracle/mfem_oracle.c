@@ -664,16 +664,22 @@ typedef struct {
   real *r, *z, *pp, *q, *xtrial, *strial;
   /* stats: per Newton iteration */
   int stat_n; real stats[4096];
+  /* self-contact: surface vertices and edges; barrier gradient and
+     diagonal Hessian blocks (FP64, sym6 order 00 11 22 01 02 12) */
+  int npts, nedges; int* pts; int* edges;
+  double *sc_g, *sc_h;
 } orc_ctx;
 
 enum {
   P_ENERGY = 0, P_GX, P_GY, P_GZ, P_ATTACH_K, P_D0, P_D1, P_KAPPA, P_EPSV,
   P_ITERS, P_CG_MAXIT, P_CG_TOL, P_PRECOND, P_LS, P_LS_MAX, P_LS_A0, P_LS_TAU,
-  P_LS_C, P_LS_THR, P_SING, P_TRI_CONTACT, P_GUESS, P_QB0, P_QB1, P_QB2, P_NPARAMS
+  P_LS_C, P_LS_THR, P_SING, P_TRI_CONTACT, P_GUESS, P_SC, P_SC_DHAT, P_SC_KAPPA, P_SC_S,
+  P_QB0, P_QB1, P_QB2, P_NPARAMS
 };
 static const char* PNAMES[] = {"energy", "gx", "gy", "gz", "attach_k", "d0", "d1", "kappa", "eps_v",
                                "iterations", "cg_maxiter", "cg_tol", "precond", "line_search", "ls_max", "ls_alpha0",
-                               "ls_tau", "ls_c", "ls_thr", "sing_thr", "tri_contact", "guess", NULL};
+                               "ls_tau", "ls_c", "ls_thr", "sing_thr", "tri_contact", "guess", "self_contact",
+                               "sc_dhat", "sc_kappa", "sc_ccd_s", NULL};
 
 #define ALLOC(ptr, n) ptr = calloc((size_t)((n) > 0 ? (n) : 1), sizeof(*(ptr)))
 
@@ -704,7 +710,18 @@ EXPORT orc_ctx* orc_create(int nv, int nt, int ntris, int nshapes, int npins) {
   C->p[P_ITERS] = 8; C->p[P_CG_MAXIT] = 1000; C->p[P_CG_TOL] = (real)1e-6; C->p[P_PRECOND] = 1;
   C->p[P_LS] = 1; C->p[P_LS_MAX] = 20; C->p[P_LS_A0] = 1; C->p[P_LS_TAU] = (real)0.5; C->p[P_LS_C] = (real)0.01;
   C->p[P_LS_THR] = (real)1e-8; C->p[P_SING] = (real)1e-20; C->p[P_TRI_CONTACT] = 1;
+  C->p[P_SC] = 0; C->p[P_SC_DHAT] = (real)1e-3; C->p[P_SC_KAPPA] = (real)1e7; C->p[P_SC_S] = (real)0.1;
+  ALLOC(C->sc_g, 3 * nv); ALLOC(C->sc_h, 6 * nv);
   return C;
+}
+
+/* surface vertices and edges for self-contact (the triangles are tris[]) */
+EXPORT void orc_set_surface(orc_ctx* C, int npts, const int* pts, int nedges, const int* edges) {
+  free(C->pts); free(C->edges);
+  C->npts = npts; C->nedges = nedges;
+  ALLOC(C->pts, npts); ALLOC(C->edges, 2 * nedges);
+  memcpy(C->pts, pts, sizeof(int) * (size_t)npts);
+  memcpy(C->edges, edges, sizeof(int) * 2 * (size_t)nedges);
 }
 
 EXPORT int orc_set_param(orc_ctx* C, const char* name, double value) {
@@ -718,7 +735,7 @@ EXPORT double orc_get_param(orc_ctx* C, const char* name) {
   return NAN;
 }
 
-/* buffer table for the ctypes wrapper: name -> pointer, element count, kind (0 real, 1 int) */
+/* buffer table for the ctypes wrapper: name -> pointer, element count, kind (0 real, 1 int, 2 double) */
 EXPORT void* orc_buf(orc_ctx* C, const char* name, int64_t* count, int* is_int) {
 #define B(nm, ptr, n, k) if (!strcmp(name, nm)) { *count = (int64_t)(n); *is_int = (k); return (void*)(ptr); }
   B("tets", C->tets, 4 * C->nt, 1) B("dminv", C->dminv, 9 * C->nt, 0) B("materials", C->mat, 3 * C->nt, 0)
@@ -736,7 +753,7 @@ EXPORT void* orc_buf(orc_ctx* C, const char* name, int64_t* count, int* is_int) 
   B("g", C->g, 3 * C->nv, 0) B("dx", C->dx, 3 * C->nv, 0) B("ds", C->ds, 6 * C->nt, 0) B("Minv", C->Minv, 9 * C->nv, 0)
   B("rowptr", C->rowptr, C->nv + 1, 1) B("col", C->col, C->nnzb, 1) B("val", C->val, 9 * C->nnzb, 0)
   B("diag", C->diag, C->nv, 1) B("tdist", C->tdist, C->ntris, 0) B("tbary", C->tbary, 3 * C->ntris, 0)
-  B("stats", C->stats, 4096, 0)
+  B("stats", C->stats, 4096, 0) B("sc_g", C->sc_g, 3 * C->nv, 2) B("sc_h", C->sc_h, 6 * C->nv, 2)
 #undef B
   *count = 0; *is_int = 0;
   return NULL;
@@ -960,6 +977,199 @@ static int pin_of(orc_ctx* C, int i) {
 }
 
 /* _update_system (solver.py:633-639, 930-980) */
+/* ---------------------------------------------------------------------------
+   Self-contact: IPC barrier on point-triangle and edge-edge pairs of surface
+   primitives inside the Newton solve.  The reference's live solve has no
+   such term (the hook at src/mfem/solver.py:443-447 is commented out); the
+   primitives follow src/mfem/collision.py:220-322 (Ericson closest points)
+   and the barrier b(d) = -(d - dhat)^2 ln(d / dhat) follows
+   collision.py:529-546.  Energy kappa b(d) per pair below dhat, gradient
+   kappa b'(d) cw_j n per vertex (n the unit vector between the closest
+   points, cw_j the vertex's weight in their difference), Gauss-Newton
+   diagonal blocks kappa max(b'', 0) cw_j^2 n n^T; the line search's first
+   step is cut to the additive-CCD time of impact (Li, Kaufman, Jiang 2021,
+   Alg. 1).  Every pair is enumerated here (boxes inflated by dhat / 2 as a
+   prefilter), in FP64 whatever the build's real type.
+   --------------------------------------------------------------------------- */
+static double dot3d(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+static void cp_tri_bary_d(const double* x, const double* a, const double* b, const double* c, double* w) {
+  double ab[3], ac[3], ax[3], bx[3], cx[3];
+  for (int i = 0; i < 3; ++i) { ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i]; ax[i] = x[i] - a[i]; bx[i] = x[i] - b[i]; cx[i] = x[i] - c[i]; }
+  double d1 = dot3d(ab, ax), d2 = dot3d(ac, ax);
+  if (d1 <= 0 && d2 <= 0) { w[0] = 1; w[1] = 0; w[2] = 0; return; }
+  double d3 = dot3d(ab, bx), d4 = dot3d(ac, bx);
+  if (d3 >= 0 && d4 <= d3) { w[0] = 0; w[1] = 1; w[2] = 0; return; }
+  double d5 = dot3d(ab, cx), d6 = dot3d(ac, cx);
+  if (d6 >= 0 && d5 <= d6) { w[0] = 0; w[1] = 0; w[2] = 1; return; }
+  double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) { double v = d1 / (d1 - d3); w[0] = 1 - v; w[1] = v; w[2] = 0; return; }
+  double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0 && d2 >= 0 && d6 <= 0) { double t = d2 / (d2 - d6); w[0] = 1 - t; w[1] = 0; w[2] = t; return; }
+  double va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    double t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    w[0] = 0; w[1] = 1 - t; w[2] = t; return;
+  }
+  double tot = va + vb + vc;
+  if (tot <= 0) { w[0] = w[1] = w[2] = 1.0 / 3.0; return; }
+  double v = vb / tot, t = vc / tot;
+  w[0] = 1 - v - t; w[1] = v; w[2] = t;
+}
+
+/* parameters (s, t) of the closest points of segments p1q1, p2q2 (Ericson 5.1.9) */
+static void seg_seg_d(const double* p1, const double* q1, const double* p2, const double* q2, double* s_out, double* t_out) {
+  double d1[3], d2[3], r[3];
+  for (int i = 0; i < 3; ++i) { d1[i] = q1[i] - p1[i]; d2[i] = q2[i] - p2[i]; r[i] = p1[i] - p2[i]; }
+  double a = dot3d(d1, d1), e = dot3d(d2, d2), f = dot3d(d2, r), eps = 1e-12, s, t;
+  if (a <= eps && e <= eps) { s = 0; t = 0; }
+  else if (a <= eps) { s = 0; t = f / e; t = t < 0 ? 0 : (t > 1 ? 1 : t); }
+  else {
+    double c = dot3d(d1, r);
+    if (e <= eps) { t = 0; s = -c / a; s = s < 0 ? 0 : (s > 1 ? 1 : s); }
+    else {
+      double b = dot3d(d1, d2), den = a * e - b * b;
+      s = den != 0 ? (b * f - c * e) / den : 0; s = s < 0 ? 0 : (s > 1 ? 1 : s);
+      t = (b * s + f) / e;
+      if (t < 0) { t = 0; s = -c / a; s = s < 0 ? 0 : (s > 1 ? 1 : s); }
+      else if (t > 1) { t = 1; s = (b - c) / a; s = s < 0 ? 0 : (s > 1 ? 1 : s); }
+    }
+  }
+  *s_out = s; *t_out = t;
+}
+
+/* distance of pair kind (0: point X0 vs triangle X1 X2 X3; 1: edge X0X1 vs
+   edge X2X3), the vertices' weights cw and the unit vector n */
+EXPORT double orc_sc_distance(int kind, const double* X, double* cw, double* n) {
+  double dv[3];
+  if (kind == 0) {
+    double w[3];
+    cp_tri_bary_d(X, X + 3, X + 6, X + 9, w);
+    for (int c = 0; c < 3; ++c) dv[c] = X[c] - (w[0] * X[3 + c] + w[1] * X[6 + c] + w[2] * X[9 + c]);
+    cw[0] = 1; cw[1] = -w[0]; cw[2] = -w[1]; cw[3] = -w[2];
+  } else {
+    double s, t;
+    seg_seg_d(X, X + 3, X + 6, X + 9, &s, &t);
+    for (int c = 0; c < 3; ++c) dv[c] = (X[c] + s * (X[3 + c] - X[c])) - (X[6 + c] + t * (X[9 + c] - X[6 + c]));
+    cw[0] = 1 - s; cw[1] = s; cw[2] = -(1 - t); cw[3] = -t;
+  }
+  double d = sqrt(dot3d(dv, dv)), inv = d > 0 ? 1.0 / d : 0.0;
+  for (int c = 0; c < 3; ++c) n[c] = dv[c] * inv;
+  return d;
+}
+
+static void ipc64(double d, double dh, double* e, double* de, double* d2e) {
+  double l = log(d / dh), u = d - dh;
+  *e = -u * u * l;
+  *de = -(2 * u * l + u * u / d);
+  *d2e = -(2 * l + 4 * u / d - u * u / (d * d));
+}
+
+/* additive CCD time of impact of pair kind along displacements P (<= 1) */
+EXPORT double orc_sc_accd(int kind, const double* X0, const double* P0, double s) {
+  double X[12], P[12], m[3] = {0, 0, 0}, len[4], cw[4], n[3];
+  memcpy(X, X0, sizeof X);
+  memcpy(P, P0, sizeof P);
+  for (int j = 0; j < 4; ++j) for (int c = 0; c < 3; ++c) m[c] += 0.25 * P[3 * j + c];
+  for (int j = 0; j < 4; ++j) {
+    for (int c = 0; c < 3; ++c) P[3 * j + c] -= m[c];
+    len[j] = sqrt(dot3d(P + 3 * j, P + 3 * j));
+  }
+  double lp = kind == 0 ? len[0] + fmax(len[1], fmax(len[2], len[3])) : fmax(len[0], len[1]) + fmax(len[2], len[3]);
+  if (!(lp > 0)) return 1.0;
+  double d = orc_sc_distance(kind, X, cw, n);
+  if (!(d > 0)) return 0.0;
+  double g = s * d, t = 0, tl = (1 - s) * d / lp;
+  for (int it = 0; it < 256; ++it) {
+    for (int k = 0; k < 12; ++k) X[k] += tl * P[k];
+    d = orc_sc_distance(kind, X, cw, n);
+    if (t > 0 && d < g) return t;
+    t += tl;
+    if (t > 1) return 1.0;
+    tl = 0.9 * d / lp;
+  }
+  return t;
+}
+
+/* boxes of the pair's two primitives (vertices 0..na-1 and na..3) at x and,
+   with dxs, at x + dxs, inflated by r each: do they overlap? */
+static int sc_boxes_overlap(const double* X, const double* P, int na, double r) {
+  double lo[2][3], hi[2][3];
+  for (int g = 0; g < 2; ++g)
+    for (int c = 0; c < 3; ++c) { lo[g][c] = INFINITY; hi[g][c] = -INFINITY; }
+  for (int j = 0; j < 4; ++j) {
+    int g = j < na ? 0 : 1;
+    for (int c = 0; c < 3; ++c) {
+      double a = X[3 * j + c], b = P ? a + P[3 * j + c] : a;
+      lo[g][c] = fmin(lo[g][c], fmin(a, b));
+      hi[g][c] = fmax(hi[g][c], fmax(a, b));
+    }
+  }
+  for (int c = 0; c < 3; ++c)
+    if (lo[0][c] - r > hi[1][c] + r || lo[1][c] - r > hi[0][c] + r) return 0;
+  return 1;
+}
+
+/* mode 0: gradient / diagonal Hessian blocks at x into sc_g / sc_h (returns
+   the energy); mode 1: energy at x; mode 2: the smallest time of impact
+   along dxs (returns it, <= 1) */
+static double sc_pass(orc_ctx* C, const real* x, const real* dxs, int mode) {
+  const double dh = C->p[P_SC_DHAT], kap = C->p[P_SC_KAPPA], cs = C->p[P_SC_S];
+  double E = 0, toi = 1.0;
+  if (mode == 0) {
+    memset(C->sc_g, 0, sizeof(double) * 3 * C->nv);
+    memset(C->sc_h, 0, sizeof(double) * 6 * C->nv);
+  }
+  for (int kind = 0; kind < 2; ++kind) {
+    int na = kind == 0 ? 1 : 2;
+    int nA = kind == 0 ? C->npts : C->nedges, nB = kind == 0 ? C->ntris : C->nedges;
+    for (int a = 0; a < nA; ++a)
+      for (int bidx = kind == 0 ? 0 : a + 1; bidx < nB; ++bidx) {
+        int v[4];
+        if (kind == 0) {
+          const int* t = C->tris + 3 * bidx;
+          v[0] = C->pts[a]; v[1] = t[0]; v[2] = t[1]; v[3] = t[2];
+          if (v[0] == v[1] || v[0] == v[2] || v[0] == v[3]) continue;
+        } else {
+          v[0] = C->edges[2 * a]; v[1] = C->edges[2 * a + 1]; v[2] = C->edges[2 * bidx]; v[3] = C->edges[2 * bidx + 1];
+          if (v[0] == v[2] || v[0] == v[3] || v[1] == v[2] || v[1] == v[3]) continue;
+        }
+        double X[12], P[12], cw[4], n[3];
+        for (int j = 0; j < 4; ++j)
+          for (int c = 0; c < 3; ++c) {
+            X[3 * j + c] = (double)x[3 * v[j] + c];
+            P[3 * j + c] = dxs ? (double)dxs[3 * v[j] + c] : 0.0;
+          }
+        if (!sc_boxes_overlap(X, mode == 2 ? P : NULL, na, 0.5 * dh)) continue;
+        if (mode == 2) {
+          double t = orc_sc_accd(kind, X, P, cs);
+          if (t < toi) toi = t;
+          continue;
+        }
+        double d = orc_sc_distance(kind, X, cw, n);
+        if (!(d < dh)) continue;
+        if (!(d > 0)) { E += 1e300; continue; }
+        double e, de, d2e;
+        ipc64(d, dh, &e, &de, &d2e);
+        E += kap * e;
+        if (mode == 0) {
+          double h = kap * fmax(d2e, 0.0);
+          double nn[6] = {n[0] * n[0], n[1] * n[1], n[2] * n[2], n[0] * n[1], n[0] * n[2], n[1] * n[2]};
+          for (int j = 0; j < 4; ++j) {
+            for (int c = 0; c < 3; ++c) C->sc_g[3 * v[j] + c] += kap * de * cw[j] * n[c];
+            for (int q = 0; q < 6; ++q) C->sc_h[6 * v[j] + q] += h * cw[j] * cw[j] * nn[q];
+          }
+        }
+      }
+  }
+  return mode == 2 ? toi : E;
+}
+
+/* the pass functions for the tests (x, dxs: the oracle's real type) */
+EXPORT double orc_sc_energy(orc_ctx* C, const real* x) { return sc_pass(C, x, NULL, 1); }
+EXPORT double orc_sc_gradient(orc_ctx* C, const real* x) { return sc_pass(C, x, NULL, 0); }
+EXPORT double orc_sc_toi(orc_ctx* C, const real* x, const real* dxs) { return sc_pass(C, x, dxs, 2); }
+
 EXPORT void orc_update_system(orc_ctx* C, real dt) {
   int nv = C->nv, nt = C->nt;
   real idt2 = 1 / (dt * dt);
@@ -1007,6 +1217,7 @@ EXPORT void orc_update_system(orc_ctx* C, real dt) {
       }
     }
   }
+  if (C->p[P_SC] != 0) sc_pass(C, C->x, NULL, 0);
   int energy = (int)C->p[P_ENERGY];
 #pragma omp parallel for schedule(static)
   for (int e = 0; e < nt; ++e) {
@@ -1083,6 +1294,13 @@ EXPORT void orc_assemble(orc_ctx* C, real dt) {
       C->g[3 * i + r] -= C->gB[3 * i + r] + C->gD[3 * i + r];
     }
     for (int q = 0; q < 9; ++q) blk[q] += C->HB[9 * i + q] + C->HD[9 * i + q];
+    if (C->p[P_SC] != 0) {
+      /* self-contact gradient and diagonal block (sym6 order 00 11 22 01 02 12) */
+      const double* h = C->sc_h + 6 * i;
+      const int map[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+      for (int r = 0; r < 3; ++r) C->g[3 * i + r] -= (real)C->sc_g[3 * i + r];
+      for (int q = 0; q < 9; ++q) blk[q] += (real)h[map[q]];
+    }
   }
   /* triangle-capsule barrier Hessian blocks on the tri's vertex pairs; its
      energy and gradient were folded into eB / gB by orc_update_system
@@ -1271,6 +1489,7 @@ EXPORT double orc_merit(orc_ctx* C, const real* x, const real* s, real dt) {
       }
     }
   }
+  if (C->p[P_SC] != 0) bar += sc_pass(C, x, NULL, 1);
   return kin * idt2 + att * idt2 + ela + bar + fri;
 }
 
@@ -1296,6 +1515,8 @@ EXPORT int orc_line_search(orc_ctx* C, real dt, int* accepted, double* alpha_out
     }
   }
   gdx = vdot(gxf, C->dx, 3 * nv);
+  if (C->p[P_SC] != 0)
+    for (int k = 0; k < 3 * nv; ++k) gdx += C->sc_g[k] * (double)C->dx[k];
 #pragma omp parallel for schedule(static) reduction(+ : gds)
   for (int e = 0; e < nt; ++e)
     for (int k = 0; k < 6; ++k) gds += ((double)C->gs[6 * e + k] + (double)SYMD[k] * (double)C->lam[6 * e + k]) * (double)C->ds[6 * e + k];
@@ -1304,6 +1525,11 @@ EXPORT int orc_line_search(orc_ctx* C, real dt, int* accepted, double* alpha_out
   double t = -c * gdx + -c * gds;
   double f0 = orc_merit(C, C->x, C->s, dt);
   double alpha = C->p[P_LS_A0];
+  if (C->p[P_SC] != 0) {
+    /* the first step cut to the self-contact time of impact */
+    double toi = sc_pass(C, C->x, C->dx, 2);
+    if (toi < alpha) alpha = (double)(float)toi;
+  }
   int it = 0, cond;
   do {
     for (int k = 0; k < 3 * nv; ++k) C->xtrial[k] = C->x[k] + (real)alpha * C->dx[k];
@@ -1337,8 +1563,10 @@ EXPORT void orc_step(orc_ctx* C, real dt) {
     double alpha = 1;
     if (C->p[P_LS] != 0) lsit = orc_line_search(C, dt, &acc, &alpha);
     else {
-      for (int k = 0; k < 3 * C->nv; ++k) C->x[k] += C->dx[k];
-      for (int k = 0; k < 6 * C->nt; ++k) C->s[k] += C->ds[k];
+      /* (with self-contact: the step cut to the time of impact) */
+      real a = C->p[P_SC] != 0 ? (real)(float)sc_pass(C, C->x, C->dx, 2) : (real)1;
+      for (int k = 0; k < 3 * C->nv; ++k) C->x[k] += a * C->dx[k];
+      for (int k = 0; k < 6 * C->nt; ++k) C->s[k] += a * C->ds[k];
     }
     if (4 * it + 3 < 4096) {
       C->stats[4 * it + 0] = (real)cgit;
@@ -1358,7 +1586,8 @@ EXPORT void orc_destroy(orc_ctx* C) {
                   C->val, C->x, C->v, C->s, C->lam, C->xt, C->xstart, C->xin, C->lag_n, C->lag_fn, C->lag_sid,
                   C->c, C->G, C->gs, C->Hs, C->Hsinv, C->Hl, C->gl, C->psi, C->gx, C->gB, C->gD, C->HB, C->HD,
                   C->dist, C->ddist, C->hdist, C->sid, C->eB, C->eD, C->g, C->dx, C->ds, C->Minv, C->tdist,
-                  C->tbary, C->tnorm, C->tsid, C->r, C->z, C->pp, C->q, C->xtrial, C->strial};
+                  C->tbary, C->tnorm, C->tsid, C->r, C->z, C->pp, C->q, C->xtrial, C->strial, C->pts, C->edges,
+                  C->sc_g, C->sc_h};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k) free(ptrs[k]);
   free(C);
 }

@@ -326,15 +326,28 @@ __device__ __forceinline__ uint32_t cell_hash(int ix, int iy, int iz, uint32_t m
   return ((uint32_t)ix * 73856093u ^ (uint32_t)iy * 19349663u ^ (uint32_t)iz * 83492791u) & mask;
 }
 // box of nv vertices (ids) inflated by r, in the same FP32 operations as
-// the brute-force reference (min / max, then -r / +r)
+// the brute-force reference (min / max, then -r / +r); with dx (FP64
+// displacements, may be null) the box of the swept primitive: every vertex
+// at x and at x + dx (so that the box holds the primitive at x + a dx for
+// every a in [0, 1])
 template <int NV>
-__device__ Box prim_box(const float* x, const int32_t* ids, float r) {
+__device__ Box prim_box(const float* x, const double* dx, const int32_t* ids, float r) {
   f3 lo = ld3(x, ids[0]), hi = lo;
 #pragma unroll
   for (int k = 1; k < NV; ++k) {
     const f3 v = ld3(x, ids[k]);
     lo = {fminf(lo.x, v.x), fminf(lo.y, v.y), fminf(lo.z, v.z)};
     hi = {fmaxf(hi.x, v.x), fmaxf(hi.y, v.y), fmaxf(hi.z, v.z)};
+  }
+  if (dx) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const f3 v = ld3(x, ids[k]);
+      const double* d = dx + 3 * (size_t)ids[k];
+      const f3 w = {v.x + (float)d[0], v.y + (float)d[1], v.z + (float)d[2]};
+      lo = {fminf(lo.x, w.x), fminf(lo.y, w.y), fminf(lo.z, w.z)};
+      hi = {fmaxf(hi.x, w.x), fmaxf(hi.y, w.y), fmaxf(hi.z, w.z)};
+    }
   }
   return {{lo.x - r, lo.y - r, lo.z - r}, {hi.x + r, hi.y + r, hi.z + r}};
 }
@@ -363,12 +376,13 @@ __global__ void k_fill_i32(int32_t* p, long n, int32_t v) {
 
 // insertion pass: MODE 0 counts per bucket, MODE 1 writes entries
 template <int NV, int MODE>
-__global__ void __launch_bounds__(kThreads) k_insert(const float* x, const int32_t* prims, int n, float r, Grid g,
+__global__ void __launch_bounds__(kThreads) k_insert(const float* x, const double* dx, const int32_t* prims, int n,
+                                                     float r, Grid g,
                                                      int32_t* counts, const int32_t* start, int32_t* cursor,
                                                      Entry* entries, int cap, int32_t* status) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
-  const Box b = prim_box<NV>(x, prims + NV * i, r);
+  const Box b = prim_box<NV>(x, dx, prims + NV * i, r);
   int lo[3], hi[3];
   if (!cell_range(b, g, lo, hi)) {
     atomicOr(status, MFEM_BP_SPAN);
@@ -392,7 +406,7 @@ __global__ void __launch_bounds__(kThreads) k_insert(const float* x, const int32
 // ENV = 2, only partners with a larger id); MODE 0 counts, MODE 1 writes and
 // sorts the query's segment
 template <int QNV, int ENV, int MODE>
-__global__ void __launch_bounds__(kThreads) k_query(const float* x, const int32_t* qprims, int nq,
+__global__ void __launch_bounds__(kThreads) k_query(const float* x, const double* dx, const int32_t* qprims, int nq,
                                                     const int32_t* eprims, float r, Grid g, const int32_t* counts,
                                                     const int32_t* start, const Entry* entries, int cap,
                                                     int32_t* qcount, const int32_t* qstart, int32_t* pairs,
@@ -400,7 +414,7 @@ __global__ void __launch_bounds__(kThreads) k_query(const float* x, const int32_
   const int q = blockIdx.x * kThreads + threadIdx.x;
   if (q >= nq) return;
   const int32_t* qv = qprims + QNV * q;
-  const Box bq = prim_box<QNV>(x, qv, r);
+  const Box bq = prim_box<QNV>(x, dx, qv, r);
   int lo[3], hi[3];
   if (!cell_range(bq, g, lo, hi)) {
     atomicOr(status, MFEM_BP_SPAN);
@@ -428,7 +442,7 @@ __global__ void __launch_bounds__(kThreads) k_query(const float* x, const int32_
 #pragma unroll
             for (int b = 0; b < ENV; ++b) shared |= qv[a] == ev[b];
           if (shared) continue;
-          const Box be = prim_box<ENV>(x, ev, r);
+          const Box be = prim_box<ENV>(x, dx, ev, r);
           if (!overlap(bq, be)) continue;
           // report in the cell holding the minimum corner of the intersection
           if (cell_of(fmaxf(bq.lo.x, be.lo.x), g.inv_h) != ix || cell_of(fmaxf(bq.lo.y, be.lo.y), g.inv_h) != iy ||
@@ -553,7 +567,8 @@ BpLayout bp_layout(long n_insert, long n_query, int max_span) {
 }
 
 template <int QNV, int ENV>
-int broadphase(const float* x, const int32_t* qprims, int nq, const int32_t* eprims, int ne, float d_hat, float cell,
+int broadphase(const float* x, const double* dx, const int32_t* qprims, int nq, const int32_t* eprims, int ne,
+               float d_hat, float cell,
                int max_span, int32_t* pairs, int max_pairs, int32_t* n_pairs, int32_t* status, void* workspace,
                size_t ws_bytes, hipStream_t st) {
   if (!x || !qprims || !eprims || !pairs || !n_pairs || !status || !workspace) return fail(MFEM_EINVAL, "null pointer");
@@ -579,22 +594,22 @@ int broadphase(const float* x, const int32_t* qprims, int nq, const int32_t* epr
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kThreads), 0, st, n_pairs, 1L, 0);
   if (int rc = launched()) return rc;
   if (ne > 0) {
-    hipLaunchKernelGGL((k_insert<ENV, 0>), dim3(blocks(ne)), dim3(kThreads), 0, st, x, eprims, ne, r, g, counts,
+    hipLaunchKernelGGL((k_insert<ENV, 0>), dim3(blocks(ne)), dim3(kThreads), 0, st, x, dx, eprims, ne, r, g, counts,
                        (const int32_t*)nullptr, cursor, entries, L.cap, status);
     if (int rc = launched()) return rc;
     if (int rc = scan_i32(counts, start, L.tsize, 0, total, sws, sws_bytes, st)) return rc;
-    hipLaunchKernelGGL((k_insert<ENV, 1>), dim3(blocks(ne)), dim3(kThreads), 0, st, x, eprims, ne, r, g, counts,
+    hipLaunchKernelGGL((k_insert<ENV, 1>), dim3(blocks(ne)), dim3(kThreads), 0, st, x, dx, eprims, ne, r, g, counts,
                        start, cursor, entries, L.cap, status);
     if (int rc = launched()) return rc;
   } else {
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks(L.tsize)), dim3(kThreads), 0, st, start, (long)L.tsize, 0);
   }
   if (nq == 0) return launched();
-  hipLaunchKernelGGL((k_query<QNV, ENV, 0>), dim3(blocks(nq)), dim3(kThreads), 0, st, x, qprims, nq, eprims, r, g,
+  hipLaunchKernelGGL((k_query<QNV, ENV, 0>), dim3(blocks(nq)), dim3(kThreads), 0, st, x, dx, qprims, nq, eprims, r, g,
                      counts, start, entries, L.cap, qcount, (const int32_t*)nullptr, pairs, max_pairs, status);
   if (int rc = launched()) return rc;
   if (int rc = scan_i32(qcount, qstart, nq, 0, n_pairs, sws, sws_bytes, st)) return rc;
-  hipLaunchKernelGGL((k_query<QNV, ENV, 1>), dim3(blocks(nq)), dim3(kThreads), 0, st, x, qprims, nq, eprims, r, g,
+  hipLaunchKernelGGL((k_query<QNV, ENV, 1>), dim3(blocks(nq)), dim3(kThreads), 0, st, x, dx, qprims, nq, eprims, r, g,
                      counts, start, entries, L.cap, qcount, qstart, pairs, max_pairs, status);
   return launched();
 }
@@ -641,15 +656,28 @@ size_t mfem_broadphase_workspace_bytes(int n_insert, int n_query, int max_span) 
 int mfem_broadphase_pt(const float* x, const int32_t* points, int n_points, const int32_t* tris, int n_tris,
                        float d_hat, float cell, int max_span, int32_t* pairs, int max_pairs, int32_t* n_pairs,
                        int32_t* status, void* workspace, size_t workspace_bytes, void* stream) {
-  return broadphase<1, 3>(x, points, n_points, tris, n_tris, d_hat, cell, max_span, pairs, max_pairs, n_pairs, status,
-                          workspace, workspace_bytes, (hipStream_t)stream);
+  return broadphase<1, 3>(x, nullptr, points, n_points, tris, n_tris, d_hat, cell, max_span, pairs, max_pairs, n_pairs,
+                          status, workspace, workspace_bytes, (hipStream_t)stream);
 }
 
 int mfem_broadphase_ee(const float* x, const int32_t* edges, int n_edges, float d_hat, float cell, int max_span,
                        int32_t* pairs, int max_pairs, int32_t* n_pairs, int32_t* status, void* workspace,
                        size_t workspace_bytes, void* stream) {
-  return broadphase<2, 2>(x, edges, n_edges, edges, n_edges, d_hat, cell, max_span, pairs, max_pairs, n_pairs, status,
-                          workspace, workspace_bytes, (hipStream_t)stream);
+  return broadphase<2, 2>(x, nullptr, edges, n_edges, edges, n_edges, d_hat, cell, max_span, pairs, max_pairs, n_pairs,
+                          status, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int mfem_broadphase_swept(int kind, const float* x, const double* dx, const int32_t* prims, int n_prims,
+                          const int32_t* tris, int n_tris, float d_hat, float cell, int max_span, int32_t* pairs,
+                          int max_pairs, int32_t* n_pairs, int32_t* status, void* workspace, size_t workspace_bytes,
+                          void* stream) {
+  if (kind == 0)
+    return broadphase<1, 3>(x, dx, prims, n_prims, tris, n_tris, d_hat, cell, max_span, pairs, max_pairs, n_pairs,
+                            status, workspace, workspace_bytes, (hipStream_t)stream);
+  if (kind == 1)
+    return broadphase<2, 2>(x, dx, prims, n_prims, prims, n_prims, d_hat, cell, max_span, pairs, max_pairs, n_pairs,
+                            status, workspace, workspace_bytes, (hipStream_t)stream);
+  return fail(MFEM_EINVAL, "mfem_broadphase_swept: kind must be 0 (point-triangle) or 1 (edge-edge)");
 }
 
 int mfem_soft_rigid_distance(const float* points, const int32_t* tris, int rigid_points_count, int rigid_tri_count,

@@ -14,6 +14,7 @@
 //   k_local        per tet      lambda, ds
 //   k_linesearch   persistent, cooperative: whole backtracking loop, FP64 merit
 //   k_end_step     per vertex   velocity
+//   (self-contact, mfem_selfcontact.hip: candidates, pair barrier, CCD)
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -24,6 +25,7 @@
 
 #include "../../include/mfem_amd.h"
 #include "mfem_math.h"
+#include "mfem_selfcontact.h"
 
 using namespace mfem;
 
@@ -633,15 +635,27 @@ __global__ void __launch_bounds__(kBlock) k_rhs(mfem_solver S, float dt) {
     g[1] += f[1];
     g[2] += f[2];
   }
+  float gsc[3] = {0.f, 0.f, 0.f}, hsc[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (self_contact_on(S)) {
+    // the self-contact barrier's gradient and diagonal block (FP64 sums)
 #pragma unroll
-  for (int k = 0; k < 3; ++k) S.g[3 * i + k] = g[k] - S.gB[3 * i + k] - S.gD[3 * i + k];
+    for (int k = 0; k < 3; ++k) gsc[k] = (float)S.sc_g[3 * i + k];
+    const double* h = S.sc_h + 6 * (size_t)i;
+    const float h6[6] = {(float)h[0], (float)h[1], (float)h[2], (float)h[3], (float)h[4], (float)h[5]};
+    hsc[0] = h6[0]; hsc[4] = h6[1]; hsc[8] = h6[2];
+    hsc[1] = hsc[3] = h6[3];
+    hsc[2] = hsc[6] = h6[4];
+    hsc[5] = hsc[7] = h6[5];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) S.g[3 * i + k] = g[k] - S.gB[3 * i + k] - S.gD[3 * i + k] - gsc[k];
   const float idt2 = 1.f / (dt * dt);
   float im = S.inv_mass[i];
   float md = (im == 0.f ? 1.f + S.attach_k : 1.f / im) * idt2;
   int dl = S.diag_lane[i];
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
-    float add = S.Hv[9 * i + q] + ((q % 4 == 0) ? md : 0.f);
+    float add = S.Hv[9 * i + q] + ((q % 4 == 0) ? md : 0.f) + hsc[q];
     S.sell_val[val_index(dl, q)] += add;
   }
 }
@@ -2808,6 +2822,9 @@ __device__ double vertex_slope_own(const mfem_solver& S, int i) {
 #pragma unroll
   for (int k = 0; k < 3; ++k)
     acc += (double)(S.gx[3 * i + k] + S.gB[3 * i + k] + S.gD[3 * i + k]) * S.dx[3 * i + k];
+  if (self_contact_on(S))
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc += S.sc_g[3 * i + k] * S.dx[3 * i + k];
   return acc;
 }
 
@@ -2858,7 +2875,10 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
   // the first pass also evaluates the first trial (alpha0): one pass and one
   // grid reduction fewer when it is accepted.  Each sum is formed in the same
   // order as in a trial pass of its own, so f1 is the same number.
-  const float a_first = (float)S.ls_alpha0;
+  // with self-contact the first step is cut to the CCD bound (<= 1)
+  const bool sc = self_contact_on(S);
+  const double alpha0 = sc ? fmin((double)S.ls_alpha0, (double)S.sc_alpha[0]) : (double)S.ls_alpha0;
+  const float a_first = (float)alpha0;
   double f = 0.0, sl = 0.0, fa = 0.0;
   for (int i = v0 + tid; i < v1; i += nth) {
     f += vertex_merit(S, i, 0.f, dt);
@@ -2875,11 +2895,22 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
       f += tri_merit(S, t, 0.f);
       fa += tri_merit(S, t, a_first);
     }
+  if (sc) {
+    const int npt = sc_pair_count(S, 0), nee = sc_pair_count(S, 1);
+    for (int k = tid; k < npt; k += nth) {
+      f += sc_pair_merit<0>(S, k, 0.f);
+      fa += sc_pair_merit<0>(S, k, a_first);
+    }
+    for (int k = tid; k < nee; k += nth) {
+      f += sc_pair_merit<1>(S, k, 0.f);
+      fa += sc_pair_merit<1>(S, k, a_first);
+    }
+  }
   double r0[3] = {f, sl, fa};
   if (!grid_reduce(gs, r0)) return;
   const double f0 = r0[0];
   const double t = -(double)S.ls_c * r0[1];
-  double alpha = S.ls_alpha0, used = alpha, f1 = f0;
+  double alpha = alpha0, used = alpha, f1 = f0;
   int it = 0;
   bool cond;
   do {
@@ -2892,6 +2923,11 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
       for (int e = e0 + tid; e < e1; e += nth) f += tet_merit(S, e, a);
       if (tri_contact_on(S))
         for (int t = f0t + tid; t < f1t; t += nth) f += tri_merit(S, t, a);
+      if (sc) {
+        const int npt = sc_pair_count(S, 0), nee = sc_pair_count(S, 1);
+        for (int k = tid; k < npt; k += nth) f += sc_pair_merit<0>(S, k, a);
+        for (int k = tid; k < nee; k += nth) f += sc_pair_merit<1>(S, k, a);
+      }
       double r1[1] = {f};
       if (!grid_reduce(gs, r1)) return;
       f1 = r1[0];
@@ -2927,12 +2963,14 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
 __global__ void __launch_bounds__(kBlock) k_plain_update(mfem_solver S, int newton_iter) {
   if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   int i = blockIdx.x * kBlock + threadIdx.x;
+  // (with self-contact, the CCD-bounded step: x + a dx, s + a ds)
+  const float a = self_contact_on(S) ? S.sc_alpha[0] : 1.f;
   if (i < S.n_verts)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) S.x[3 * i + k] += (float)S.dx[3 * i + k];
+    for (int k = 0; k < 3; ++k) S.x[3 * i + k] += a * (float)S.dx[3 * i + k];
   if (i < S.n_tets)
 #pragma unroll
-    for (int k = 0; k < 6; ++k) S.s[6 * i + k] += S.ds[6 * i + k];
+    for (int k = 0; k < 6; ++k) S.s[6 * i + k] += a * S.ds[6 * i + k];
   if (i < (S.n_bodies > 1 ? S.n_bodies : 1) && newton_iter >= 0 && newton_iter < MFEM_MAX_STATS_ITERS) {
     double* st = S.stats + ((size_t)i * MFEM_MAX_STATS_ITERS + newton_iter) * MFEM_STATS_PER_ITER;
     st[2] = 0.0;
@@ -3099,6 +3137,12 @@ int check_solver(const mfem_solver* S) {
     if (S->ls_blocks < S->n_bodies || S->ls_blocks > MFEM_RED_MAX_BLOCKS)
       return fail(MFEM_EINVAL, "ls_blocks must be in [n_bodies, MFEM_RED_MAX_BLOCKS]");
   }
+  if (S->self_contact) {
+    if (S->n_bodies > 1) return fail(MFEM_EINVAL, "self-contact needs a single body (n_bodies <= 1)");
+    if (!S->sc_pt || !S->sc_ee || !S->sc_count || !S->sc_g || !S->sc_h || !S->sc_alpha || !S->sc_ws ||
+        (S->n_sc_points > 0 && !S->sc_points) || (S->n_sc_edges > 0 && !S->sc_edges) || (S->n_tris > 0 && !S->tris))
+      return fail(MFEM_EINVAL, "self-contact enabled without its buffers");
+  }
   if (S->tri_contact && S->n_tris > 0 &&
       (!S->tris || !S->tcon_ptr || !S->tcon_src || !S->vtri_ptr || !S->vtri_src || !S->tri_dist || !S->tri_bary ||
        !S->tri_nrm || !S->tri_sid))
@@ -3124,7 +3168,7 @@ int launched() {
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
   X(lag_sid) X(G) X(Hl) X(gl) X(Hsinv) X(gs) X(psi) X(Kb) X(fe) X(gx) X(gB) X(gD) X(Hv) X(dist) X(eB)      \
-  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega) X(dx_hist) X(dx_same) X(dx_same_iters) X(coarse_w) X(coarse_omega) X(cw_x) X(cw_y) X(cw_w) X(cw_wbf)
+  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega) X(dx_hist) X(dx_same) X(dx_same_iters) X(coarse_w) X(coarse_omega) X(cw_x) X(cw_y) X(cw_w) X(cw_wbf) X(self_contact) X(n_sc_points) X(n_sc_edges) X(sc_max_pairs) X(sc_max_span) X(sc_dhat) X(sc_kappa) X(sc_cell) X(sc_ccd_s) X(sc_points) X(sc_edges) X(sc_pt) X(sc_ee) X(sc_count) X(sc_g) X(sc_h) X(sc_alpha) X(sc_ws) X(sc_ws_bytes)
 
 extern "C" {
 
@@ -3159,7 +3203,12 @@ int mfem_begin_step(const mfem_solver* S, float dt, void* stream) {
   if (int rc = check_solver(S)) return rc;
   if (!(dt > 0.f)) return fail(MFEM_EINVAL, "dt must be > 0");
   hipLaunchKernelGGL(k_begin_step, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, (hipStream_t)stream, *S, dt);
-  return launched();
+  if (int rc = launched()) return rc;
+  // self-contact: candidates on the static boxes of the step's start
+  if (S->self_contact)
+    if (int rc = mfem_self_contact_candidates(S, 0, stream))
+      return fail(rc, std::string("self-contact: ") + mfem_self_contact_last_error());
+  return MFEM_OK;
 }
 
 int mfem_update_system(const mfem_solver* S, float dt, void* stream) {
@@ -3172,6 +3221,9 @@ int mfem_update_system(const mfem_solver* S, float dt, void* stream) {
   }
   hipLaunchKernelGGL(k_vertex, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, st, *S, dt);
   if (int rc = launched()) return rc;
+  if (S->self_contact)
+    if (int rc = mfem_self_contact_eval(S, stream))
+      return fail(rc, std::string("self-contact: ") + mfem_self_contact_last_error());
   hipLaunchKernelGGL(k_element, dim3(blocks_for(S->n_tets)), dim3(kBlock), 0, st, *S);
   return launched();
 }
@@ -3242,6 +3294,15 @@ int mfem_local_solve(const mfem_solver* S, void* stream) {
 int mfem_line_search(const mfem_solver* S, float dt, int newton_iter, void* stream) {
   if (int rc = check_solver(S)) return rc;
   hipStream_t st = (hipStream_t)stream;
+  if (S->self_contact) {
+    // candidates on the boxes swept by the Newton step (every pair within
+    // dhat anywhere on x + a dx, a in [0, 1]: the merit's and the next
+    // iteration's), then the CCD bound of the first step
+    if (int rc = mfem_self_contact_candidates(S, 1, stream))
+      return fail(rc, std::string("self-contact: ") + mfem_self_contact_last_error());
+    if (int rc = mfem_self_contact_ccd(S, stream))
+      return fail(rc, std::string("self-contact: ") + mfem_self_contact_last_error());
+  }
   if (!S->line_search) {
     long n = S->n_verts > S->n_tets ? S->n_verts : S->n_tets;
     hipLaunchKernelGGL(k_plain_update, dim3(blocks_for(n)), dim3(kBlock), 0, st, *S, newton_iter);

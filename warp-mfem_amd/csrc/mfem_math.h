@@ -523,10 +523,10 @@ MFEM_DEV void closest_point_triangle_bary(const T* x, const T* a, const T* b, co
 }
 
 // closest points of segments p1q1 and p2q2 (Ericson 5.1.9); returns the
-// parameter on the first segment
+// parameter on the first segment (t_out, if given: on the second)
 template <typename T>
-MFEM_DEV T segment_segment(const T* p1, const T* q1, const T* p2, const T* q2, T* c1,
-                               T* c2) {
+MFEM_DEV T segment_segment(const T* p1, const T* q1, const T* p2, const T* q2, T* c1, T* c2,
+                           T* t_out = nullptr) {
   const T d1[3] = {q1[0] - p1[0], q1[1] - p1[1], q1[2] - p1[2]};
   const T d2[3] = {q2[0] - p2[0], q2[1] - p2[1], q2[2] - p2[2]};
   const T r[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
@@ -556,6 +556,7 @@ MFEM_DEV T segment_segment(const T* p1, const T* q1, const T* p2, const T* q2, T
     c1[i] = p1[i] + d1[i] * s;
     c2[i] = p2[i] + d2[i] * t;
   }
+  if (t_out) *t_out = t;
   return s;
 }
 

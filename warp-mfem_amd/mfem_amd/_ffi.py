@@ -33,7 +33,11 @@ FIELDS = [
     "body_wg", "body_vert", "body_tet", "body_tri", "body_ls_wg", "body_epoch", "tet_energy",
 )] + [("body_wg_align", c_i32), ("local_sweeps", c_i32), ("local_omega", c_f32), ("dx_hist", c_ptr), ("dx_same", c_ptr), ("dx_same_iters", c_i32),
    ("coarse_w", c_ptr), ("coarse_omega", c_f32), ("cw_x", c_ptr), ("cw_y", c_ptr), ("cw_w", c_ptr),
-   ("cw_wbf", c_ptr)]
+   ("cw_wbf", c_ptr)] + [(n, c_i32) for n in (
+    "self_contact", "n_sc_points", "n_sc_edges", "sc_max_pairs", "sc_max_span",
+)] + [(n, c_f32) for n in ("sc_dhat", "sc_kappa", "sc_cell", "sc_ccd_s")] + [(n, c_ptr) for n in (
+    "sc_points", "sc_edges", "sc_pt", "sc_ee", "sc_count", "sc_g", "sc_h", "sc_alpha", "sc_ws",
+)] + [("sc_ws_bytes", ctypes.c_int64)]
 
 
 class MfemSolverStruct(ctypes.Structure):
@@ -48,6 +52,7 @@ SHAPE_PLANE, SHAPE_CAPSULE = 0, 1
 PCG_PIPELINED, PCG_STREAM_ROWS, PCG_LEARNED_GUESS, PCG_COARSE = 1, 2, 4, 8
 MFEM_MLP_NPARAMS = 6288
 MFEM_SCAN_NOALLOC_MAX = 1 << 26
+MFEM_ERR_BARRIER, MFEM_ERR_CONTACT = 1, 2
 
 ENTRY_POINTS = [
     "mfem_solver_sizeof", "mfem_solver_offsetof", "mfem_version", "mfem_wall_clock_khz", "mfem_last_error",
@@ -61,7 +66,9 @@ ENTRY_POINTS = [
     "mfem_refine_last_error", "mfem_pcg_variant_count", "mfem_pcg_variant_info", "mfem_line_search_max_blocks",
     "mfem_contact_distance", "mfem_pattern_workspace_bytes", "mfem_pattern_count", "mfem_pattern_fill",
     "mfem_pattern_last_error", "mfem_pose_shapes", "mfem_scan_i32_noalloc", "mfem_coarse_features",
-    "mfem_coarse_weights",
+    "mfem_coarse_weights", "mfem_broadphase_swept", "mfem_self_contact_workspace_bytes",
+    "mfem_self_contact_candidates", "mfem_self_contact_eval", "mfem_self_contact_ccd",
+    "mfem_self_contact_last_error",
 ]
 
 
@@ -137,6 +144,13 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mfem_line_search_max_blocks.argtypes = []
     lib.mfem_pcg_variant_info.argtypes = [P, i, vp]
     lib.mfem_pose_shapes.argtypes = [vp, i, vp, vp, i, vp, vp]
+    lib.mfem_broadphase_swept.argtypes = [i, vp, vp, vp, i, vp, i, f, f, i, vp, i, vp, vp, vp, sz, vp]
+    lib.mfem_self_contact_workspace_bytes.argtypes = [i, i, i, i]
+    lib.mfem_self_contact_workspace_bytes.restype = sz
+    for name in ("mfem_self_contact_eval", "mfem_self_contact_ccd"):
+        getattr(lib, name).argtypes = [P, vp]
+    lib.mfem_self_contact_candidates.argtypes = [P, i, vp]
+    lib.mfem_self_contact_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 

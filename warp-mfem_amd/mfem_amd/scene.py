@@ -234,6 +234,34 @@ def drop_box_scene(cells=(4, 3, 3), size=(0.2, 0.15, 0.15), clearance=5.0e-4, E=
     return m
 
 
+def stacked_boxes_scene(cells=(4, 4, 2), size=(0.12, 0.12, 0.04), gap=4.0e-4, twist_deg=30.0, E=5.0e4,
+                        energy="neohookean", dhat=1.0e-3, friction_mu=0.3, velocity=(0.0, 0.0, -0.15)) -> Model:
+    """Self-contact: two slabs in one mesh, the lower one resting on the
+    plane z = 0, the upper one twisted about z by ``twist_deg`` (so that
+    their edges cross) and falling onto it from ``gap`` above (inside dhat)
+    at ``velocity``.  Self-contact on (point-triangle and edge-edge barrier
+    between the slabs, same dhat and stiffness as the ground's)."""
+    p0, t0 = _mesh.box_tet_mesh((-0.5 * size[0], -0.5 * size[1], 5.0e-4),
+                                (0.5 * size[0], 0.5 * size[1], 5.0e-4 + size[2]), cells)
+    th = np.deg2rad(twist_deg)
+    R = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0], [0.0, 0.0, 1.0]])
+    p1 = p0 @ R.T + np.array([0.01, 0.005, size[2] + gap])
+    p = np.concatenate([p0, p1])
+    t = np.concatenate([t0, t0 + len(p0)])
+    mu, lam = _mesh.lame(E, 0.4)
+    vel = np.zeros_like(p)
+    vel[len(p0):] = velocity
+    m = build_model(p, t, _materials(mu, lam, len(t)), 1000.0, shapes=[Shape(SHAPE_PLANE, mu=friction_mu)],
+                    name="stacked_boxes")
+    m.particle_qd = vel
+    kappa = auto_kappa(m, dhat)
+    m.params.update(energy=energy, iterations=8, cg_max_iterations=2000, cg_tolerance=1e-6,
+                    line_search=True, line_search_max_iterations=30, contact_d0=0.1 * dhat, contact_d1=dhat,
+                    contact_stiffness=kappa, friction_mu=[friction_mu], friction_eps_v=1e-3,
+                    self_contact=True, self_contact_dhat=dhat, self_contact_kappa=kappa)
+    return m
+
+
 def soft_body_batch(n_bodies: int = 64, seed: int = 360, tets_min: float = 2.0e3, tets_max: float = 5.0e4,
                     dhat: float = 1.0e-3, friction_mu: float = 0.3, iterations: int = 12,
                     spacing: float = 1.0) -> list:

@@ -76,6 +76,16 @@ def _barrier_at(d, d0, d1):
     return (e, de, d2e)
 
 
+def _surface_edges(tris: np.ndarray) -> np.ndarray:
+    """Unique edges (i < j) of triangles (F, 3), sorted."""
+    tris = np.asarray(tris, np.int64).reshape(-1, 3)
+    if len(tris) == 0:
+        return np.zeros((0, 2), np.int32)
+    e = np.concatenate([tris[:, [0, 1]], tris[:, [1, 2]], tris[:, [2, 0]]])
+    e.sort(axis=1)
+    return np.unique(e, axis=0).astype(np.int32)
+
+
 class RefinementSolver:
     def __init__(self, model, iterations: int, max_tets: int, max_particles: int | None = None,
                  max_tris: int | None = None, refine_every_n_steps: int = 10, min_refine_score: float = 10.1,
@@ -283,6 +293,7 @@ class RefinementSolver:
             if hasattr(S, name):
                 setattr(S, name, t.data_ptr())
         self._S = S
+        self._init_self_contact(kwargs, tris, Vc, Fc, d1)
         # device error bits as of the last step that has completed, copied
         # into pinned host memory at the end of every step (no host sync)
         self._status_mirror = torch.zeros(1, dtype=torch.int32, pin_memory=self.device.type == "cuda")
@@ -326,6 +337,83 @@ class RefinementSolver:
         for name in ("chunk_ptr", "sell_col", "sell_val", "diag_lane", "contrib_ptr", "contrib_src", "vinc_ptr",
                      "vinc_src", "pz", "px", "tcon_ptr", "tcon_src", "vtri_ptr", "vtri_src"):
             setattr(S, name, b[name].data_ptr())
+        if S.self_contact:
+            self._set_self_contact_surface(int(n_tris))
+
+    # -- self-contact (no counterpart in the reference's live solve) ----------
+    def _init_self_contact(self, kwargs: dict, tris: np.ndarray, Vc: int, Fc: int, d1: float) -> None:
+        """IPC point-triangle / edge-edge self-contact among the surface
+        triangles inside the Newton solve (``self_contact=True``): barrier
+        -(d - dhat)^2 ln(d / dhat) (src/mfem/collision.py:529-546) with
+        ``self_contact_dhat`` (default contact_d1) and
+        ``self_contact_kappa`` (default contact_stiffness), candidates from
+        the spatial-hash broad phase on boxes swept by the Newton step, the
+        line search's first step cut by a conservative CCD bound.  The hook
+        in the reference's solve is commented out (src/mfem/solver.py:443-447)."""
+        S, b, dev = self._S, self.buffers, self.device
+        S.self_contact = 0
+        if not kwargs.get("self_contact", False):
+            return
+        if len(tris) == 0:
+            raise ValueError("self_contact needs the model's surface triangles (tri_indices)")
+        dhat = float(kwargs.get("self_contact_dhat", d1))
+        S.sc_dhat = dhat
+        S.sc_kappa = float(kwargs.get("self_contact_kappa", S.kappa))
+        S.sc_ccd_s = float(kwargs.get("self_contact_ccd_s", 0.1))
+        S.sc_max_span = int(kwargs.get("self_contact_max_span", 4))
+        x = np.asarray(self.model.particle_q, np.float64)
+        e = _surface_edges(tris)
+        max_edge = float(np.linalg.norm(x[e[:, 0]] - x[e[:, 1]], axis=1).max()) if len(e) else dhat
+        # boxes of a primitive and of its motion over a step span up to
+        # max_span cells: a cell of two edge lengths leaves room for motions
+        # of several edge lengths per Newton step
+        S.sc_cell = float(kwargs.get("self_contact_cell", max(2.0 * max_edge, 4.0 * dhat)))
+        n_pts_cap, n_edge_cap = Vc, max(2 * Fc, 1)
+        S.sc_max_pairs = int(kwargs.get("self_contact_max_pairs", max(1 << 16, 16 * Vc)))
+        i32z = lambda *shape: torch.zeros(*shape, dtype=torch.int32, device=dev)
+        b["sc_points"] = i32z(max(n_pts_cap, 1))
+        b["sc_edges"] = i32z(n_edge_cap, 2)
+        b["sc_pt"] = i32z(S.sc_max_pairs, 2)
+        b["sc_ee"] = i32z(S.sc_max_pairs, 2)
+        b["sc_count"] = i32z(4)
+        b["sc_g"] = torch.zeros(Vc, 3, dtype=torch.float64, device=dev)
+        b["sc_h"] = torch.zeros(Vc, 6, dtype=torch.float64, device=dev)
+        b["sc_alpha"] = torch.ones(1, dtype=torch.float32, device=dev)
+        nbytes = int(self.lib.mfem_self_contact_workspace_bytes(n_pts_cap, max(Fc, 1), n_edge_cap, S.sc_max_span))
+        b["sc_ws"] = torch.zeros(nbytes, dtype=torch.uint8, device=dev)
+        S.sc_ws_bytes = nbytes
+        self._sc_caps = (n_pts_cap, n_edge_cap)
+        for name in ("sc_points", "sc_edges", "sc_pt", "sc_ee", "sc_count", "sc_g", "sc_h", "sc_alpha", "sc_ws"):
+            setattr(S, name, b[name].data_ptr())
+        S.self_contact = 1
+
+    def _set_self_contact_surface(self, n_tris: int) -> None:
+        """Surface vertices and edges of the active triangles (host, once per
+        topology)."""
+        S, b = self._S, self.buffers
+        tris = b["tris"][:n_tris].cpu().numpy()
+        pts = np.unique(tris.reshape(-1)).astype(np.int32)
+        e = _surface_edges(tris)
+        if len(pts) > self._sc_caps[0] or len(e) > self._sc_caps[1]:
+            raise RuntimeError("surface grew past the self-contact capacity")
+        if len(pts):
+            b["sc_points"][: len(pts)] = torch.as_tensor(pts, device=self.device)
+        if len(e):
+            b["sc_edges"][: len(e)] = torch.as_tensor(e, device=self.device)
+        S.n_sc_points, S.n_sc_edges = int(len(pts)), int(len(e))
+
+    def self_contact_candidates(self) -> dict:
+        """The current candidate lists (host copies): point-triangle pairs
+        (vertex, triangle), edge-edge pairs (edge, edge), the surface edges
+        and the broad-phase status bits."""
+        torch.cuda.synchronize(self.device)
+        b, S = self.buffers, self._S
+        cnt = b["sc_count"].cpu().numpy()
+        npt, nee = min(int(cnt[0]), S.sc_max_pairs), min(int(cnt[1]), S.sc_max_pairs)
+        return {"pt": b["sc_pt"][:npt].cpu().numpy(), "ee": b["sc_ee"][:nee].cpu().numpy(),
+                "edges": b["sc_edges"][: S.n_sc_edges].cpu().numpy(),
+                "points": b["sc_points"][: S.n_sc_points].cpu().numpy(), "status": cnt[2:].copy(),
+                "alpha": float(b["sc_alpha"].item())}
 
     def _set_bodies(self, body_vert, body_tet, body_tri, tet_energy=None) -> None:
         """Declares the mesh a batch of independent bodies (vertex ranges
