@@ -89,8 +89,10 @@ _PARAM_NAMES = {
     "preconditioner_singular_threshold": "sing_thr",
     "self_contact_dhat": "sc_dhat", "self_contact_kappa": "sc_kappa", "self_contact_ccd_s": "sc_ccd_s",
 }
-# solver keywords with no oracle counterpart (GPU data-structure sizes)
-_GPU_ONLY = {"self_contact_max_pairs", "self_contact_cell", "self_contact_max_span"}
+# solver keywords with no oracle counterpart (GPU data-structure sizes,
+# refinement: the oracle steps a fixed mesh)
+_GPU_ONLY = {"self_contact_max_pairs", "self_contact_cell", "self_contact_max_span", "refine_tet_score_weight",
+             "refine_vertex_score_weight", "refine_density"}
 
 
 def _real(bits: int):
@@ -250,6 +252,7 @@ class Oracle:
         nshapes = len(scene.shapes)
         pins = np.nonzero(scene.particle_inv_mass == 0)[0].astype(np.int32)
         self.ctx = self.lib.orc_create(nv, nt, ntris, nshapes, len(pins))
+        self._pins = pins
         self.nv, self.nt = nv, nt
         self.buf("tets")[:] = scene.tet_indices.reshape(-1)
         self.buf("dminv")[:] = scene.tet_poses.reshape(-1)
@@ -378,6 +381,14 @@ class Oracle:
 
     def step(self, dt):
         self.lib.orc_step(self.ctx, dt)
+
+    def set_attachment_targets(self, indices, positions) -> None:
+        """New pin positions of the pinned vertices ``indices`` (penalty
+        attachments; src/mfem/boundary_condition.py:85-92)."""
+        slot = {int(v): k for k, v in enumerate(self._pins)}
+        pp = self.buf("pin_pos").reshape(-1, 3)
+        for v, p in zip(np.asarray(indices).tolist(), np.asarray(positions, np.float64)):
+            pp[slot[int(v)]] = p
 
     # self-contact (surface point-triangle / edge-edge barrier), all pairs
     def _ptr(self, a):

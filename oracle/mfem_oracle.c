@@ -1091,40 +1091,48 @@ EXPORT double orc_sc_accd(int kind, const double* X0, const double* P0, double s
   return t;
 }
 
-/* boxes of the pair's two primitives (vertices 0..na-1 and na..3) at x and,
-   with dxs, at x + dxs, inflated by r each: do they overlap? */
-static int sc_boxes_overlap(const double* X, const double* P, int na, double r) {
-  double lo[2][3], hi[2][3];
-  for (int g = 0; g < 2; ++g)
-    for (int c = 0; c < 3; ++c) { lo[g][c] = INFINITY; hi[g][c] = -INFINITY; }
-  for (int j = 0; j < 4; ++j) {
-    int g = j < na ? 0 : 1;
+/* box of primitive ids[0..nv-1] at x (and x + dxs when given), inflated by r */
+static void sc_box(const real* x, const real* dxs, const int* ids, int nv, double r, double* lo, double* hi) {
+  for (int c = 0; c < 3; ++c) { lo[c] = INFINITY; hi[c] = -INFINITY; }
+  for (int j = 0; j < nv; ++j)
     for (int c = 0; c < 3; ++c) {
-      double a = X[3 * j + c], b = P ? a + P[3 * j + c] : a;
-      lo[g][c] = fmin(lo[g][c], fmin(a, b));
-      hi[g][c] = fmax(hi[g][c], fmax(a, b));
+      double a = (double)x[3 * ids[j] + c], b = dxs ? a + (double)dxs[3 * ids[j] + c] : a;
+      lo[c] = fmin(lo[c], fmin(a, b));
+      hi[c] = fmax(hi[c], fmax(a, b));
     }
-  }
-  for (int c = 0; c < 3; ++c)
-    if (lo[0][c] - r > hi[1][c] + r || lo[1][c] - r > hi[0][c] + r) return 0;
-  return 1;
+  for (int c = 0; c < 3; ++c) { lo[c] -= r; hi[c] += r; }
 }
 
 /* mode 0: gradient / diagonal Hessian blocks at x into sc_g / sc_h (returns
    the energy); mode 1: energy at x; mode 2: the smallest time of impact
-   along dxs (returns it, <= 1) */
+   along dxs (returns it, <= 1).  Pairs whose boxes (over x, and x + dxs in
+   mode 2) inflated by dhat / 2 do not overlap are skipped: their distance is
+   at least dhat all along. */
 static double sc_pass(orc_ctx* C, const real* x, const real* dxs, int mode) {
-  const double dh = C->p[P_SC_DHAT], kap = C->p[P_SC_KAPPA], cs = C->p[P_SC_S];
+  const double dh = C->p[P_SC_DHAT], kap = C->p[P_SC_KAPPA], cs = C->p[P_SC_S], r = 0.5 * dh;
+  const real* sw = mode == 2 ? dxs : NULL;
   double E = 0, toi = 1.0;
   if (mode == 0) {
     memset(C->sc_g, 0, sizeof(double) * 3 * C->nv);
     memset(C->sc_h, 0, sizeof(double) * 6 * C->nv);
   }
+  /* boxes of the points, triangles and edges */
+  double* bp = malloc(sizeof(double) * 6 * (C->npts > 0 ? C->npts : 1));
+  double* bt = malloc(sizeof(double) * 6 * (C->ntris > 0 ? C->ntris : 1));
+  double* be = malloc(sizeof(double) * 6 * (C->nedges > 0 ? C->nedges : 1));
+  for (int a = 0; a < C->npts; ++a) sc_box(x, sw, C->pts + a, 1, r, bp + 6 * a, bp + 6 * a + 3);
+  for (int t = 0; t < C->ntris; ++t) sc_box(x, sw, C->tris + 3 * t, 3, r, bt + 6 * t, bt + 6 * t + 3);
+  for (int e = 0; e < C->nedges; ++e) sc_box(x, sw, C->edges + 2 * e, 2, r, be + 6 * e, be + 6 * e + 3);
   for (int kind = 0; kind < 2; ++kind) {
-    int na = kind == 0 ? 1 : 2;
     int nA = kind == 0 ? C->npts : C->nedges, nB = kind == 0 ? C->ntris : C->nedges;
+    const double* BA = kind == 0 ? bp : be;
+    const double* BB = kind == 0 ? bt : be;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : E) reduction(min : toi)
     for (int a = 0; a < nA; ++a)
       for (int bidx = kind == 0 ? 0 : a + 1; bidx < nB; ++bidx) {
+        const double* A = BA + 6 * a;
+        const double* B = BB + 6 * bidx;
+        if (A[0] > B[3] || B[0] > A[3] || A[1] > B[4] || B[1] > A[4] || A[2] > B[5] || B[2] > A[5]) continue;
         int v[4];
         if (kind == 0) {
           const int* t = C->tris + 3 * bidx;
@@ -1140,7 +1148,6 @@ static double sc_pass(orc_ctx* C, const real* x, const real* dxs, int mode) {
             X[3 * j + c] = (double)x[3 * v[j] + c];
             P[3 * j + c] = dxs ? (double)dxs[3 * v[j] + c] : 0.0;
           }
-        if (!sc_boxes_overlap(X, mode == 2 ? P : NULL, na, 0.5 * dh)) continue;
         if (mode == 2) {
           double t = orc_sc_accd(kind, X, P, cs);
           if (t < toi) toi = t;
@@ -1156,12 +1163,19 @@ static double sc_pass(orc_ctx* C, const real* x, const real* dxs, int mode) {
           double h = kap * fmax(d2e, 0.0);
           double nn[6] = {n[0] * n[0], n[1] * n[1], n[2] * n[2], n[0] * n[1], n[0] * n[2], n[1] * n[2]};
           for (int j = 0; j < 4; ++j) {
-            for (int c = 0; c < 3; ++c) C->sc_g[3 * v[j] + c] += kap * de * cw[j] * n[c];
-            for (int q = 0; q < 6; ++q) C->sc_h[6 * v[j] + q] += h * cw[j] * cw[j] * nn[q];
+            for (int c = 0; c < 3; ++c) {
+#pragma omp atomic
+              C->sc_g[3 * v[j] + c] += kap * de * cw[j] * n[c];
+            }
+            for (int q = 0; q < 6; ++q) {
+#pragma omp atomic
+              C->sc_h[6 * v[j] + q] += h * cw[j] * cw[j] * nn[q];
+            }
           }
         }
       }
   }
+  free(bp); free(bt); free(be);
   return mode == 2 ? toi : E;
 }
 

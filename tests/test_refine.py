@@ -215,7 +215,7 @@ def _refining_beam(gpu, every=3):
     # threshold, refine every few steps (tests/test_refinement.py:68-76, 98-100)
     solver = RefinementSolver(m, m.params["iterations"], max_tets=4 * m.tet_count, max_particles=4 * m.particle_count,
                               device=gpu, enable_refinement=True, refine_every_n_steps=every, min_refine_score=1e-3,
-                              refine_tet_score_weight=1.0, refine_density=1000.0, **kw)
+                              refine_tet_score_weight=1.0, **kw)  # (refine_density 1000 from the model)
     return m, solver, model_state(m, gpu, capacity=4 * m.particle_count)
 
 
@@ -309,7 +309,7 @@ def test_solver_geometric_refinement_near_the_tool(gpu):
     kw = {k: v for k, v in m.params.items() if k != "iterations"}
     solver = RefinementSolver(m, m.params["iterations"], max_tets=4 * m.tet_count, max_particles=4 * m.particle_count,
                               device=gpu, enable_refinement=True, refine_every_n_steps=2, refine_scoring="geometric",
-                              refine_elastic_weight=0.0, refine_geometric_threshold=1.0, refine_density=1000.0, **kw)
+                              refine_elastic_weight=0.0, refine_geometric_threshold=1.0, **kw)
     state = model_state(m, gpu, capacity=4 * m.particle_count)
     out = state.clone()
     vol0 = _rest_volume(solver)

@@ -3079,6 +3079,9 @@ const PcgVariant kPcgVariants[] = {
     // block Jacobi plus the workgroup-local rigid-mode correction
     // (MFEM_PCG_COARSE): pipelined, resident rows
     PCG_V(1, 16, 3), PCG_V(1, 15, 3),
+    // wider resident rows for refined meshes (a split edge raises the
+    // valence of the vertices around it past 15)
+    PCG_V(1, 20, 1), PCG_V(1, 24, 1), PCG_V(1, 20, 3),
 };
 #undef PCG_V
 constexpr int kNumPcgVariants = sizeof(kPcgVariants) / sizeof(kPcgVariants[0]);
@@ -3252,15 +3255,17 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
   }
   for (int k0 = 0; k0 < kNumPcgVariants && sel < 0; ++k0) {
     // resident rows: the narrowest form of this kind that holds the widest
-    // chunk stands in for the wider ones
+    // chunk stands in for the others of its kind
     int k = k0;
-    if (kPcgVariants[k0].rw >= S->max_width) {
+    if (kPcgVariants[k0].rw > 0) {
+      int best = -1;
       for (int j = 0; j < kNumPcgVariants; ++j) {
         const PcgVariant& u = kPcgVariants[j];
-        if (u.mode == kPcgVariants[k0].mode && u.cpw == kPcgVariants[k0].cpw && u.rw >= S->max_width &&
-            u.rw < kPcgVariants[k].rw)
-          k = j;
+        if (u.rw > 0 && u.mode == kPcgVariants[k0].mode && u.cpw == kPcgVariants[k0].cpw && u.rw >= S->max_width &&
+            (best < 0 || u.rw < kPcgVariants[best].rw))
+          best = j;
       }
+      if (best >= 0) k = best;
     }
     const PcgVariant& v = kPcgVariants[k];
     const bool want = (S->pcg_flags & MFEM_PCG_PIPELINED) != 0;

@@ -133,7 +133,9 @@ def build_model(particles, tets, materials, density, *, pinned=None, shapes=(), 
         tri_indices=_mesh.surface_tris(tets),
         shapes=list(shapes),
         gravity=np.asarray(gravity, dtype=np.float64),
-        params=dict(params or {}),
+        # a refinement pass re-lumps the masses with refine_density
+        # (refinement.py:1253): the density the model was built with
+        params={"refine_density": float(density), **dict(params or {})},
         name=name,
     )
 
@@ -219,6 +221,78 @@ def octopus_scene(h: float = 7.3e-3, seed: int = 7, clearance: float = 2.0e-3, d
                     contact_stiffness=auto_kappa(m, dhat), friction_mu=[friction_mu], friction_eps_v=1e-3,
                     attachment_stiffness=10.0)
     return m
+
+
+class TipDrive:
+    """Arm tips driven as moving penalty attachments: each arm's tip vertices
+    (pinned: inv_mass = 0) follow base + a_k (sin(2 pi f_k t + phi_k) -
+    sin(phi_k)) (at rest at t = 0), with
+    amplitude vector a_k (``amplitude`` along a seeded random horizontal
+    direction),
+    frequency f_k ~ U[0.5, 2] Hz and phase phi_k ~ U[0, 2 pi) per arm.  The
+    targets move the attachments' pin positions, as
+    DirichletBoundaryCondition.update_b does for new pin positions
+    (src/mfem/boundary_condition.py:85-92)."""
+
+    def __init__(self, indices, arm, base, amplitude, freq, phase):
+        self.indices = np.asarray(indices, np.int64)
+        self.arm = np.asarray(arm, np.int64)
+        self.base = np.asarray(base, np.float64)
+        self.amplitude = np.asarray(amplitude, np.float64)  # (arms, 3)
+        self.freq = np.asarray(freq, np.float64)
+        self.phase = np.asarray(phase, np.float64)
+
+    def targets(self, t: float) -> np.ndarray:
+        """Pin positions at time t (the rest positions at t = 0)."""
+        w = np.sin(2.0 * np.pi * self.freq * t + self.phase) - np.sin(self.phase)  # (arms,)
+        return self.base + self.amplitude[self.arm] * w[self.arm, None]
+
+
+def octopus_drive_scene(h: float = 7.3e-3, seed: int = 11, tip_radius: float = 0.025, amplitude: float = 0.1,
+                        refine_every: int = 10, self_contact: bool = True, arms: int = 8):
+    """The octopus of ``octopus_scene`` with self-contact between its surface
+    primitives (point-triangle and edge-edge barrier, dhat = 1e-3, the
+    ground's stiffness), its arm tips (vertices within ``tip_radius`` of each
+    arm's end) driven along seeded sinusoids (``TipDrive``: amplitude 0.1 m,
+    0.5 to 2 Hz, random phase and horizontal direction), and adaptive refinement every
+    ``refine_every`` steps on a strain-energy threshold: a tet's edges are
+    split when its strain energy exceeds tau * (mean rest tet volume), tau
+    drawn log-uniformly in [20, 200] J/m^3 from the seed.  Returns (model,
+    drive)."""
+    m = octopus_scene(h=h)
+    rng = np.random.default_rng(seed)
+    r0, r1, L = 0.03, 0.01, 0.6
+    x = m.particle_q
+    idx, arm = [], []
+    for k in range(arms):
+        th = 2.0 * np.pi * (k + 0.5) / arms
+        d = np.array([np.cos(th), np.sin(th), 0.0])
+        a = np.array([0.08 * np.cos(th), 0.06 * np.sin(th), 0.0])
+        tip = a + L * d
+        tip[2] = x[:, 2].min() + r1
+        near = np.nonzero(np.linalg.norm(x - tip, axis=1) < tip_radius)[0]
+        idx.append(near)
+        arm.append(np.full(len(near), k))
+    idx = np.concatenate(idx)
+    arm = np.concatenate(arm)
+    # horizontal directions: the tips slide over the ground (a downward
+    # component would drive the attachments into the plane)
+    ang = rng.uniform(0.0, 2.0 * np.pi, arms)
+    dirs = np.stack([np.cos(ang), np.sin(ang), np.zeros(arms)], axis=1)
+    drive = TipDrive(idx, arm, x[idx].copy(), amplitude * dirs, rng.uniform(0.5, 2.0, arms),
+                     rng.uniform(0.0, 2.0 * np.pi, arms))
+    m.particle_inv_mass = m.particle_inv_mass.copy()
+    m.particle_inv_mass[idx] = 0.0
+    vol = 1.0 / (6.0 * np.linalg.det(m.tet_poses.reshape(-1, 3, 3)))
+    tau = float(np.exp(rng.uniform(np.log(20.0), np.log(200.0))))
+    m.params.update(self_contact=self_contact, self_contact_dhat=m.params["contact_d1"],
+                    self_contact_kappa=m.params["contact_stiffness"], refine_tet_score_weight=1.0,
+                    refine_vertex_score_weight=0.1)
+    m.refine_every = refine_every
+    m.min_refine_score = tau * float(vol.mean())
+    m.strain_energy_threshold = tau
+    m.name = "octopus_drive_h%.1fmm" % (h * 1e3)
+    return m, drive
 
 
 def drop_box_scene(cells=(4, 3, 3), size=(0.2, 0.15, 0.15), clearance=5.0e-4, E=2.0e4, energy="neohookean",

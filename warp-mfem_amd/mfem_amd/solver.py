@@ -402,6 +402,16 @@ class RefinementSolver:
             b["sc_edges"][: len(e)] = torch.as_tensor(e, device=self.device)
         S.n_sc_points, S.n_sc_edges = int(len(pts)), int(len(e))
 
+    def set_attachment_targets(self, indices: torch.Tensor, positions: torch.Tensor) -> None:
+        """Moves the penalty attachments of the pinned vertices ``indices``
+        (inv_mass = 0) to ``positions`` (device tensors; no host
+        synchronisation, capture-safe): DirichletBoundaryCondition with new
+        pin_positions followed by update_b
+        (src/mfem/boundary_condition.py:85-92).  Rows of vertices that are not
+        pinned are stored but unused (their attachment term is off)."""
+        idx = indices.to(device=self.device, dtype=torch.long)
+        self.buffers["pin_pos"].index_copy_(0, idx, positions.to(device=self.device, dtype=torch.float32))
+
     def self_contact_candidates(self) -> dict:
         """The current candidate lists (host copies): point-triangle pairs
         (vertex, triangle), edge-edge pairs (edge, edge), the surface edges
