@@ -247,18 +247,21 @@ class Oracle:
         self.lib = _lib(bits, omp=threads is not None)
         self.threads = self.lib.orc_set_threads(threads) if threads is not None else 1
         self.dt = _real(bits)
-        nv, nt = scene.particle_q.shape[0], scene.tet_indices.shape[0]
+        # the active particles (a model padded for refinement headroom carries
+        # unreferenced rows after them)
+        nt = scene.tet_indices.shape[0]
+        nv = int(np.asarray(scene.tet_indices).max()) + 1 if nt else scene.particle_q.shape[0]
         ntris = 0 if scene.tri_indices is None else scene.tri_indices.shape[0]
         nshapes = len(scene.shapes)
-        pins = np.nonzero(scene.particle_inv_mass == 0)[0].astype(np.int32)
+        pins = np.nonzero(np.asarray(scene.particle_inv_mass)[:nv] == 0)[0].astype(np.int32)
         self.ctx = self.lib.orc_create(nv, nt, ntris, nshapes, len(pins))
         self._pins = pins
         self.nv, self.nt = nv, nt
         self.buf("tets")[:] = scene.tet_indices.reshape(-1)
         self.buf("dminv")[:] = scene.tet_poses.reshape(-1)
         self.buf("materials")[:] = scene.tet_materials.reshape(-1)
-        self.buf("mass")[:] = scene.particle_mass
-        self.buf("inv_mass")[:] = scene.particle_inv_mass
+        self.buf("mass")[:] = np.asarray(scene.particle_mass)[:nv]
+        self.buf("inv_mass")[:] = np.asarray(scene.particle_inv_mass)[:nv]
         if len(pins):
             self.buf("pin_idx")[:] = pins
             self.buf("pin_pos")[:] = scene.particle_q[pins].reshape(-1)
@@ -269,8 +272,8 @@ class Oracle:
             self.buf("shape_scale")[3 * k:3 * k + 3] = sh.scale
         if nshapes:
             self.set_body_q(getattr(scene, "body_q", None), scene)
-        self.buf("x")[:] = scene.particle_q.reshape(-1)
-        self.buf("v")[:] = scene.particle_qd.reshape(-1)
+        self.buf("x")[:] = np.asarray(scene.particle_q)[:nv].reshape(-1)
+        self.buf("v")[:] = np.asarray(scene.particle_qd)[:nv].reshape(-1)
         p = dict(scene.params)
         if params:
             p.update(params)

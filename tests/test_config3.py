@@ -35,8 +35,10 @@ def _solver(m, gpu, refine, **extra):
     kw = {k: v for k, v in m.params.items() if k != "iterations"}
     kw.update(extra)
     T, V = m.tet_count, m.particle_count
+    if refine:
+        m = scene.pad_particles(m, int(1.5 * V))
     solver = RefinementSolver(m, m.params["iterations"], max_tets=int(1.5 * T) if refine else T,
-                              max_particles=int(1.5 * V) if refine else V,
+                              max_particles=m.particle_count,
                               refine_every_n_steps=m.refine_every, min_refine_score=m.min_refine_score,
                               enable_refinement=refine, device=gpu, pcg_pipelined=True, **kw)
     return solver, model_state(m, gpu, capacity=solver.max_particles)

@@ -213,9 +213,12 @@ def _refining_beam(gpu, every=3):
     kw = {k: v for k, v in m.params.items() if k != "iterations"}
     # the reference's own refinement test: elastic-energy scoring, a low
     # threshold, refine every few steps (tests/test_refinement.py:68-76, 98-100)
-    solver = RefinementSolver(m, m.params["iterations"], max_tets=4 * m.tet_count, max_particles=4 * m.particle_count,
+    # (the particle buffer padded for the new vertices, as the reference's
+    # caller does; refine_density 1000 from the model)
+    solver = RefinementSolver(scene.pad_particles(m, 4 * m.particle_count), m.params["iterations"],
+                              max_tets=4 * m.tet_count, max_particles=4 * m.particle_count,
                               device=gpu, enable_refinement=True, refine_every_n_steps=every, min_refine_score=1e-3,
-                              refine_tet_score_weight=1.0, **kw)  # (refine_density 1000 from the model)
+                              refine_tet_score_weight=1.0, **kw)
     return m, solver, model_state(m, gpu, capacity=4 * m.particle_count)
 
 
@@ -307,7 +310,8 @@ def test_solver_geometric_refinement_near_the_tool(gpu):
 
     m = scene.capsule_poke_scene()
     kw = {k: v for k, v in m.params.items() if k != "iterations"}
-    solver = RefinementSolver(m, m.params["iterations"], max_tets=4 * m.tet_count, max_particles=4 * m.particle_count,
+    solver = RefinementSolver(scene.pad_particles(m, 4 * m.particle_count), m.params["iterations"],
+                              max_tets=4 * m.tet_count, max_particles=4 * m.particle_count,
                               device=gpu, enable_refinement=True, refine_every_n_steps=2, refine_scoring="geometric",
                               refine_elastic_weight=0.0, refine_geometric_threshold=1.0, **kw)
     state = model_state(m, gpu, capacity=4 * m.particle_count)

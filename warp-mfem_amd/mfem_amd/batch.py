@@ -60,6 +60,9 @@ def union_model(models: list, far: float = 1.0e3, align: int = 256):
               tet_poses=np.concatenate(ps), tet_materials=np.concatenate(mats),
               tri_indices=np.concatenate(trs).astype(np.int32), shapes=list(m0.shapes), gravity=np.asarray(m0.gravity),
               params=dict(m0.params), name=f"batch_of_{len(models)}")
+    # the alignment rows after the last body are part of the layout: every
+    # row is active (not only those up to the largest tet vertex)
+    u.active_particle_count = nv
     return u, np.array(bv), np.array(bt), np.array(bf), np.concatenate(en)
 
 

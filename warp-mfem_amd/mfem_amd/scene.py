@@ -140,6 +140,31 @@ def build_model(particles, tets, materials, density, *, pinned=None, shapes=(), 
     )
 
 
+def pad_particles(model: Model, max_particles: int) -> Model:
+    """The model with its particle buffer padded to ``max_particles`` rows
+    (zero position, velocity and mass; inactive), as a caller pads it for
+    refinement headroom before constructing the solver
+    (src/mfem/refinement/models/refinement_model.py:104-143)."""
+    n = model.particle_count
+    if max_particles < n:
+        raise ValueError(f"max_particles={max_particles} < model.particle_count={n}")
+    import dataclasses
+
+    def pad(a):
+        a = np.asarray(a)
+        out = np.zeros((max_particles,) + a.shape[1:], dtype=a.dtype)
+        out[:n] = a
+        return out
+
+    out = dataclasses.replace(model, particle_q=pad(model.particle_q), particle_qd=pad(model.particle_qd),
+                              particle_mass=pad(model.particle_mass), particle_inv_mass=pad(model.particle_inv_mass),
+                              params=dict(model.params))
+    # attributes a scene added beside the fields (drive settings and the like)
+    names = {f.name for f in dataclasses.fields(model)}
+    out.__dict__.update({k: v for k, v in model.__dict__.items() if k not in names})
+    return out
+
+
 def _materials(mu, lam, n):
     m = np.zeros((n, 3))
     m[:, 0] = mu

@@ -99,14 +99,23 @@ class RefinementSolver:
         energy = kwargs.get("energy", "arap")
         if energy not in _ENERGIES:
             raise ValueError(f"Unknown energy type: {energy}")
-        nv, nt = int(model.particle_count), int(model.tet_count)
-        if max_particles is not None and max_particles < nv:
-            raise ValueError(f"max_particles={max_particles} < model.particle_count={nv}")
+        nt = int(model.tet_count)
+        # the model's particle buffer is already padded to max_particles by
+        # the caller (solver.py:130-136); the active particles are those up
+        # to the largest tet vertex (AdditionalState.from_model, get_active_particles)
+        if max_particles is not None and max_particles != model.particle_count:
+            raise ValueError(
+                f"max_particles={max_particles} does not match model.particle_count={model.particle_count}; the "
+                "model's particle buffer must already be padded to max_particles by the caller (e.g. "
+                "MFEMRefinementModel.load_sim_model(..., max_particles=...) or scene.pad_particles) before the "
+                "solver is constructed")
+        nv = int(np.asarray(model.tet_indices).max()) + 1 if nt > 0 else 0
+        nv = int(getattr(model, "active_particle_count", nv))  # a layout may declare its active rows
         if max_tets < nt:
             raise ValueError(f"max_tets={max_tets} < model.tet_count={nt}")
         tris = np.zeros((0, 3), np.int32) if model.tri_indices is None else np.asarray(model.tri_indices, np.int32)
         tris = tris.reshape(-1, 3)
-        Vc = nv if max_particles is None else int(max_particles)
+        Vc = int(model.particle_count)
         Tc = int(max_tets)
         # surface triangles grow with the tets (same headroom when not given)
         Fc = int(max_tris) if max_tris is not None else int(np.ceil(len(tris) * Tc / max(nt, 1)))
@@ -232,7 +241,12 @@ class RefinementSolver:
         b["ws_y"] = zf(Vc, 3)
         b["guess"] = zf(Vc, 3)
         self.buffers = b
-        self._pinned = torch.as_tensor(pad(inv_mass == 0, Vc, np.bool_), device=dev)
+        # attachments: the active particles with inv_mass = 0, fixed at
+        # construction (padding rows and vertices added by refinement are
+        # never pinned; solver.py:320-393, 395-425)
+        pinned = np.zeros(Vc, np.bool_)
+        pinned[:nv] = inv_mass[:nv] == 0
+        self._pinned = torch.as_tensor(pinned, device=dev)
         self._refine_density = float(kwargs.get("refine_density", 1.0))
         self._refine_kw = dict(
             threshold=float(min_refine_score),
