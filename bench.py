@@ -44,6 +44,7 @@ usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                        [--no-learned] [--workload octopus|beam|batch]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -92,6 +93,7 @@ def main():
                     help="uniform smoothing weight of the coarse basis when the network is off")
     ap.add_argument("--uniform-omega", type=float, default=0.75,
                     help="the hand-tuned uniform weight of the ablation that replaces the network")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph legs")
     ap.add_argument("--dump-iters", default=None,
                     help="write the CG iteration count of every PCG launch, in launch order, to this JSON file "
                          "(to divide per-dispatch hardware counters by iterations)")
@@ -137,12 +139,11 @@ def main():
     state = model_state(m, "cuda:0")
     out = state.clone()
     print(f"[bench] {wl}: {m.tet_count} tets, {m.particle_count} verts, warmup {args.warmup}", flush=True)
-    launch_iters = []  # CG iterations per PCG launch, in launch order (with --dump-iters)
+    if args.dump_iters:
+        solver.launch_log = []  # (variant, CG iterations) of every PCG launch, in launch order
     for k in range(args.warmup):
         solver.step(state, out, None, None, dt)
         state, out = out, state
-        if args.dump_iters:
-            launch_iters += solver.last_stats()[:, 0].astype(int).tolist()
     torch.cuda.synchronize()
     if solver.status() != 0:
         raise SystemExit("device error during warmup")
@@ -296,12 +297,6 @@ def main():
     s0 = state.clone()
     snap0 = solver.snapshot()  # every leg starts from the same solver state
     wall_p, st_p, timers_p, end_p = timed_leg(s0, None, None, False, ws=False)
-    if args.dump_iters:
-        launch_iters += st_p[:, :, 0].astype(int).reshape(-1).tolist()
-        os.makedirs(os.path.dirname(os.path.abspath(args.dump_iters)), exist_ok=True)
-        with open(args.dump_iters, "w") as f:
-            json.dump({"kernel": "k_pcg", "iters_per_launch": launch_iters, "tets": m.tet_count,
-                       "verts": m.particle_count, "nnz_blocks": int(len(solver.pattern.cols))}, f)
     print(f"[bench] plain: {args.steps} steps in {wall_p:.3f} s", flush=True)
     abl = {}
     if learned:
@@ -327,14 +322,18 @@ def main():
         wall, st = wall_p, st_p
     # the plain and the value configuration replayed as HIP graphs; the
     # value is the faster mode, compared with the plain leg in the same mode
-    wall_pg, st_pg, _, _ = timed_leg(s0, None, None, False, ws=False, mode="graph")
-    if learned:
-        wall_g, st_g, _, _ = timed_leg(s0, net, cn, True, args.coarse_omega, mode="graph")
+    if args.no_graph:
+        wall_pg, st_pg, wall_g, st_g = float("inf"), st_p, float("inf"), st
     else:
-        wall_g, st_g = wall_pg, st_pg
+        wall_pg, st_pg, _, _ = timed_leg(s0, None, None, False, ws=False, mode="graph")
+        if learned:
+            wall_g, st_g, _, _ = timed_leg(s0, net, cn, True, args.coarse_omega, mode="graph")
+        else:
+            wall_g, st_g = wall_pg, st_pg
     print(f"[bench] HIP graph replays: plain {wall_pg:.3f} s, value configuration {wall_g:.3f} s", flush=True)
     eager = {"value_ms_per_step": wall * 1e3 / args.steps, "plain_ms_per_step": wall_p * 1e3 / args.steps}
-    graph = {"value_ms_per_step": wall_g * 1e3 / args.steps, "plain_ms_per_step": wall_pg * 1e3 / args.steps}
+    graph = None if args.no_graph else {"value_ms_per_step": wall_g * 1e3 / args.steps,
+                                         "plain_ms_per_step": wall_pg * 1e3 / args.steps}
     wall_e = wall
     step_mode = "graph" if wall_g < wall else "eager"
     if step_mode == "graph":
@@ -468,6 +467,8 @@ def main():
     result["roofline_spmv"] = spmv_roofline(solver, V, nnz)
     if learned:
         result["mlp"] = mlp_mfma(solver, cnet, V)
+    if args.dump_iters:
+        dump_launch_log(args.dump_iters, solver, m, nnz)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(m, dt, s0, snap0, single_thread=not args.cpu_no_single_thread)
     print(json.dumps(result), flush=True)
@@ -817,26 +818,52 @@ def assembly_roofline(phase_ms, n_newton, T, V, B, octopus):
 
 def spmv_roofline(solver, V, B):
     """The stand-alone SELL-64 SpMV (k_spmv, mfem_spmv) on the last assembled
-    system, event timed on the launching stream: algorithmic bytes = 40 B +
+    system, event timed on the launching stream over graph replays (and, for
+    comparison, over back-to-back host launches): algorithmic bytes = 40 B +
     4 (V + 1) + 24 V (block values and column, row pointer, x and y)."""
     import torch
 
+    from mfem_amd import _ffi
+
     x = torch.randn(solver.max_particles, 3, device=solver.device)
+    y = torch.empty_like(x)
+    S = solver._S
+    stream = solver._stream
+
+    def launch():
+        _ffi.call("mfem_spmv", ctypes.byref(S), ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                  stream())
+
     for _ in range(3):
-        solver.spmv(x)
+        launch()
     n = 50
     a = torch.cuda.Event(enable_timing=True)
     b = torch.cuda.Event(enable_timing=True)
     a.record()
     for _ in range(n):
-        solver.spmv(x)
+        launch()
     b.record()
     torch.cuda.synchronize()
-    us = a.elapsed_time(b) * 1e3 / n
+    us_eager = a.elapsed_time(b) * 1e3 / n
+    # the same launches replayed from a HIP graph: no host launch cost
+    # between them (the kernel's own duration plus the device's dispatch)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(n):
+            launch()
+    g.replay()
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(4):
+        g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    us = a.elapsed_time(b) * 1e3 / (4 * n)
     algo = 40.0 * B + 4.0 * (V + 1) + 24.0 * V
     gbs = algo / (us * 1e-6) / 1e9
     return {"kernel": "k_spmv", "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": gbs / HBM_PEAK_GBS, "us_per_launch": us, "bytes_per_launch": algo}
+            "frac": gbs / HBM_PEAK_GBS, "us_per_launch": us, "us_per_launch_eager": us_eager,
+            "bytes_per_launch": algo, "timing": "HIP events around 4 replays of a graph of 50 launches"}
 
 
 def mlp_mfma(solver, net, V):
@@ -860,6 +887,20 @@ def mlp_mfma(solver, net, V):
     tflops = flop / (us * 1e-6) / 1e12
     return {"kernel": "k_mlp_forward", "bound": "mfma", "achieved": tflops, "peak": BF16_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": tflops / BF16_PEAK_TFLOPS, "us_per_launch": us, "flop_per_launch": flop}
+
+
+def dump_launch_log(path, solver, m, nnz):
+    """Every PCG launch of the run in launch order (kernel name, CG
+    iterations), to divide per-dispatch hardware counters by iterations."""
+    modes = {"standard": 0, "pipelined": 1, "standard_lowreg": 2, "pipelined_coarse": 3}
+    names = {}
+    for v in solver.pcg_variants():
+        names[v["variant"]] = ("k_pcg_lowreg" if v["mode"] == "standard_lowreg" else
+                               f"k_pcg<{v['chunks_per_wave']}, {v['resident_slots']}, {modes[v['mode']]}>")
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump({"launches": [[names.get(v, str(v)), n] for v, n in solver.launch_log], "tets": m.tet_count,
+                   "verts": m.particle_count, "nnz_blocks": nnz}, f)
 
 
 def _cpu_info():

@@ -533,6 +533,9 @@ const char* mfem_refine_last_error(void);
    only n_chunks, max_width, n_bodies, body_wg_align, local_sweeps and
    pcg_flags of S (a host-only struct may ask before any buffer exists).
    Returns MFEM_EINVAL when k >= mfem_pcg_variant_count(). */
+/* the PCG variant index the calling thread's last mfem_global_solve (or
+   mfem_step) launched, -1 before any (profiling: per-dispatch counters) */
+int mfem_last_pcg_variant(void);
 int mfem_pcg_variant_count(void);
 /* Co-resident workgroups of the persistent line-search kernel (the bound
    for ls_blocks of a batch). */

@@ -44,6 +44,7 @@ static constexpr uint64_t kStatusCheckAfter = 2000ull;
 static constexpr int kPollPasses = 6;
 
 static thread_local std::string g_last_error;
+static thread_local int g_last_pcg_variant = -1;  // the variant the last mfem_global_solve launched
 
 static int fail(int code, const std::string& msg) {
   g_last_error = msg;
@@ -3284,6 +3285,7 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
     }
   }
   if (sel < 0) return fail(MFEM_ECAPACITY, "system too large for the persistent PCG kernel");
+  g_last_pcg_variant = sel;
   mfem_solver Sc = *S;
   void* args[] = {(void*)&Sc, (void*)&newton_iter};
   HIP_TRY(hipLaunchCooperativeKernel(kPcgVariants[sel].fn, dim3(nblocks), dim3(kBlock), args, 0, (hipStream_t)stream));
@@ -3364,6 +3366,8 @@ int mfem_step(const mfem_solver* S, float dt, void* stream) {
 }
 
 int mfem_pcg_variant_count(void) { return kNumPcgVariants; }
+
+int mfem_last_pcg_variant(void) { return g_last_pcg_variant; }
 
 int mfem_line_search_max_blocks(void) {
   CoopInfo* ci = nullptr;

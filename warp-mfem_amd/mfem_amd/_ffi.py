@@ -68,7 +68,7 @@ ENTRY_POINTS = [
     "mfem_pattern_last_error", "mfem_pose_shapes", "mfem_scan_i32_noalloc", "mfem_coarse_features",
     "mfem_coarse_weights", "mfem_broadphase_swept", "mfem_self_contact_workspace_bytes",
     "mfem_self_contact_candidates", "mfem_self_contact_eval", "mfem_self_contact_ccd",
-    "mfem_self_contact_last_error",
+    "mfem_self_contact_last_error", "mfem_last_pcg_variant",
 ]
 
 
@@ -151,6 +151,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         getattr(lib, name).argtypes = [P, vp]
     lib.mfem_self_contact_candidates.argtypes = [P, i, vp]
     lib.mfem_self_contact_last_error.restype = ctypes.c_char_p
+    lib.mfem_last_pcg_variant.argtypes = []
     _lib = lib
     return lib
 

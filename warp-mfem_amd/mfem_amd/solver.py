@@ -313,6 +313,7 @@ class RefinementSolver:
         self._status_mirror = torch.zeros(1, dtype=torch.int32, pin_memory=self.device.type == "cuda")
         self._pattern_builder = None
         self._set_topology(nv, nt, len(tris))
+        self.launch_log = None  # a list: every step's PCG launches (profiling)
         self.warm_start_net = None
         self.set_warm_start(kwargs.get("warm_start"))
         self.coarse_net = None
@@ -651,6 +652,16 @@ class RefinementSolver:
             self._refine_pass(state_out)
         self._call("mfem_step", ctypes.c_float(dt), self._stream())
         self._mirror_status()
+        self._log_launches()
+
+    def _log_launches(self, newton_iter: int | None = None) -> None:
+        """With ``launch_log`` a list (profiling), appends the PCG launches of
+        the last step (or of Newton iteration ``newton_iter``): the variant
+        and the CG iterations of each, in launch order (synchronises)."""
+        if self.launch_log is not None:
+            v = int(self.lib.mfem_last_pcg_variant())
+            it = self.last_stats()[:, 0].astype(int).tolist()
+            self.launch_log += [(v, n) for n in (it if newton_iter is None else [it[newton_iter]])]
 
     def _check_device_status(self) -> None:
         """Raises when an earlier step ended with a device error (a grid
@@ -746,6 +757,7 @@ class RefinementSolver:
             self._call("mfem_global_solve", ctypes.c_int(newton_iter), self._stream())
         finally:
             self._S.pcg_flags = flags
+        self._log_launches(newton_iter)
 
     def local_solve(self):
         self._call("mfem_local_solve", self._stream())
@@ -964,8 +976,12 @@ class RefinementSolver:
         self.step(state_in, state_out, None, None, dt)  # warm-up outside the capture
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            self.step(state_in, state_out, None, None, dt)
+        log, self.launch_log = self.launch_log, None  # (no host synchronisation under capture)
+        try:
+            with torch.cuda.graph(g):
+                self.step(state_in, state_out, None, None, dt)
+        finally:
+            self.launch_log = log
         return g
 
     # diagnostics ------------------------------------------------------------
