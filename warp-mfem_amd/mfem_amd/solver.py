@@ -668,10 +668,16 @@ class RefinementSolver:
         barrier timed out): every kernel of a step then does nothing, so the
         state stops advancing until ``reset_status``.  Reads the pinned
         mirror of the last completed step: no host synchronisation."""
-        if int(self._status_mirror[0]) != 0:
-            raise _ffi.MfemError("a grid barrier timed out in an earlier step (device error bits "
-                                 f"{int(self._status_mirror[0])}); the state was left untouched since; "
-                                 "call reset_status() to continue")
+        bits = int(self._status_mirror[0])
+        if bits != 0:
+            what = []
+            if bits & _ffi.MFEM_ERR_BARRIER:
+                what.append("a grid barrier timed out")
+            if bits & _ffi.MFEM_ERR_CONTACT:
+                what.append("self-contact candidates overflowed (raise self_contact_max_pairs / "
+                            "self_contact_max_span or self_contact_cell)")
+            raise _ffi.MfemError(f"{' and '.join(what) or 'device error'} in an earlier step (device error bits "
+                                 f"{bits}); the state was left untouched since; call reset_status() to continue")
 
     def _mirror_status(self) -> None:
         self._status_mirror.copy_(self.buffers["status"][:1], non_blocking=True)
@@ -996,9 +1002,10 @@ class RefinementSolver:
         return st[:, : self.iterations].max(axis=0)
 
     def status(self) -> int:
-        """Device error bits (bit 0: a grid barrier timed out; every kernel of
-        a step then returns at once, leaving x, v, s and lambda untouched,
-        until ``reset_status``; the next ``step`` raises)."""
+        """Device error bits (MFEM_ERR_BARRIER: a grid barrier timed out;
+        MFEM_ERR_CONTACT: self-contact candidates overflowed; every kernel
+        of a step then returns at once until ``reset_status``; the next
+        ``step`` raises)."""
         torch.cuda.synchronize(self.device)
         return int(self.buffers["status"][0].item())
 
