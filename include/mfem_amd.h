@@ -275,6 +275,21 @@ typedef struct mfem_solver {
   float* sc_alpha;           /* [1] largest line-search step the CCD allows (<= 1) */
   void* sc_ws;               /* broad-phase workspace */
   int64_t sc_ws_bytes;
+  /* membrane shell over the surface triangles (src/mfem/refinement/
+     shell_energy.py): per triangle E = mu_s A |F - R(F)|^2 / 2, F = [x1 - x0,
+     x2 - x0] W the 3x2 in-plane deformation gradient, R its polar factor, A
+     and W = Dm^-1 from the rest positions; its gradient joins g and the
+     slope, its energy the merit, and its constant Gauss-Newton Hessian
+     mu_s A (w_a . w_b) I3 on the triangle's vertex pairs joins H
+     (solver.py:297-302, 638-639, 789-797, 864-867, 910-911).  tri_W,
+     tri_area and shell_h are written by mfem_shell_refresh. */
+  int32_t shell_on;          /* 1: on (needs n_tris > 0 and the tcon / vtri lists) */
+  float shell_mu;            /* surface shear modulus mu_s in Pa m (shell_mu kwarg) */
+  const float* rest;         /* [n_verts][3] rest positions (AdditionalState.rest_particle_q) */
+  float* tri_W;              /* [n_tris][4] W row-major (0 for a degenerate rest triangle) */
+  float* tri_area;           /* [n_tris] rest area (0 for a degenerate rest triangle) */
+  float* shell_h;            /* [n_slots*64] per lane-slot sum of mu_s A w_a . w_b (block = value I3) */
+  float* gS;                 /* [n_verts][3] shell gradient */
 } mfem_solver;
 
 /* ABI self-description (checked against the Python mirror by the tests) */
@@ -299,6 +314,13 @@ int mfem_begin_step(const mfem_solver* S, float dt, void* stream);
    (elastic_energy.py), constraints c, G (kernels.py:131-169) and the Schur
    blocks H_lambda, g_lambda (kernels.py:189-249). */
 int mfem_update_system(const mfem_solver* S, float dt, void* stream);
+
+/* Shell.refresh (src/mfem/refinement/shell_energy.py:137-186, 262-293):
+   tri_W and tri_area of every triangle from the rest positions, then
+   shell_h of every lane-slot from its tcon list.  Call once per topology
+   (construction and after every refine pass that changed the mesh), with
+   shell_on set. */
+int mfem_shell_refresh(const mfem_solver* S, void* stream);
 
 /* _assemble_system (solver.py:641-799): H into the SELL pattern, g. */
 int mfem_assemble(const mfem_solver* S, float dt, void* stream);

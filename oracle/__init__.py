@@ -76,6 +76,12 @@ def _lib(bits: int, omp: bool = False) -> ctypes.CDLL:
         getattr(lib, name).restype = ctypes.c_double
     lib.orc_sc_toi.argtypes = [P, P, P]
     lib.orc_sc_toi.restype = ctypes.c_double
+    lib.orc_shell_refresh.argtypes = [P]
+    lib.orc_shell_refresh.restype = None
+    lib.orc_tri_rest_frame.argtypes = [P, P, P, P]
+    lib.orc_tri_rest_frame.restype = real
+    lib.orc_membrane.argtypes = [P, P, P, P, real, real, P]
+    lib.orc_membrane.restype = real
     _LIBS[(bits, omp)] = lib
     return lib
 
@@ -220,6 +226,24 @@ class Kernels:
                                      self._p(w), self._p(n), self._p(g), self._p(H))
         return float(E), g.reshape(3, 3), H.reshape(3, 3, 3, 3), float(d[0]), w, n
 
+    def tri_rest_frame(self, r0, r1, r2):
+        """(W (2, 2) inverse in-plane rest-shape matrix, rest area) of one
+        rest triangle (src/mfem/refinement/shell_energy.py:37-63)."""
+        r0, r1, r2 = (self._arr(v, 3) for v in (r0, r1, r2))
+        W = np.zeros(4, self.dt)
+        area = self.lib.orc_tri_rest_frame(self._p(r0), self._p(r1), self._p(r2), self._p(W))
+        return W.reshape(2, 2), float(area)
+
+    def membrane(self, x0, x1, x2, W, area, mu):
+        """(energy, gradient (3, 3) per corner) of one membrane triangle
+        (shell_energy.py:66-125, 190-229)."""
+        x0, x1, x2 = (self._arr(v, 3) for v in (x0, x1, x2))
+        W = self._arr(W, 4)
+        g = np.zeros(9, self.dt)
+        real = self._real_t()
+        E = self.lib.orc_membrane(self._p(x0), self._p(x1), self._p(x2), self._p(W), real(area), real(mu), self._p(g))
+        return float(E), g.reshape(3, 3)
+
     def closest_point_triangle_bary(self, x, a, b, c):
         x, a, b, c = (self._arr(v, 3) for v in (x, a, b, c))
         w = np.zeros(3, self.dt)
@@ -242,7 +266,8 @@ class Oracle:
 
     INT_FIELDS = {"tets", "pin_idx", "tris", "shape_type", "rowptr", "col", "diag", "lag_sid"}
 
-    def __init__(self, scene, bits: int = 64, params: dict | None = None, threads: int | None = None):
+    def __init__(self, scene, bits: int = 64, params: dict | None = None, threads: int | None = None,
+                 rest=None):
         self.bits = bits
         self.lib = _lib(bits, omp=threads is not None)
         self.threads = self.lib.orc_set_threads(threads) if threads is not None else 1
@@ -304,6 +329,11 @@ class Oracle:
                 raise KeyError(k)
         if self.lib.orc_finalize(self.ctx) != 0:
             raise ValueError("bad topology")
+        # rest positions (AdditionalState.rest_particle_q): the model's
+        # positions unless given (a refined mesh's interpolated rest)
+        self.buf("rest")[:] = np.asarray(scene.particle_q if rest is None else rest)[:nv].reshape(-1)
+        if ntris:
+            self.lib.orc_shell_refresh(self.ctx)
         self.lib.orc_init_stretch(self.ctx)
 
     def set_body_q(self, body_q, scene=None) -> None:

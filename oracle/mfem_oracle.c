@@ -30,11 +30,14 @@
  *                           src/mfem/ipc/distance.py:4-57
  *   triangle vs capsule     src/mfem/refinement/contact.py:417-775,
  *                           src/mfem/collision.py:276-322
+ *   membrane shell          src/mfem/refinement/shell_energy.py:37-331,
+ *                           solver.py:291-302, 638-639, 789-797, 864-867, 910-911
  *
  * Functions with no check in tests/ that pins them are listed as
  * "parity unpinned" in the header comment of tests/test_oracle.py; the pins
  * of friction, shape distances, the triangle barrier, the local solve and
- * the GMW81 correction are in tests/test_oracle_pins.py.
+ * the GMW81 correction are in tests/test_oracle_pins.py, those of the
+ * membrane shell in tests/test_oracle_shell.py.
  */
 #include <math.h>
 #include <stdint.h>
@@ -635,6 +638,71 @@ EXPORT real orc_closest_point_triangle_segment(const real* a, const real* b, con
 }
 
 /* ------------------------------------------------------------------ */
+/* membrane shell over the surface triangles                            */
+/* (src/mfem/refinement/shell_energy.py:37-233)                         */
+/* ------------------------------------------------------------------ */
+/* Inverse 2D rest-shape matrix W (row-major 2x2) of rest triangle r0 r1 r2
+   in the in-plane frame u = e1/|e1|, v = n x u, n the unit normal; returns
+   the rest area, or 0 with W = 0 for a degenerate triangle (|e1 x e2| <
+   1e-12) (shell_energy.py:37-63). */
+EXPORT real orc_tri_rest_frame(const real* r0, const real* r1, const real* r2, real* W) {
+  real e1[3], e2[3], n[3];
+  for (int k = 0; k < 3; ++k) { e1[k] = r1[k] - r0[k]; e2[k] = r2[k] - r0[k]; }
+  cross3(e1, e2, n);
+  real ta = norm3(n);
+  if (ta < (real)1e-12) { W[0] = W[1] = W[2] = W[3] = 0; return 0; }
+  real u[3], nn[3], v[3], le = norm3(e1);
+  for (int k = 0; k < 3; ++k) { u[k] = e1[k] / le; nn[k] = n[k] / ta; }
+  cross3(nn, u, v);
+  real a = dot3(e1, u), b = dot3(e2, u), c = dot3(e1, v), d = dot3(e2, v);
+  real det = a * d - b * c;
+  W[0] = d / det; W[1] = -b / det; W[2] = -c / det; W[3] = a / det;
+  return (real)0.5 * ta;
+}
+
+/* Membrane energy mu A |F - R(F)|^2 / 2 of one triangle, F = [x1 - x0,
+   x2 - x0] W (3x2), R its polar factor, through the closed forms of the
+   2x2 C = F^T F: sigma1 + sigma2 = sqrt(tr C + 2 sqrt(det C)) and
+   F - R = F (I - S^-1), S = (C + sqrt(det C) I) / (sigma1 + sigma2); zero
+   stress when sqrt(det C) < 1e-10 (shell_energy.py:66-125, 190-229).
+   grad (9, may be null) = dE/dx0, dE/dx1, dE/dx2.  Returns the energy. */
+EXPORT real orc_membrane(const real* x0, const real* x1, const real* x2, const real* W, real area, real mu,
+                         real* grad) {
+  real d1[3], d2[3], f1[3], f2[3];
+  for (int k = 0; k < 3; ++k) {
+    d1[k] = x1[k] - x0[k];
+    d2[k] = x2[k] - x0[k];
+    f1[k] = d1[k] * W[0] + d2[k] * W[2];
+    f2[k] = d1[k] * W[1] + d2[k] * W[3];
+  }
+  real c11 = dot3(f1, f1), c12 = dot3(f1, f2), c22 = dot3(f2, f2);
+  real dc = c11 * c22 - c12 * c12;
+  if (dc < 0) dc = 0;
+  real tr2 = c11 + c22 + 2 * (real)sqrt((double)dc);
+  real ssum = (real)sqrt((double)(tr2 > (real)1e-20 ? tr2 : (real)1e-20));
+  real E = mu * area * (real)0.5 * (c11 + c22 - 2 * ssum + 2);
+  if (grad) {
+    real s = (real)sqrt((double)dc), p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0};
+    if (s >= (real)1e-10) {
+      real den = ssum * s;
+      real i11 = (c22 + s) / den, i12 = -c12 / den, i22 = (c11 + s) / den;
+      for (int k = 0; k < 3; ++k) {
+        p1[k] = mu * area * (f1[k] - (f1[k] * i11 + f2[k] * i12));
+        p2[k] = mu * area * (f2[k] - (f1[k] * i12 + f2[k] * i22));
+      }
+    }
+    for (int k = 0; k < 3; ++k) {
+      real g1 = p1[k] * W[0] + p2[k] * W[1];
+      real g2 = p1[k] * W[2] + p2[k] * W[3];
+      grad[3 + k] = g1;
+      grad[6 + k] = g2;
+      grad[k] = -(g1 + g2);
+    }
+  }
+  return E;
+}
+
+/* ------------------------------------------------------------------ */
 /* solver context                                                       */
 /* ------------------------------------------------------------------ */
 typedef struct {
@@ -668,18 +736,22 @@ typedef struct {
      diagonal Hessian blocks (FP64, sym6 order 00 11 22 01 02 12) */
   int npts, nedges; int* pts; int* edges;
   double *sc_g, *sc_h;
+  /* membrane shell: rest positions, per-triangle inverse rest-shape matrix
+     and rest area (set by orc_shell_refresh), energy per triangle,
+     gradient per vertex */
+  real *rest, *tW, *tarea, *eS, *gS;
 } orc_ctx;
 
 enum {
   P_ENERGY = 0, P_GX, P_GY, P_GZ, P_ATTACH_K, P_D0, P_D1, P_KAPPA, P_EPSV,
   P_ITERS, P_CG_MAXIT, P_CG_TOL, P_PRECOND, P_LS, P_LS_MAX, P_LS_A0, P_LS_TAU,
-  P_LS_C, P_LS_THR, P_SING, P_TRI_CONTACT, P_GUESS, P_SC, P_SC_DHAT, P_SC_KAPPA, P_SC_S,
+  P_LS_C, P_LS_THR, P_SING, P_TRI_CONTACT, P_GUESS, P_SC, P_SC_DHAT, P_SC_KAPPA, P_SC_S, P_SHELL_MU,
   P_QB0, P_QB1, P_QB2, P_NPARAMS
 };
 static const char* PNAMES[] = {"energy", "gx", "gy", "gz", "attach_k", "d0", "d1", "kappa", "eps_v",
                                "iterations", "cg_maxiter", "cg_tol", "precond", "line_search", "ls_max", "ls_alpha0",
                                "ls_tau", "ls_c", "ls_thr", "sing_thr", "tri_contact", "guess", "self_contact",
-                               "sc_dhat", "sc_kappa", "sc_ccd_s", NULL};
+                               "sc_dhat", "sc_kappa", "sc_ccd_s", "shell_mu", NULL};
 
 #define ALLOC(ptr, n) ptr = calloc((size_t)((n) > 0 ? (n) : 1), sizeof(*(ptr)))
 
@@ -712,6 +784,7 @@ EXPORT orc_ctx* orc_create(int nv, int nt, int ntris, int nshapes, int npins) {
   C->p[P_LS_THR] = (real)1e-8; C->p[P_SING] = (real)1e-20; C->p[P_TRI_CONTACT] = 1;
   C->p[P_SC] = 0; C->p[P_SC_DHAT] = (real)1e-3; C->p[P_SC_KAPPA] = (real)1e7; C->p[P_SC_S] = (real)0.1;
   ALLOC(C->sc_g, 3 * nv); ALLOC(C->sc_h, 6 * nv);
+  ALLOC(C->rest, 3 * nv); ALLOC(C->tW, 4 * ntris); ALLOC(C->tarea, ntris); ALLOC(C->eS, ntris); ALLOC(C->gS, 3 * nv);
   return C;
 }
 
@@ -754,6 +827,8 @@ EXPORT void* orc_buf(orc_ctx* C, const char* name, int64_t* count, int* is_int) 
   B("rowptr", C->rowptr, C->nv + 1, 1) B("col", C->col, C->nnzb, 1) B("val", C->val, 9 * C->nnzb, 0)
   B("diag", C->diag, C->nv, 1) B("tdist", C->tdist, C->ntris, 0) B("tbary", C->tbary, 3 * C->ntris, 0)
   B("stats", C->stats, 4096, 0) B("sc_g", C->sc_g, 3 * C->nv, 2) B("sc_h", C->sc_h, 6 * C->nv, 2)
+  B("rest", C->rest, 3 * C->nv, 0) B("tW", C->tW, 4 * C->ntris, 0) B("tarea", C->tarea, C->ntris, 0)
+  B("eS", C->eS, C->ntris, 0) B("gS", C->gS, 3 * C->nv, 0)
 #undef B
   *count = 0; *is_int = 0;
   return NULL;
@@ -1184,6 +1259,37 @@ EXPORT double orc_sc_energy(orc_ctx* C, const real* x) { return sc_pass(C, x, NU
 EXPORT double orc_sc_gradient(orc_ctx* C, const real* x) { return sc_pass(C, x, NULL, 0); }
 EXPORT double orc_sc_toi(orc_ctx* C, const real* x, const real* dxs) { return sc_pass(C, x, dxs, 2); }
 
+/* Shell.refresh (shell_energy.py:262-293): every triangle's W and rest
+   area from the rest positions.  Call after the rest positions or the
+   triangles change. */
+EXPORT void orc_shell_refresh(orc_ctx* C) {
+  for (int t = 0; t < C->ntris; ++t) {
+    const int* tr = C->tris + 3 * t;
+    C->tarea[t] = orc_tri_rest_frame(C->rest + 3 * tr[0], C->rest + 3 * tr[1], C->rest + 3 * tr[2], C->tW + 4 * t);
+  }
+}
+
+/* shell energy per triangle into eS, gradient per vertex into gS, at x
+   (Shell.evaluate, shell_energy.py:295-314); returns the summed energy */
+static double shell_eval(orc_ctx* C, const real* x, int with_grad) {
+  double sum = 0;
+  real mu = C->p[P_SHELL_MU];
+  if (with_grad) memset(C->gS, 0, sizeof(real) * 3 * (size_t)C->nv);
+  for (int t = 0; t < C->ntris; ++t) {
+    const int* tr = C->tris + 3 * t;
+    real gr[9];
+    real E = orc_membrane(x + 3 * tr[0], x + 3 * tr[1], x + 3 * tr[2], C->tW + 4 * t, C->tarea[t], mu,
+                          with_grad ? gr : NULL);
+    if (with_grad) {
+      C->eS[t] = E;
+      for (int a = 0; a < 3; ++a)
+        for (int r = 0; r < 3; ++r) C->gS[3 * tr[a] + r] += gr[3 * a + r];
+    }
+    sum += (double)E;
+  }
+  return sum;
+}
+
 EXPORT void orc_update_system(orc_ctx* C, real dt) {
   int nv = C->nv, nt = C->nt;
   real idt2 = 1 / (dt * dt);
@@ -1232,6 +1338,7 @@ EXPORT void orc_update_system(orc_ctx* C, real dt) {
     }
   }
   if (C->p[P_SC] != 0) sc_pass(C, C->x, NULL, 0);
+  if (C->p[P_SHELL_MU] > 0) shell_eval(C, C->x, 1);
   int energy = (int)C->p[P_ENERGY];
 #pragma omp parallel for schedule(static)
   for (int e = 0; e < nt; ++e) {
@@ -1340,6 +1447,33 @@ EXPORT void orc_assemble(orc_ctx* C, real dt) {
           if (pos < 0) continue;
           for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) C->val[9 * pos + 3 * r + c] += w[a] * w[bb] * k * h2 * n[r] * n[c];
+        }
+      }
+    }
+  }
+  /* membrane shell: g -= grad, and the constant Gauss-Newton blocks
+     mu A (w_a . w_b) I3, w_1 = row 0 of W, w_2 = row 1, w_0 = -(w_1 + w_2)
+     (shell_energy.py:154-186, solver.py:789-797) */
+  if (C->p[P_SHELL_MU] > 0) {
+    real mu = C->p[P_SHELL_MU];
+    for (int i = 0; i < nv; ++i)
+      for (int r = 0; r < 3; ++r) C->g[3 * i + r] -= C->gS[3 * i + r];
+    for (int t = 0; t < C->ntris; ++t) {
+      const real* W = C->tW + 4 * t;
+      real w[3][2] = {{-(W[0] + W[2]), -(W[1] + W[3])}, {W[0], W[1]}, {W[2], W[3]}};
+      for (int a = 0; a < 3; ++a) {
+        int vi = C->tris[3 * t + a];
+        for (int bb = 0; bb < 3; ++bb) {
+          int vj = C->tris[3 * t + bb];
+          int lo = C->rowptr[vi], hi = C->rowptr[vi + 1] - 1, pos = -1;
+          while (lo <= hi) {
+            int mid = (lo + hi) / 2;
+            if (C->col[mid] == vj) { pos = mid; break; }
+            if (C->col[mid] < vj) lo = mid + 1; else hi = mid - 1;
+          }
+          if (pos < 0) continue;
+          real h = mu * C->tarea[t] * (w[a][0] * w[bb][0] + w[a][1] * w[bb][1]);
+          for (int r = 0; r < 3; ++r) C->val[9 * pos + 4 * r] += h;
         }
       }
     }
@@ -1504,7 +1638,8 @@ EXPORT double orc_merit(orc_ctx* C, const real* x, const real* s, real dt) {
     }
   }
   if (C->p[P_SC] != 0) bar += sc_pass(C, x, NULL, 1);
-  return kin * idt2 + att * idt2 + ela + bar + fri;
+  double shell = C->p[P_SHELL_MU] > 0 ? shell_eval(C, x, 0) : 0.0;
+  return kin * idt2 + att * idt2 + ela + bar + fri + shell;
 }
 
 /* backtracking line search (line_search.py:76-163, solver.py:871-928).
@@ -1515,6 +1650,8 @@ EXPORT int orc_line_search(orc_ctx* C, real dt, int* accepted, double* alpha_out
   double gdx = 0, gds = 0;
   real* gxf = C->q; /* scratch: g_x + gB + gD - G^T lambda */
   for (int k = 0; k < 3 * nv; ++k) gxf[k] = C->gx[k] + C->gB[k] + C->gD[k];
+  if (C->p[P_SHELL_MU] > 0)
+    for (int k = 0; k < 3 * nv; ++k) gxf[k] += C->gS[k];
 #pragma omp parallel for schedule(static)
   for (int e = 0; e < nt; ++e) {
     const real* G = C->G + 72 * e;
@@ -1601,7 +1738,7 @@ EXPORT void orc_destroy(orc_ctx* C) {
                   C->c, C->G, C->gs, C->Hs, C->Hsinv, C->Hl, C->gl, C->psi, C->gx, C->gB, C->gD, C->HB, C->HD,
                   C->dist, C->ddist, C->hdist, C->sid, C->eB, C->eD, C->g, C->dx, C->ds, C->Minv, C->tdist,
                   C->tbary, C->tnorm, C->tsid, C->r, C->z, C->pp, C->q, C->xtrial, C->strial, C->pts, C->edges,
-                  C->sc_g, C->sc_h};
+                  C->sc_g, C->sc_h, C->rest, C->tW, C->tarea, C->eS, C->gS};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; ++k) free(ptrs[k]);
   free(C);
 }

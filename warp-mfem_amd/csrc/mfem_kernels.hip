@@ -265,6 +265,130 @@ __device__ __forceinline__ bool tri_contact_on(const mfem_solver& S) {
 }
 
 // ---------------------------------------------------------------------------
+// membrane shell (src/mfem/refinement/shell_energy.py)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool shell_on(const mfem_solver& S) { return S.shell_on && S.n_tris > 0; }
+
+// columns f1, f2 of the 3x2 F = [x1 - x0, x2 - x0] W of triangle t at
+// x + a dx (shell_energy.py:66-80)
+__device__ __forceinline__ void shell_F(const mfem_solver& S, int t, float a, const float* W, float* f1, float* f2) {
+  const int i0 = S.tris[3 * t], i1 = S.tris[3 * t + 1], i2 = S.tris[3 * t + 2];
+  float x0[3], d1[3], d2[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    x0[k] = S.x[3 * i0 + k] + a * (float)S.dx[3 * i0 + k];
+    d1[k] = S.x[3 * i1 + k] + a * (float)S.dx[3 * i1 + k] - x0[k];
+    d2[k] = S.x[3 * i2 + k] + a * (float)S.dx[3 * i2 + k] - x0[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    f1[k] = d1[k] * W[0] + d2[k] * W[2];
+    f2[k] = d1[k] * W[1] + d2[k] * W[3];
+  }
+}
+
+// mu A |F - R|^2 / 2 = mu A (|F|^2 - 2 (sigma1 + sigma2) + 2) / 2 with
+// sigma1 + sigma2 = sqrt(tr C + 2 sqrt(det C)), C = F^T F
+// (shell_energy.py:83-96); FP64 from the FP32 columns (the merit sums it
+// against terms of similar size)
+__device__ __forceinline__ double shell_energy64(const float* f1, const float* f2, float mu, float area) {
+  const double c11 = (double)f1[0] * f1[0] + (double)f1[1] * f1[1] + (double)f1[2] * f1[2];
+  const double c12 = (double)f1[0] * f2[0] + (double)f1[1] * f2[1] + (double)f1[2] * f2[2];
+  const double c22 = (double)f2[0] * f2[0] + (double)f2[1] * f2[1] + (double)f2[2] * f2[2];
+  const double dc = fmax(c11 * c22 - c12 * c12, 0.0);
+  const double ssum = sqrt(fmax(c11 + c22 + 2.0 * sqrt(dc), 1e-20));
+  return (double)mu * (double)area * 0.5 * (c11 + c22 - 2.0 * ssum + 2.0);
+}
+
+// gradient of corner `corner` of a triangle: mu A (F - R) W^T mapped to the
+// corner, with F - R = F (I - S^-1), S = (C + sqrt(det C) I) / (sigma1 +
+// sigma2), zero stress when sqrt(det C) < 1e-10 (shell_energy.py:99-125,
+// 190-229)
+__device__ __forceinline__ void shell_corner_grad(const float* f1, const float* f2, const float* W, float mu,
+                                                  float area, int corner, float* g) {
+  const float c11 = f1[0] * f1[0] + f1[1] * f1[1] + f1[2] * f1[2];
+  const float c12 = f1[0] * f2[0] + f1[1] * f2[1] + f1[2] * f2[2];
+  const float c22 = f2[0] * f2[0] + f2[1] * f2[1] + f2[2] * f2[2];
+  const float sd = sqrtf(fmaxf(c11 * c22 - c12 * c12, 0.f));
+  const float den = sqrtf(fmaxf(c11 + c22 + 2.f * sd, 1e-20f));
+  if (sd < 1e-10f) {
+    g[0] = g[1] = g[2] = 0.f;
+    return;
+  }
+  const float sc = mu * area, inv = 1.f / (den * sd);
+  const float i11 = (c22 + sd) * inv, i12 = -c12 * inv, i22 = (c11 + sd) * inv;
+  // corner 1 takes row 0 of W, corner 2 row 1, corner 0 minus their sum
+  const float wa = corner == 1 ? W[0] : corner == 2 ? W[2] : -(W[0] + W[2]);
+  const float wb = corner == 1 ? W[1] : corner == 2 ? W[3] : -(W[1] + W[3]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float p1 = sc * (f1[k] - (f1[k] * i11 + f2[k] * i12));
+    const float p2 = sc * (f2[k] - (f1[k] * i12 + f2[k] * i22));
+    g[k] = p1 * wa + p2 * wb;
+  }
+}
+
+// per triangle: W = Dm^-1 in the in-plane frame u = e1 / |e1|, v = n x u,
+// and the rest area; both 0 for a degenerate rest triangle
+// (shell_energy.py:37-63, 137-152)
+__global__ void __launch_bounds__(kBlock) k_shell_frames(mfem_solver S) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= S.n_tris) return;
+  float r0[3], r1[3], r2[3];
+  load3(S.rest, S.tris[3 * t], r0);
+  load3(S.rest, S.tris[3 * t + 1], r1);
+  load3(S.rest, S.tris[3 * t + 2], r2);
+  const float e1[3] = {r1[0] - r0[0], r1[1] - r0[1], r1[2] - r0[2]};
+  const float e2[3] = {r2[0] - r0[0], r2[1] - r0[1], r2[2] - r0[2]};
+  float n[3];
+  cross3f(e1, e2, n);
+  const float ta = sqrtf(dot3f(n, n));
+  float W[4] = {0.f, 0.f, 0.f, 0.f}, area = 0.f;
+  if (!(ta < 1e-12f)) {
+    const float le = sqrtf(dot3f(e1, e1));
+    const float u[3] = {e1[0] / le, e1[1] / le, e1[2] / le};
+    const float nn[3] = {n[0] / ta, n[1] / ta, n[2] / ta};
+    float v[3];
+    cross3f(nn, u, v);
+    const float a = dot3f(e1, u), b = dot3f(e2, u), c = dot3f(e1, v), d = dot3f(e2, v);
+    const float det = a * d - b * c;
+    W[0] = d / det;
+    W[1] = -b / det;
+    W[2] = -c / det;
+    W[3] = a / det;
+    area = 0.5f * ta;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) S.tri_W[4 * t + k] = W[k];
+  S.tri_area[t] = area;
+}
+
+// per lane-slot: the constant Gauss-Newton coefficient sum over the
+// triangle vertex pairs (a, b) on this block of mu A (w_a . w_b), w_1 = row 0
+// of W, w_2 = row 1, w_0 = -(w_1 + w_2) (shell_energy.py:154-186)
+__global__ void __launch_bounds__(kBlock) k_shell_hess(mfem_solver S) {
+  const int l = blockIdx.x * kBlock + threadIdx.x;
+  if (l >= S.n_slots * 64) return;
+  float acc = 0.f;
+  for (int k = S.tcon_ptr[l]; k < S.tcon_ptr[l + 1]; ++k) {
+    const int src = S.tcon_src[k], t = src / 9, a = (src / 3) % 3, b = src % 3;
+    const float* W = S.tri_W + 4 * t;
+    const float w[3][2] = {{-(W[0] + W[2]), -(W[1] + W[3])}, {W[0], W[1]}, {W[2], W[3]}};
+    acc += S.shell_mu * S.tri_area[t] * (w[a][0] * w[b][0] + w[a][1] * w[b][1]);
+  }
+  S.shell_h[l] = acc;
+}
+
+// shell energy of triangle t at x + a dx (shell_energy.py:216-229)
+__device__ __forceinline__ double shell_merit(const mfem_solver& S, int t, float a) {
+  const float* W = S.tri_W + 4 * t;
+  const float Wl[4] = {W[0], W[1], W[2], W[3]};
+  float f1[3], f2[3];
+  shell_F(S, t, a, Wl, f1, f2);
+  return shell_energy64(f1, f2, S.shell_mu, S.tri_area[t]);
+}
+
+// ---------------------------------------------------------------------------
 // per-vertex kernels
 // ---------------------------------------------------------------------------
 
@@ -368,6 +492,21 @@ __global__ void __launch_bounds__(kBlock) k_vertex(mfem_solver S, float dt) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) gb[k] += wa * kde * S.tri_nrm[3 * t + k];
     }
+  }
+  if (shell_on(S)) {
+    // membrane shell gradient of the incident triangles, in incidence order
+    float gsh[3] = {0.f, 0.f, 0.f};
+    for (int kk = S.vtri_ptr[i]; kk < S.vtri_ptr[i + 1]; ++kk) {
+      const int src = S.vtri_src[kk], t = src / 3, a = src - 3 * t;
+      const float W[4] = {S.tri_W[4 * t], S.tri_W[4 * t + 1], S.tri_W[4 * t + 2], S.tri_W[4 * t + 3]};
+      float f1[3], f2[3], gc[3];
+      shell_F(S, t, 0.f, W, f1, f2);
+      shell_corner_grad(f1, f2, W, S.shell_mu, S.tri_area[t], a, gc);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gsh[k] += gc[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) S.gS[3 * i + k] = gsh[k];
   }
   float gd[3], hd[9];
   float ed = friction_eval(S, i, x, dt, gd, hd);
@@ -616,6 +755,13 @@ __global__ void __launch_bounds__(kBlock) k_assemble(mfem_solver S) {
         for (int c = 0; c < 3; ++c) acc[3 * r + c] += h * n[r] * n[c];
     }
   }
+  if (shell_on(S)) {
+    // the membrane shell's constant Gauss-Newton block (value I3)
+    const float h = S.shell_h[l];
+    acc[0] += h;
+    acc[4] += h;
+    acc[8] += h;
+  }
 #pragma unroll
   for (int q = 0; q < 9; ++q) S.sell_val[val_index(l, q)] = acc[q];
 }
@@ -648,6 +794,9 @@ __global__ void __launch_bounds__(kBlock) k_rhs(mfem_solver S, float dt) {
     hsc[2] = hsc[6] = h6[4];
     hsc[5] = hsc[7] = h6[5];
   }
+  if (shell_on(S))
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gsc[k] += S.gS[3 * i + k];
 #pragma unroll
   for (int k = 0; k < 3; ++k) S.g[3 * i + k] = g[k] - S.gB[3 * i + k] - S.gD[3 * i + k] - gsc[k];
   const float idt2 = 1.f / (dt * dt);
@@ -2826,6 +2975,9 @@ __device__ double vertex_slope_own(const mfem_solver& S, int i) {
   if (self_contact_on(S))
 #pragma unroll
     for (int k = 0; k < 3; ++k) acc += S.sc_g[3 * i + k] * S.dx[3 * i + k];
+  if (shell_on(S))
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc += (double)S.gS[3 * i + k] * S.dx[3 * i + k];
   return acc;
 }
 
@@ -2896,6 +3048,11 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
       f += tri_merit(S, t, 0.f);
       fa += tri_merit(S, t, a_first);
     }
+  if (shell_on(S))
+    for (int t = f0t + tid; t < f1t; t += nth) {
+      f += shell_merit(S, t, 0.f);
+      fa += shell_merit(S, t, a_first);
+    }
   if (sc) {
     const int npt = sc_pair_count(S, 0), nee = sc_pair_count(S, 1);
     for (int k = tid; k < npt; k += nth) {
@@ -2924,6 +3081,8 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
       for (int e = e0 + tid; e < e1; e += nth) f += tet_merit(S, e, a);
       if (tri_contact_on(S))
         for (int t = f0t + tid; t < f1t; t += nth) f += tri_merit(S, t, a);
+      if (shell_on(S))
+        for (int t = f0t + tid; t < f1t; t += nth) f += shell_merit(S, t, a);
       if (sc) {
         const int npt = sc_pair_count(S, 0), nee = sc_pair_count(S, 1);
         for (int k = tid; k < npt; k += nth) f += sc_pair_merit<0>(S, k, a);
@@ -3043,6 +3202,7 @@ __global__ void __launch_bounds__(kBlock) k_merit(mfem_solver S, float dt, float
   if (tid < S.n_verts) f += vertex_merit(S, tid, a, dt);
   if (tid < S.n_tets) f += tet_merit(S, tid, a);
   if (tid < S.n_tris && tri_contact_on(S)) f += tri_merit(S, tid, a);
+  if (tid < S.n_tris && shell_on(S)) f += shell_merit(S, tid, a);
   f = wave_sum(f);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = f;
   __syncthreads();
@@ -3147,6 +3307,10 @@ int check_solver(const mfem_solver* S) {
         (S->n_sc_points > 0 && !S->sc_points) || (S->n_sc_edges > 0 && !S->sc_edges) || (S->n_tris > 0 && !S->tris))
       return fail(MFEM_EINVAL, "self-contact enabled without its buffers");
   }
+  if (S->shell_on && S->n_tris > 0 &&
+      (!S->tris || !S->tcon_ptr || !S->tcon_src || !S->vtri_ptr || !S->vtri_src || !S->rest || !S->tri_W ||
+       !S->tri_area || !S->shell_h || !S->gS))
+    return fail(MFEM_EINVAL, "membrane shell enabled without its buffers");
   if (S->tri_contact && S->n_tris > 0 &&
       (!S->tris || !S->tcon_ptr || !S->tcon_src || !S->vtri_ptr || !S->vtri_src || !S->tri_dist || !S->tri_bary ||
        !S->tri_nrm || !S->tri_sid))
@@ -3172,7 +3336,7 @@ int launched() {
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
   X(lag_sid) X(G) X(Hl) X(gl) X(Hsinv) X(gs) X(psi) X(Kb) X(fe) X(gx) X(gB) X(gD) X(Hv) X(dist) X(eB)      \
-  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega) X(dx_hist) X(dx_same) X(dx_same_iters) X(coarse_w) X(coarse_omega) X(cw_x) X(cw_y) X(cw_w) X(cw_wbf) X(self_contact) X(n_sc_points) X(n_sc_edges) X(sc_max_pairs) X(sc_max_span) X(sc_dhat) X(sc_kappa) X(sc_cell) X(sc_ccd_s) X(sc_points) X(sc_edges) X(sc_pt) X(sc_ee) X(sc_count) X(sc_g) X(sc_h) X(sc_alpha) X(sc_ws) X(sc_ws_bytes)
+  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega) X(dx_hist) X(dx_same) X(dx_same_iters) X(coarse_w) X(coarse_omega) X(cw_x) X(cw_y) X(cw_w) X(cw_wbf) X(self_contact) X(n_sc_points) X(n_sc_edges) X(sc_max_pairs) X(sc_max_span) X(sc_dhat) X(sc_kappa) X(sc_cell) X(sc_ccd_s) X(sc_points) X(sc_edges) X(sc_pt) X(sc_ee) X(sc_count) X(sc_g) X(sc_h) X(sc_alpha) X(sc_ws) X(sc_ws_bytes) X(shell_on) X(shell_mu) X(rest) X(tri_W) X(tri_area) X(shell_h) X(gS)
 
 extern "C" {
 
@@ -3229,6 +3393,21 @@ int mfem_update_system(const mfem_solver* S, float dt, void* stream) {
     if (int rc = mfem_self_contact_eval(S, stream))
       return fail(rc, std::string("self-contact: ") + mfem_self_contact_last_error());
   hipLaunchKernelGGL(k_element, dim3(blocks_for(S->n_tets)), dim3(kBlock), 0, st, *S);
+  return launched();
+}
+
+int mfem_shell_refresh(const mfem_solver* S, void* stream) {
+  // (needs the topology and the shell's buffers only: state need not be bound yet)
+  if (!S) return fail(MFEM_EINVAL, "null solver");
+  if (!S->shell_on || S->n_tris == 0) return MFEM_OK;
+  if (S->n_tris < 0 || S->n_slots <= 0) return fail(MFEM_EINVAL, "bad sizes");
+  if (!(S->shell_mu > 0.f)) return fail(MFEM_EINVAL, "shell_mu must be > 0 with the shell on");
+  if (!S->tris || !S->tcon_ptr || !S->tcon_src || !S->rest || !S->tri_W || !S->tri_area || !S->shell_h)
+    return fail(MFEM_EINVAL, "membrane shell enabled without its buffers");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_shell_frames, dim3(blocks_for(S->n_tris)), dim3(kBlock), 0, st, *S);
+  if (int rc = launched()) return rc;
+  hipLaunchKernelGGL(k_shell_hess, dim3(blocks_for((long)S->n_slots * 64)), dim3(kBlock), 0, st, *S);
   return launched();
 }
 

@@ -37,7 +37,9 @@ FIELDS = [
     "self_contact", "n_sc_points", "n_sc_edges", "sc_max_pairs", "sc_max_span",
 )] + [(n, c_f32) for n in ("sc_dhat", "sc_kappa", "sc_cell", "sc_ccd_s")] + [(n, c_ptr) for n in (
     "sc_points", "sc_edges", "sc_pt", "sc_ee", "sc_count", "sc_g", "sc_h", "sc_alpha", "sc_ws",
-)] + [("sc_ws_bytes", ctypes.c_int64)]
+)] + [("sc_ws_bytes", ctypes.c_int64), ("shell_on", c_i32), ("shell_mu", c_f32)] + [(n, c_ptr) for n in (
+    "rest", "tri_W", "tri_area", "shell_h", "gS",
+)]
 
 
 class MfemSolverStruct(ctypes.Structure):
@@ -68,7 +70,7 @@ ENTRY_POINTS = [
     "mfem_pattern_last_error", "mfem_pose_shapes", "mfem_scan_i32_noalloc", "mfem_coarse_features",
     "mfem_coarse_weights", "mfem_broadphase_swept", "mfem_self_contact_workspace_bytes",
     "mfem_self_contact_candidates", "mfem_self_contact_eval", "mfem_self_contact_ccd",
-    "mfem_self_contact_last_error", "mfem_last_pcg_variant",
+    "mfem_self_contact_last_error", "mfem_last_pcg_variant", "mfem_shell_refresh",
 ]
 
 

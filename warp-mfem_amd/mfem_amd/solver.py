@@ -94,8 +94,9 @@ class RefinementSolver:
         self._refine_scoring = kwargs.get("refine_scoring", "legacy")
         if self._refine_scoring not in ("legacy", "geometric"):
             raise ValueError(f"Unknown refine_scoring {self._refine_scoring!r} (expected 'legacy' or 'geometric')")
-        if kwargs.get("shell_mu", 0.0) > 0.0:
-            raise NotImplementedError("membrane shell energy (shell_mu > 0) is not implemented by mfem_amd")
+        # membrane shell over the surface triangles (solver.py:291-302): a
+        # surface shear modulus in Pa m, 0 disables
+        self._shell_mu = float(kwargs.get("shell_mu", 0.0))
         energy = kwargs.get("energy", "arap")
         if energy not in _ENERGIES:
             raise ValueError(f"Unknown energy type: {energy}")
@@ -240,6 +241,11 @@ class RefinementSolver:
         b["ws_aux"] = zf(Vc, 10)
         b["ws_y"] = zf(Vc, 3)
         b["guess"] = zf(Vc, 3)
+        if self._shell_mu > 0.0:
+            # shell rest frames, areas and gradient (shell_energy.py:248-259)
+            b["tri_W"] = zf(max(Fc, 1), 4)
+            b["tri_area"] = zf(max(Fc, 1))
+            b["gS"] = zf(Vc, 3)
         self.buffers = b
         # attachments: the active particles with inv_mass = 0, fixed at
         # construction (padding rows and vertices added by refinement are
@@ -303,6 +309,7 @@ class RefinementSolver:
         S.eps_v = float(kwargs.get("friction_eps_v", 1.0e-3))
         S.qb[:] = qb
         S.gravity[:] = [float(g) for g in np.asarray(model.gravity).reshape(3)]
+        S.shell_mu = self._shell_mu
         for name, t in b.items():
             if hasattr(S, name):
                 setattr(S, name, t.data_ptr())
@@ -348,12 +355,19 @@ class RefinementSolver:
         b["dx_same"].zero_()
         S.max_width = int(np.diff(pat.chunk_ptr).max())
         S.tri_contact = int(self._tri_contact_kw and n_tris > 0)
+        S.shell_on = int(self._shell_mu > 0.0 and n_tris > 0)
         S.pcg_variant = -1  # block counts change with the topology: back to the automatic choice
         for name in ("chunk_ptr", "sell_col", "sell_val", "diag_lane", "contrib_ptr", "contrib_src", "vinc_ptr",
                      "vinc_src", "pz", "px", "tcon_ptr", "tcon_src", "vtri_ptr", "vtri_src"):
             setattr(S, name, b[name].data_ptr())
         if S.self_contact:
             self._set_self_contact_surface(int(n_tris))
+        if S.shell_on:
+            # Shell.refresh (shell_energy.py:262-293): rest frames and the
+            # constant Gauss-Newton Hessian of the current triangulation
+            b["shell_h"] = torch.zeros(pat.n_slots * 64, dtype=torch.float32, device=dev)
+            S.shell_h = b["shell_h"].data_ptr()
+            self._call("mfem_shell_refresh", self._stream())
 
     # -- self-contact (no counterpart in the reference's live solve) ----------
     def _init_self_contact(self, kwargs: dict, tris: np.ndarray, Vc: int, Fc: int, d1: float) -> None:
