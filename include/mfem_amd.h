@@ -611,6 +611,30 @@ int mfem_spmv(const mfem_solver* S, const float* x, float* y, void* stream);
    (solver.py:803-869) into out[0] (device double). */
 int mfem_merit(const mfem_solver* S, float dt, float alpha, double* out, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Point-to-surface queries of the surface-tracking / correspondence losses
+ * (src/mfem/refinement/surface_loss.py:202-245, 456-502), FP64.
+ * ------------------------------------------------------------------------ */
+/* The k (1..8) nearest of the n_cand vertices verts[vmap[j]] (vmap null:
+   verts[0..n_cand)) to every query point, ascending by squared distance,
+   ties to the lower index: out_idx / out_d2 [n_query][k] (index -1 and
+   DBL_MAX when fewer than k candidates).  Replaces the cKDTree queries at
+   surface_loss.py:214-220 and :486-490. */
+int mfem_surface_knn(const double* query, int n_query, const double* verts, const int32_t* vmap, int n_cand, int k,
+                     int32_t* out_idx, double* out_d2, void* stream);
+/* Squared distance of every query point to the triangle mesh (verts, faces)
+   over the triangles incident to its k nearest vertices (vert_faces:
+   [n_verts][deg] padded with -1; face_ok: [n_faces] 0/1 or null), the
+   smallest in neighbour-then-incidence order; without a usable triangle,
+   the nearest vertex's squared distance and out_face = -1.  out_face and
+   out_cp ([n_query][3] closest points) may be null.  Replaces the face
+   projection at surface_loss.py:222-245 and :491-502. */
+int mfem_surface_project(const double* query, int n_query, const double* verts, const int32_t* faces,
+                         const int32_t* vert_faces, int deg, const uint8_t* face_ok, const int32_t* knn_idx,
+                         const double* knn_d2, int k, double* out_sq, int32_t* out_face, double* out_cp,
+                         void* stream);
+const char* mfem_surface_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "mfem_amd", "libmfem_amd.so")
 SOURCES = ["mfem_kernels.hip", "mfem_mlp.hip", "mfem_collide.hip", "mfem_refine.hip", "mfem_pattern.hip",
-           "mfem_selfcontact.hip"]
+           "mfem_selfcontact.hip", "mfem_surface.hip"]
 ARCH = os.environ.get("MFEM_AMD_ARCH", "gfx950")
 
 

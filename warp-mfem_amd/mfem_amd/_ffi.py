@@ -71,6 +71,7 @@ ENTRY_POINTS = [
     "mfem_coarse_weights", "mfem_broadphase_swept", "mfem_self_contact_workspace_bytes",
     "mfem_self_contact_candidates", "mfem_self_contact_eval", "mfem_self_contact_ccd",
     "mfem_self_contact_last_error", "mfem_last_pcg_variant", "mfem_shell_refresh",
+    "mfem_surface_knn", "mfem_surface_project", "mfem_surface_last_error",
 ]
 
 
@@ -154,6 +155,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.mfem_self_contact_candidates.argtypes = [P, i, vp]
     lib.mfem_self_contact_last_error.restype = ctypes.c_char_p
     lib.mfem_last_pcg_variant.argtypes = []
+    lib.mfem_shell_refresh.argtypes = [P, vp]
+    lib.mfem_surface_knn.argtypes = [vp, i, vp, vp, i, i, vp, vp, vp]
+    lib.mfem_surface_project.argtypes = [vp, i, vp, vp, vp, i, vp, vp, vp, i, vp, vp, vp, vp]
+    lib.mfem_surface_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 
