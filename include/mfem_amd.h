@@ -290,6 +290,14 @@ typedef struct mfem_solver {
   float* tri_area;           /* [n_tris] rest area (0 for a degenerate rest triangle) */
   float* shell_h;            /* [n_slots*64] per lane-slot sum of mu_s A w_a . w_b (block = value I3) */
   float* gS;                 /* [n_verts][3] shell gradient */
+  /* 1: the element kernel stores only what assembly needs (K blocks,
+     forces, energies) and the line search's first pass runs the local solve
+     (lambda, ds) from the element recomputed at the same positions,
+     forming its merit and slope terms there; mfem_local_solve is then a
+     no-op.  Needs line_search.  0: G, H_l, g_l, H_s^-1, g_s are stored by
+     mfem_update_system and read by mfem_local_solve and the line search
+     (the reference's stage order, solver.py:533-557). */
+  int32_t fused_local;
 } mfem_solver;
 
 /* ABI self-description (checked against the Python mirror by the tests) */

@@ -81,7 +81,7 @@ def _sync_oracle(o, solver, state):
 @pytest.mark.parametrize("name", ["beam_arap", "beam_nh", "drop_box", "capsule_poke"])
 def test_newton_stages_match_oracle(gpu, name):
     m, dt = _perturbed_scene(name)
-    solver, state = _make(m, gpu)
+    solver, state = _make(m, gpu, fuse_local_solve=False)  # the stage order: records, then the local solve
     rng = np.random.default_rng(1)
     # a deformed, moving configuration: perturb positions and velocities
     q = m.particle_q + 0.004 * rng.normal(size=m.particle_q.shape)

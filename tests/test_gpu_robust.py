@@ -53,7 +53,7 @@ def test_inverted_and_large_strain_tets_match_oracle(gpu):
     GMW81 corrected branch) are both present; G, H_s^-1, g_s and psi of every
     well-conditioned tet match the oracle."""
     m = scene.beam_scene(cells=(6, 3, 3), energy="neohookean")
-    solver, state = _make(m, gpu)
+    solver, state = _make(m, gpu, fuse_local_solve=False)  # (stores the element records inspected below)
     rng = np.random.default_rng(11)
     h = 0.2 / 3
     q = m.particle_q + 0.45 * h * rng.normal(size=m.particle_q.shape)

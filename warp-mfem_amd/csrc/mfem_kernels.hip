@@ -11,7 +11,8 @@
 //   k_rhs          per vertex   gather of tet forces, diagonal mass/contact terms
 //   k_pcg          persistent, cooperative: whole block-Jacobi PCG loop with
 //                               FP64 reductions and register-resident row state
-//   k_local        per tet      lambda, ds
+//   k_local        per tet      lambda, ds (stage path; with fused_local the
+//                               line search's first pass does it)
 //   k_linesearch   persistent, cooperative: whole backtracking loop, FP64 merit
 //   k_end_step     per vertex   velocity
 //   (self-contact, mfem_selfcontact.hip: candidates, pair barrier, CCD)
@@ -579,11 +580,59 @@ __global__ void __launch_bounds__(kBlock) k_init_stretch(mfem_solver S) {
   for (int k = 0; k < 6; ++k) S.s[6 * e + k] = S6[k];
 }
 
+// Per-tet element quantities at the current positions (kernels.py:131-249,
+// elastic_energy.py): constraint c = SYM (sym6(S) - s), its Jacobian
+// G = -SYM dS/dF dF/dx (6x12), sym6(S), the elastic energy, gradient and
+// PSD-fixed Hessian and its inverse in stretch space, and the Schur blocks
+// H_l = INV_SYM H INV_SYM, g_l = c + SYM H^-1 g_s.
+struct TetElement {
+  float G[72], c[6], S6[6], gs[6], Hi[36], Hl[36], gl[6], psi;
+};
+
+__device__ __forceinline__ void tet_element(const mfem_solver& S, int e, const float (&xv)[4][3], const float* D,
+                                            const float* s, TetElement& o) {
+  const float mu = S.materials[3 * e], la = S.materials[3 * e + 1], vol = S.vol[e];
+  float F[9], U[9], sg[3], V[9], dS[54];
+  deformation_gradient(xv[0], xv[1], xv[2], xv[3], D, F);
+  svd3(F, U, sg, V);
+  stretch_sym6(sg, V, o.S6);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) o.c[k] = SYMD[k] * (o.S6[k] - s[k]);
+  stretch_gradient(F, U, sg, V, dS);
+  constraint_jacobian(dS, D, o.G);
+  float H[36];
+  if ((S.tet_energy ? S.tet_energy[e] : S.energy) == 0) {
+    o.psi = arap_energy(s, mu, vol);
+    arap_grad_hess(s, mu, vol, o.gs, H, o.Hi);
+  } else {
+    o.psi = nh_energy(s, mu, la, vol);
+    nh_grad_hess(s, mu, la, vol, o.gs, H, o.Hi);
+  }
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = 0; b < 6; ++b) o.Hl[a * 6 + b] = H[a * 6 + b] / (SYMD[a] * SYMD[b]);
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    float acc = 0.f;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) acc += o.Hi[a * 6 + b] * o.gs[b];
+    o.gl[a] = o.c[a] + SYMD[a] * acc;
+  }
+}
+
+// Element kernel.  FULL: also stores G, H_l, g_l, H_s^-1, g_s for the stage
+// local solve (k_local) and the line search's slope; otherwise only what
+// assembly needs (the 10 unique blocks of K = G^T H_l G, the force
+// G^T H_l g_l) and the energy, the local solve then running inside the line
+// search (k_linesearch, fused_local).
+template <bool FULL>
 __global__ void __launch_bounds__(kBlock, 2) k_element(mfem_solver S) {
   if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   // per-tet outputs leave through wave_store; lanes past the last tet compute
   // the last tet again and store nothing
-  __shared__ float slabs[kWaves][kSlabFloats];
+  constexpr int kSlab = FULL ? kSlabFloats : 64 * 13;
+  __shared__ float slabs[kWaves][kSlab];
   const int lane = threadIdx.x & 63;
   const int e0 = blockIdx.x * kBlock + (threadIdx.x & ~63);
   const int nvalid = min(64, S.n_tets - e0);
@@ -598,65 +647,33 @@ __global__ void __launch_bounds__(kBlock, 2) k_element(mfem_solver S) {
   for (int q = 0; q < 9; ++q) D[q] = S.dminv[9 * e + q];
 #pragma unroll
   for (int q = 0; q < 6; ++q) s[q] = S.s[6 * e + q];
-  const float mu = S.materials[3 * e], la = S.materials[3 * e + 1], vol = S.vol[e];
-
-  // kinematics: c = SYM (sym6(S) - s), G = -SYM dS/dF dF/dx
-  float F[9], U[9], sg[3], V[9], S6[6], dS[54], G[72];
-  deformation_gradient(xv[0], xv[1], xv[2], xv[3], D, F);
-  svd3(F, U, sg, V);
-  stretch_sym6(sg, V, S6);
-  float c[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) c[k] = SYMD[k] * (S6[k] - s[k]);
-  stretch_gradient(F, U, sg, V, dS);
-  constraint_jacobian(dS, D, G);
-
-  // elastic energy, gradient, (PSD) Hessian and inverse in stretch space
-  float gs[6], H[36], Hi[36], psi;
-  if ((S.tet_energy ? S.tet_energy[e] : S.energy) == 0) {
-    psi = arap_energy(s, mu, vol);
-    arap_grad_hess(s, mu, vol, gs, H, Hi);
-  } else {
-    psi = nh_energy(s, mu, la, vol);
-    nh_grad_hess(s, mu, la, vol, gs, H, Hi);
-  }
-  // Schur blocks: H_l = INV_SYM H INV_SYM, g_l = c + SYM H^-1 g_s
-  float Hl[36], gl[6], hg[6];
-#pragma unroll
-  for (int a = 0; a < 6; ++a)
-#pragma unroll
-    for (int b = 0; b < 6; ++b) Hl[a * 6 + b] = H[a * 6 + b] / (SYMD[a] * SYMD[b]);
+  TetElement el;
+  tet_element(S, e, xv, D, s, el);
+  float hg[6];
 #pragma unroll
   for (int a = 0; a < 6; ++a) {
     float acc = 0.f;
 #pragma unroll
-    for (int b = 0; b < 6; ++b) acc += Hi[a * 6 + b] * gs[b];
-    gl[a] = c[a] + SYMD[a] * acc;
-  }
-#pragma unroll
-  for (int a = 0; a < 6; ++a) {
-    float acc = 0.f;
-#pragma unroll
-    for (int b = 0; b < 6; ++b) acc += Hl[a * 6 + b] * gl[b];
+    for (int b = 0; b < 6; ++b) acc += el.Hl[a * 6 + b] * el.gl[b];
     hg[a] = acc;
   }
-  // stores needed by the local solve and the line search
-  wave_store<72>(slab, S.G + (size_t)72 * e0, G, nvalid, lane);
-  {
+  if constexpr (FULL) {
+    // stores needed by the stage local solve and the line search
+    wave_store<72>(slab, S.G + (size_t)72 * e0, el.G, nvalid, lane);
     float h21[21], i21[21];
 #pragma unroll
     for (int i = 0; i < 6; ++i)
 #pragma unroll
       for (int j = i; j < 6; ++j) {
-        h21[sym21(i, j)] = Hl[i * 6 + j];
-        i21[sym21(i, j)] = Hi[i * 6 + j];
+        h21[sym21(i, j)] = el.Hl[i * 6 + j];
+        i21[sym21(i, j)] = el.Hi[i * 6 + j];
       }
     wave_store<21>(slab, S.Hl + (size_t)21 * e0, h21, nvalid, lane);
     wave_store<21>(slab, S.Hsinv + (size_t)21 * e0, i21, nvalid, lane);
+    wave_store<6>(slab, S.gl + (size_t)6 * e0, el.gl, nvalid, lane);
+    wave_store<6>(slab, S.gs + (size_t)6 * e0, el.gs, nvalid, lane);
   }
-  wave_store<6>(slab, S.gl + (size_t)6 * e0, gl, nvalid, lane);
-  wave_store<6>(slab, S.gs + (size_t)6 * e0, gs, nvalid, lane);
-  if (lane < nvalid) S.psi[e] = psi;
+  if (lane < nvalid) S.psi[e] = el.psi;
   // HG = H_l G  (6x12)
   float HG[72];
 #pragma unroll
@@ -665,7 +682,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_element(mfem_solver S) {
     for (int k = 0; k < 12; ++k) {
       float acc = 0.f;
 #pragma unroll
-      for (int b = 0; b < 6; ++b) acc += Hl[a * 6 + b] * G[b * 12 + k];
+      for (int b = 0; b < 6; ++b) acc += el.Hl[a * 6 + b] * el.G[b * 12 + k];
       HG[a * 12 + k] = acc;
     }
   // force f = G^T H_l g_l and the 10 unique blocks of K = G^T H_l G
@@ -675,7 +692,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_element(mfem_solver S) {
     for (int k = 0; k < 12; ++k) {
       float acc = 0.f;
 #pragma unroll
-      for (int a = 0; a < 6; ++a) acc += G[a * 12 + k] * hg[a];
+      for (int a = 0; a < 6; ++a) acc += el.G[a * 12 + k] * hg[a];
       fe[k] = acc;
     }
     wave_store<12>(slab, S.fe + (size_t)12 * e0, fe, nvalid, lane);
@@ -690,7 +707,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_element(mfem_solver S) {
       for (int cc = 0; cc < 3; ++cc) {
         float acc = 0.f;
 #pragma unroll
-        for (int a = 0; a < 6; ++a) acc += G[a * 12 + 3 * PA[p] + r] * HG[a * 12 + 3 * PB[p] + cc];
+        for (int a = 0; a < 6; ++a) acc += el.G[a * 12 + 3 * PA[p] + r] * HG[a * 12 + 3 * PB[p] + cc];
         kb[3 * r + cc] = acc;
       }
     wave_store<9>(slab, S.Kb + ((size_t)p * T + e0) * 9, kb, nvalid, lane);
@@ -3010,6 +3027,108 @@ __device__ double tet_slope(const mfem_solver& S, int e) {
   return acc;
 }
 
+// Fused local solve (fused_local): tet e's element at the current x (the
+// records k_element<true> would store), its local solve
+// lambda = -H_l (g_l - G dx_e), ds = -H_s^-1 (SYM lambda + g_s)
+// (kernels.py:252-283; stored into lam and ds), and its line-search terms
+// with the new multipliers: the merit at alpha = 0 and at a1, and the slope
+// (g_s + SYM lambda) . ds - (G^T lambda) . dx_e -- each formed as tet_merit,
+// tet_slope and tet_lambda_slope form them from the stored records.
+__device__ void tet_local_merit(const mfem_solver& S, int e, float a1, double& f0, double& sl, double& fa) {
+  float xv[4][3];
+  int4 tv;
+  gather_tet(S, e, xv, tv);
+  const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+  float D[9], s[6];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) D[q] = S.dminv[9 * e + q];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) s[q] = S.s[6 * e + q];
+  float lam[6], ds[6], S6[6], psi;
+  double slope;
+  {
+    TetElement el;
+    tet_element(S, e, xv, D, s, el);
+    float d[12];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) d[3 * a + k] = (float)S.dx[3 * vid[a] + k];
+    float t[6], w[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      float acc = el.gl[r];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) acc -= el.G[r * 12 + k] * d[k];
+      t[r] = acc;
+    }
+    // (the symmetric blocks are read from their upper triangle, as the
+    // packed records are)
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      float acc = 0.f;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc += el.Hl[r < c ? r * 6 + c : c * 6 + r] * t[c];
+      lam[r] = -acc;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) w[c] = SYMD[c] * lam[c] + el.gs[c];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      float acc = 0.f;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc += el.Hi[r < c ? r * 6 + c : c * 6 + r] * w[c];
+      ds[r] = -acc;
+    }
+    slope = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) slope += (double)(el.gs[k] + SYMD[k] * lam[k]) * (double)ds[k];
+    double ls = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        float tt = 0.f;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) tt += el.G[q * 12 + 3 * a + r] * lam[q];
+        ls -= (double)tt * S.dx[3 * vid[a] + r];
+      }
+    slope += ls;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S6[k] = el.S6[k];
+    psi = el.psi;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    S.lam[6 * e + k] = lam[k];
+    S.ds[6 * e + k] = ds[k];
+  }
+  double lc = 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) lc += (double)(SYMD[k] * (S6[k] - s[k])) * (double)lam[k];
+  f0 += (double)psi - lc;
+  sl += slope;
+  // merit at a1 (tet_merit with the new lambda and ds)
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) xv[v][k] = xv[v][k] + a1 * (float)S.dx[3 * vid[v] + k];
+  float s1[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) s1[k] = s[k] + a1 * ds[k];
+  const float mu = S.materials[3 * e], la = S.materials[3 * e + 1], vol = S.vol[e];
+  const float psi1 = (S.tet_energy ? S.tet_energy[e] : S.energy) == 0 ? arap_energy(s1, mu, vol)
+                                                                      : nh_energy(s1, mu, la, vol);
+  float F[9], U[9], sg[3], V[9], S61[6];
+  deformation_gradient(xv[0], xv[1], xv[2], xv[3], D, F);
+  svd3(F, U, sg, V);
+  stretch_sym6(sg, V, S61);
+  double lc1 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) lc1 += (double)(SYMD[k] * (S61[k] - s1[k])) * (double)lam[k];
+  fa += (double)psi1 - lc1;
+}
+
 __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, int newton_iter) {
   if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // see k_pcg
   // a batch: the body's workgroups search over the body's vertices, tets
@@ -3038,10 +3157,15 @@ __global__ void __launch_bounds__(kBlock) k_linesearch(mfem_solver S, float dt, 
     sl += vertex_slope_own(S, i);
     fa += vertex_merit(S, i, a_first, dt);
   }
-  for (int e = e0 + tid; e < e1; e += nth) {
-    f += tet_merit(S, e, 0.f);
-    sl += tet_slope(S, e) + tet_lambda_slope(S, e);
-    fa += tet_merit(S, e, a_first);
+  if (S.fused_local) {
+    // the local solve runs here, tet by tet (k_element<false> stored no records)
+    for (int e = e0 + tid; e < e1; e += nth) tet_local_merit(S, e, a_first, f, sl, fa);
+  } else {
+    for (int e = e0 + tid; e < e1; e += nth) {
+      f += tet_merit(S, e, 0.f);
+      sl += tet_slope(S, e) + tet_lambda_slope(S, e);
+      fa += tet_merit(S, e, a_first);
+    }
   }
   if (tri_contact_on(S))
     for (int t = f0t + tid; t < f1t; t += nth) {
@@ -3307,6 +3431,8 @@ int check_solver(const mfem_solver* S) {
         (S->n_sc_points > 0 && !S->sc_points) || (S->n_sc_edges > 0 && !S->sc_edges) || (S->n_tris > 0 && !S->tris))
       return fail(MFEM_EINVAL, "self-contact enabled without its buffers");
   }
+  if (S->fused_local && !S->line_search)
+    return fail(MFEM_EINVAL, "fused_local needs the line search (its first pass runs the local solve)");
   if (S->shell_on && S->n_tris > 0 &&
       (!S->tris || !S->tcon_ptr || !S->tcon_src || !S->vtri_ptr || !S->vtri_src || !S->rest || !S->tri_W ||
        !S->tri_area || !S->shell_h || !S->gS))
@@ -3336,7 +3462,7 @@ int launched() {
   X(shape_scale) X(shape_mu) X(chunk_ptr) X(sell_col) X(sell_val) X(diag_lane) X(contrib_ptr)             \
   X(contrib_src) X(vinc_ptr) X(vinc_src) X(x) X(v) X(s) X(lam) X(x_tilde) X(x_start) X(lag_n) X(lag_fn)    \
   X(lag_sid) X(G) X(Hl) X(gl) X(Hsinv) X(gs) X(psi) X(Kb) X(fe) X(gx) X(gB) X(gD) X(Hv) X(dist) X(eB)      \
-  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega) X(dx_hist) X(dx_same) X(dx_same_iters) X(coarse_w) X(coarse_omega) X(cw_x) X(cw_y) X(cw_w) X(cw_wbf) X(self_contact) X(n_sc_points) X(n_sc_edges) X(sc_max_pairs) X(sc_max_span) X(sc_dhat) X(sc_kappa) X(sc_cell) X(sc_ccd_s) X(sc_points) X(sc_edges) X(sc_pt) X(sc_ee) X(sc_count) X(sc_g) X(sc_h) X(sc_alpha) X(sc_ws) X(sc_ws_bytes) X(shell_on) X(shell_mu) X(rest) X(tri_W) X(tri_area) X(shell_h) X(gS)
+  X(eD) X(g) X(dx) X(ds) X(pz) X(px) X(red) X(flags) X(status) X(stats) X(mlp_w) X(mlp_wbf) X(ws_x) X(ws_aux) X(ws_y) X(guess) X(tcon_ptr) X(tcon_src) X(vtri_ptr) X(vtri_src) X(tri_dist) X(tri_bary) X(tri_nrm) X(tri_sid) X(n_bodies) X(ls_blocks) X(body_wg) X(body_vert) X(body_tet) X(body_tri) X(body_ls_wg) X(body_epoch) X(tet_energy) X(body_wg_align) X(local_sweeps) X(local_omega) X(dx_hist) X(dx_same) X(dx_same_iters) X(coarse_w) X(coarse_omega) X(cw_x) X(cw_y) X(cw_w) X(cw_wbf) X(self_contact) X(n_sc_points) X(n_sc_edges) X(sc_max_pairs) X(sc_max_span) X(sc_dhat) X(sc_kappa) X(sc_cell) X(sc_ccd_s) X(sc_points) X(sc_edges) X(sc_pt) X(sc_ee) X(sc_count) X(sc_g) X(sc_h) X(sc_alpha) X(sc_ws) X(sc_ws_bytes) X(shell_on) X(shell_mu) X(rest) X(tri_W) X(tri_area) X(shell_h) X(gS) X(fused_local)
 
 extern "C" {
 
@@ -3392,7 +3518,10 @@ int mfem_update_system(const mfem_solver* S, float dt, void* stream) {
   if (S->self_contact)
     if (int rc = mfem_self_contact_eval(S, stream))
       return fail(rc, std::string("self-contact: ") + mfem_self_contact_last_error());
-  hipLaunchKernelGGL(k_element, dim3(blocks_for(S->n_tets)), dim3(kBlock), 0, st, *S);
+  if (S->fused_local)
+    hipLaunchKernelGGL(k_element<false>, dim3(blocks_for(S->n_tets)), dim3(kBlock), 0, st, *S);
+  else
+    hipLaunchKernelGGL(k_element<true>, dim3(blocks_for(S->n_tets)), dim3(kBlock), 0, st, *S);
   return launched();
 }
 
@@ -3473,6 +3602,7 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
 
 int mfem_local_solve(const mfem_solver* S, void* stream) {
   if (int rc = check_solver(S)) return rc;
+  if (S->fused_local) return MFEM_OK;  // runs inside mfem_line_search
   hipLaunchKernelGGL(k_local, dim3(blocks_for(S->n_tets)), dim3(kBlock), 0, (hipStream_t)stream, *S);
   return launched();
 }

@@ -39,7 +39,7 @@ FIELDS = [
     "sc_points", "sc_edges", "sc_pt", "sc_ee", "sc_count", "sc_g", "sc_h", "sc_alpha", "sc_ws",
 )] + [("sc_ws_bytes", ctypes.c_int64), ("shell_on", c_i32), ("shell_mu", c_f32)] + [(n, c_ptr) for n in (
     "rest", "tri_W", "tri_area", "shell_h", "gS",
-)]
+)] + [("fused_local", c_i32)]
 
 
 class MfemSolverStruct(ctypes.Structure):

@@ -292,9 +292,15 @@ class RefinementSolver:
         S.cg_maxiter = int(kwargs.get("cg_max_iterations", 1000))
         S.precond = int(bool(kwargs.get("preconditioner", True)))
         S.line_search = int(bool(kwargs.get("line_search", True)))
+        # the local solve inside the line search's first pass (no counterpart
+        # in the reference, whose stages store and re-read the element
+        # records): on unless asked off, and only with the line search
+        S.fused_local = int(bool(kwargs.get("fuse_local_solve", True)) and S.line_search)
         S.ls_max = int(kwargs.get("line_search_max_iterations", 20))
-        # Contact(tri_contact=...) keeps it only with triangles (contact.py:844)
-        S.tri_contact = int(tri_contact and len(tris) > 0)
+        # Contact(tri_contact=...) keeps it only with triangles (contact.py:844);
+        # it measures triangles against capsules, so without one it is skipped
+        self._has_capsule = any(int(sh.type) == _ffi.SHAPE_CAPSULE for sh in shapes)
+        S.tri_contact = int(tri_contact and len(tris) > 0 and self._has_capsule)
         self._tri_contact_kw = tri_contact
         S.friction_on = int(n_shapes > 0 and float(mu.max(initial=0.0)) > 0.0)
         S.cg_tol = float(kwargs.get("cg_tolerance", 1.0e-6))
@@ -354,7 +360,7 @@ class RefinementSolver:
         b["dx_hist"].zero_()
         b["dx_same"].zero_()
         S.max_width = int(np.diff(pat.chunk_ptr).max())
-        S.tri_contact = int(self._tri_contact_kw and n_tris > 0)
+        S.tri_contact = int(self._tri_contact_kw and n_tris > 0 and self._has_capsule)
         S.shell_on = int(self._shell_mu > 0.0 and n_tris > 0)
         S.pcg_variant = -1  # block counts change with the topology: back to the automatic choice
         for name in ("chunk_ptr", "sell_col", "sell_val", "diag_lane", "contrib_ptr", "contrib_src", "vinc_ptr",
