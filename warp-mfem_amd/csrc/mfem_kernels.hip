@@ -7,8 +7,9 @@
 //   k_element      per tet      F, SVD, c, G, energy, PSD Hessian, Schur blocks,
 //                               12x12 stiffness (10 unique 3x3 blocks) and force
 //   k_assemble     per lane-slot of the SELL-64 matrix: deterministic gather of
-//                               the tet blocks that land on that block
-//   k_rhs          per vertex   gather of tet forces, diagonal mass/contact terms
+//                               the tet blocks that land on that block; the
+//                               right-hand side (gather of tet forces) and the
+//                               diagonal mass/contact terms of each row
 //   k_pcg          persistent, cooperative: whole block-Jacobi PCG loop with
 //                               FP64 reductions and register-resident row state
 //   k_local        per tet      lambda, ds (stage path; with fused_local the
@@ -83,7 +84,7 @@ static constexpr float kFp64Range = 2.0f;
 
 // FP64 signed distance of x to shape k (plane: local z; capsule: distance to
 // the axis segment minus the radius) (src/mfem/ipc/distance.py:4-57)
-__device__ double shape_distance64(const mfem_solver& S, int k, const float* x) {
+__device__ __forceinline__ double shape_distance64(const mfem_solver& S, int k, const float* x) {
   const float* xf = S.shape_xf + 7 * k;
   double R[9];
   quat_to_mat(xf + 3, R);
@@ -102,7 +103,7 @@ __device__ double shape_distance64(const mfem_solver& S, int k, const float* x) 
 // (src/mfem/refinement/contact.py:62-164, src/mfem/ipc/distance.py:4-57);
 // d64 (may be null) receives the FP64 distance of the nearest shape (the FP32
 // value when it is outside kFp64Range d1)
-__device__ float shape_distance(const mfem_solver& S, const float* x, float* grad, int* sid,
+__device__ __forceinline__ float shape_distance(const mfem_solver& S, const float* x, float* grad, int* sid,
                                 double* d64 = nullptr) {
   float dm = 1.0e8f;
   grad[0] = grad[1] = grad[2] = 0.f;
@@ -142,7 +143,7 @@ __device__ float shape_distance(const mfem_solver& S, const float* x, float* gra
 }
 
 // lagged IPC friction at position x (contact.py:304-387)
-__device__ float friction_eval(const mfem_solver& S, int i, const float* x, float dt, float* grad, float* hess) {
+__device__ __forceinline__ float friction_eval(const mfem_solver& S, int i, const float* x, float dt, float* grad, float* hess) {
   float lamn = S.lag_fn[i];
   int sid = S.lag_sid[i];
   if (grad) grad[0] = grad[1] = grad[2] = 0.f;
@@ -177,7 +178,7 @@ __device__ float friction_eval(const mfem_solver& S, int i, const float* x, floa
 // triangle; the face normal oriented away from the axis midpoint when the
 // axis touches or pierces the face) (contact.py:553-681), in precision T
 template <typename T>
-__device__ T tri_capsule_one(const mfem_solver& S, int k, const T* a, const T* b, const T* c, T* bary, T* nrm) {
+__device__ __forceinline__ T tri_capsule_one(const mfem_solver& S, int k, const T* a, const T* b, const T* c, T* bary, T* nrm) {
   const float* xf = S.shape_xf + 7 * k;
   T R[9];
   quat_to_mat(xf + 3, R);
@@ -222,7 +223,7 @@ __device__ T tri_capsule_one(const mfem_solver& S, int k, const T* a, const T* b
 // nearest capsule of triangle abc: ranked in FP32, the winner recomputed in
 // FP64 inside kFp64Range d1 (weights, normal, distance); d64 (may be null) =
 // that distance in FP64
-__device__ float tri_capsule(const mfem_solver& S, const float* a, const float* b, const float* c, float* bary,
+__device__ __forceinline__ float tri_capsule(const mfem_solver& S, const float* a, const float* b, const float* c, float* bary,
                              float* nrm, int* sid, double* d64 = nullptr) {
   float dmin = 1.0e8f;
   bary[0] = bary[1] = bary[2] = 1.f / 3.f;
@@ -732,7 +733,56 @@ __device__ __forceinline__ int xcd_block() {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
-__global__ void __launch_bounds__(kBlock) k_assemble(mfem_solver S) {
+// Right-hand side of vertex i (solver.py:737-797): g = sum of its tets'
+// forces G^T H_l g_l (in incidence order) - g_x - grad B - grad D
+// [- self-contact - shell]
+__device__ __forceinline__ void vertex_g(const mfem_solver& S, int i) {
+  float g[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) g[k] = -S.gx[3 * i + k];
+  const int k0 = S.vinc_ptr[i], k1 = S.vinc_ptr[i + 1];
+  for (int k = k0; k < k1; ++k) {
+    int src = S.vinc_src[k];
+    const float* f = S.fe + (size_t)(src >> 2) * 12 + 3 * (src & 3);
+    g[0] += f[0];
+    g[1] += f[1];
+    g[2] += f[2];
+  }
+  float gsc[3] = {0.f, 0.f, 0.f};
+  if (self_contact_on(S))
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gsc[k] = (float)S.sc_g[3 * i + k];
+  if (shell_on(S))
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gsc[k] += S.gS[3 * i + k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) S.g[3 * i + k] = g[k] - S.gB[3 * i + k] - S.gD[3 * i + k] - gsc[k];
+}
+
+// additions to vertex i's diagonal block: (M + Q) / dt^2 + H_B + H_D
+// [+ the self-contact block] (solver.py:737-797)
+__device__ __forceinline__ void vertex_diag(const mfem_solver& S, int i, float dt, float (&diag)[9]) {
+  float hsc[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (self_contact_on(S)) {
+    const double* h = S.sc_h + 6 * (size_t)i;
+    const float h6[6] = {(float)h[0], (float)h[1], (float)h[2], (float)h[3], (float)h[4], (float)h[5]};
+    hsc[0] = h6[0]; hsc[4] = h6[1]; hsc[8] = h6[2];
+    hsc[1] = hsc[3] = h6[3];
+    hsc[2] = hsc[6] = h6[4];
+    hsc[5] = hsc[7] = h6[5];
+  }
+  const float idt2 = 1.f / (dt * dt);
+  float im = S.inv_mass[i];
+  float md = (im == 0.f ? 1.f + S.attach_k : 1.f / im) * idt2;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) diag[q] = S.Hv[9 * i + q] + ((q % 4 == 0) ? md : 0.f) + hsc[q];
+}
+
+// Assembly of H into the SELL-64 pattern, one thread per lane-slot, and of
+// the right-hand side: the lanes of each chunk's first slot form g of their
+// rows, and each row's diagonal lane-slot adds its vertex terms before the
+// block is stored.
+__global__ void __launch_bounds__(kBlock) k_assemble(mfem_solver S, float dt) {
   if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
   int l = xcd_block() * kBlock + threadIdx.x;
   if (l >= S.n_slots * 64) return;
@@ -779,52 +829,26 @@ __global__ void __launch_bounds__(kBlock) k_assemble(mfem_solver S) {
     acc[4] += h;
     acc[8] += h;
   }
+  // the row of this lane-slot: chunk c holds slots [chunk_ptr[c], chunk_ptr[c + 1])
+  const int slot = l >> 6;
+  int lo = 0, hi = S.n_chunks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (S.chunk_ptr[mid] <= slot) lo = mid;
+    else hi = mid - 1;
+  }
+  const int row = lo * 64 + (l & 63);
+  if (row < S.n_verts) {
+    if (slot == S.chunk_ptr[lo]) vertex_g(S, row);
+    if (S.diag_lane[row] == l) {
+      float add[9];
+      vertex_diag(S, row, dt, add);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) acc[q] += add[q];
+    }
+  }
 #pragma unroll
   for (int q = 0; q < 9; ++q) S.sell_val[val_index(l, q)] = acc[q];
-}
-
-// g = (H_l G)^T g_l - g_x - grad B - grad D; diagonal += (M + Q)/dt^2 + H_B + H_D
-__global__ void __launch_bounds__(kBlock) k_rhs(mfem_solver S, float dt) {
-  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
-  int i = xcd_block() * kBlock + threadIdx.x;
-  if (i >= S.n_verts) return;
-  float g[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) g[k] = -S.gx[3 * i + k];
-  const int k0 = S.vinc_ptr[i], k1 = S.vinc_ptr[i + 1];
-  for (int k = k0; k < k1; ++k) {
-    int src = S.vinc_src[k];
-    const float* f = S.fe + (size_t)(src >> 2) * 12 + 3 * (src & 3);
-    g[0] += f[0];
-    g[1] += f[1];
-    g[2] += f[2];
-  }
-  float gsc[3] = {0.f, 0.f, 0.f}, hsc[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  if (self_contact_on(S)) {
-    // the self-contact barrier's gradient and diagonal block (FP64 sums)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) gsc[k] = (float)S.sc_g[3 * i + k];
-    const double* h = S.sc_h + 6 * (size_t)i;
-    const float h6[6] = {(float)h[0], (float)h[1], (float)h[2], (float)h[3], (float)h[4], (float)h[5]};
-    hsc[0] = h6[0]; hsc[4] = h6[1]; hsc[8] = h6[2];
-    hsc[1] = hsc[3] = h6[3];
-    hsc[2] = hsc[6] = h6[4];
-    hsc[5] = hsc[7] = h6[5];
-  }
-  if (shell_on(S))
-#pragma unroll
-    for (int k = 0; k < 3; ++k) gsc[k] += S.gS[3 * i + k];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) S.g[3 * i + k] = g[k] - S.gB[3 * i + k] - S.gD[3 * i + k] - gsc[k];
-  const float idt2 = 1.f / (dt * dt);
-  float im = S.inv_mass[i];
-  float md = (im == 0.f ? 1.f + S.attach_k : 1.f / im) * idt2;
-  int dl = S.diag_lane[i];
-#pragma unroll
-  for (int q = 0; q < 9; ++q) {
-    float add = S.Hv[9 * i + q] + ((q % 4 == 0) ? md : 0.f) + hsc[q];
-    S.sell_val[val_index(dl, q)] += add;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2900,7 +2924,7 @@ __global__ void __launch_bounds__(kBlock) k_local(mfem_solver S) {
 // line search (line_search.py:76-163; solver.py:803-928)
 // ---------------------------------------------------------------------------
 // kinetic / attachment / barrier / friction terms of vertex i at x + a dx
-__device__ double vertex_merit(const mfem_solver& S, int i, float a, float dt) {
+__device__ __forceinline__ double vertex_merit(const mfem_solver& S, int i, float a, float dt) {
   float x[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) x[k] = S.x[3 * i + k] + a * (float)S.dx[3 * i + k];
@@ -2937,7 +2961,7 @@ __device__ double vertex_merit(const mfem_solver& S, int i, float a, float dt) {
 }
 
 // Psi(s + a ds) - lambda . c(x + a dx, s + a ds)
-__device__ double tet_merit(const mfem_solver& S, int e, float a) {
+__device__ __forceinline__ double tet_merit(const mfem_solver& S, int e, float a) {
   int4 tv = reinterpret_cast<const int4*>(S.tets)[e];
   const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
   float xv[4][3];
@@ -2964,7 +2988,7 @@ __device__ double tet_merit(const mfem_solver& S, int e, float a) {
 
 // triangle-capsule barrier energy at x + a dx, weighted by the barycentric
 // weights of the last full evaluation (contact.py:1050-1088)
-__device__ double tri_merit(const mfem_solver& S, int t, float a) {
+__device__ __forceinline__ double tri_merit(const mfem_solver& S, int t, float a) {
   float v[3][3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -2984,7 +3008,7 @@ __device__ double tri_merit(const mfem_solver& S, int t, float a) {
 }
 
 // (g_x + grad B + grad D) . dx for vertex i: the vertex part of the slope
-__device__ double vertex_slope_own(const mfem_solver& S, int i) {
+__device__ __forceinline__ double vertex_slope_own(const mfem_solver& S, int i) {
   double acc = 0.0;
 #pragma unroll
   for (int k = 0; k < 3; ++k)
@@ -3000,7 +3024,7 @@ __device__ double vertex_slope_own(const mfem_solver& S, int i) {
 
 // -(G^T lambda)_e . dx over the tet's four vertices: the multiplier part of
 // the vertices' slopes, summed per tet (each G record read once, whole)
-__device__ double tet_lambda_slope(const mfem_solver& S, int e) {
+__device__ __forceinline__ double tet_lambda_slope(const mfem_solver& S, int e) {
   const int4 tv = reinterpret_cast<const int4*>(S.tets)[e];
   const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
   float lam[6];
@@ -3019,7 +3043,7 @@ __device__ double tet_lambda_slope(const mfem_solver& S, int e) {
   return acc;
 }
 
-__device__ double tet_slope(const mfem_solver& S, int e) {
+__device__ __forceinline__ double tet_slope(const mfem_solver& S, int e) {
   double acc = 0.0;
 #pragma unroll
   for (int k = 0; k < 6; ++k)
@@ -3034,7 +3058,7 @@ __device__ double tet_slope(const mfem_solver& S, int e) {
 // with the new multipliers: the merit at alpha = 0 and at a1, and the slope
 // (g_s + SYM lambda) . ds - (G^T lambda) . dx_e -- each formed as tet_merit,
 // tet_slope and tet_lambda_slope form them from the stored records.
-__device__ void tet_local_merit(const mfem_solver& S, int e, float a1, double& f0, double& sl, double& fa) {
+__device__ __forceinline__ void tet_local_merit(const mfem_solver& S, int e, float a1, double& f0, double& sl, double& fa) {
   float xv[4][3];
   int4 tv;
   gather_tet(S, e, xv, tv);
@@ -3543,9 +3567,7 @@ int mfem_shell_refresh(const mfem_solver* S, void* stream) {
 int mfem_assemble(const mfem_solver* S, float dt, void* stream) {
   if (int rc = check_solver(S)) return rc;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_assemble, dim3(blocks_for((long)S->n_slots * 64)), dim3(kBlock), 0, st, *S);
-  if (int rc = launched()) return rc;
-  hipLaunchKernelGGL(k_rhs, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, st, *S, dt);
+  hipLaunchKernelGGL(k_assemble, dim3(blocks_for((long)S->n_slots * 64)), dim3(kBlock), 0, st, *S, dt);
   return launched();
 }
 

@@ -4,7 +4,7 @@
 //
 // Per step (mfem_step): candidates from the static boxes at the start;
 // per Newton iteration: the pairs' gradient and diagonal Hessian blocks
-// into sc_g / sc_h (added to g and H by k_rhs), the PCG, the local solve,
+// into sc_g / sc_h (added to g and H by k_assemble), the PCG, the local solve,
 // candidates again on the boxes swept by the Newton step dx (so that they
 // hold every pair within dhat anywhere on x + a dx, a in [0, 1], and at the
 // next iteration's x), the CCD bound, then the line search with the pairs'
