@@ -97,3 +97,85 @@ def test_driven_octopus_full_size_with_refinement(gpu):
           "max row width", solver._S.max_width)
     assert len(fired) == 2 and solver.active_tet_count > T0
     assert any(v["applicable"] and v["resident_slots"] >= solver._S.max_width for v in solver.pcg_variants())
+
+
+def test_driven_octopus_with_shell_matches_fp64_oracle(gpu):
+    """The membrane shell on (shell_mu = bulk mu x 2 mm, the reference
+    driver's default, sim_octopus.py:202-209, 481-483) together with
+    self-contact and the tip drive: three steps against the FP64 oracle."""
+    m, drive = scene.octopus_drive_scene(h=12e-3)
+    mu_s = float(np.median(m.tet_materials[:, 0])) * 2e-3
+    solver, state = _solver(m, gpu, refine=False, shell_mu=mu_s)
+    o = Oracle(m, 64, params={"shell_mu": mu_s}, threads=_threads())
+    out = state.clone()
+    idx = torch.as_tensor(drive.indices, device=gpu)
+    dt = 0.01
+    n = m.particle_count
+    for k in range(3):
+        tgt = drive.targets((k + 1) * dt)
+        solver.set_attachment_targets(idx, torch.as_tensor(tgt, dtype=torch.float32, device=gpu))
+        o.set_attachment_targets(drive.indices, tgt.astype(np.float32))
+        solver.step(state, out, None, None, dt)
+        state, out = out, state
+        o.step(dt)
+        st = solver.last_stats()
+        assert np.all(st[:, 1] <= 1e-6), st[:, :2]
+        assert solver.status() == 0
+    x = state.particle_q[:n].double().cpu().numpy()
+    err = np.abs(x - o.x).max() / np.abs(o.x).max()
+    print("shell + self-contact + drive: rel err", err, flush=True)
+    assert err <= 1e-4
+
+
+def test_driven_octopus_recorded_through_refinement(gpu, tmp_path):
+    """The driven octopus with refinement every 3 steps, the shell on, each
+    frame captured by the recorder: the recording holds ragged frames (the
+    mesh grows), reads back without unpickling equal to the solver's state,
+    and its last frame scores zero sim -> tracked against itself."""
+    from mfem_amd.driver import MeshRecorder, load_recording
+    from mfem_amd import surface_loss
+
+    m, drive = scene.octopus_drive_scene(h=12e-3)
+    m.refine_every = 3
+    mu_s = float(np.median(m.tet_materials[:, 0])) * 2e-3
+    solver, state = _solver(m, gpu, refine=True, shell_mu=mu_s)
+    out = state.clone()
+    idx = torch.as_tensor(drive.indices, device=gpu)
+    dt = 0.01
+    rec = MeshRecorder(tmp_path / "octo.npz", 100, capsule_radius=0.0, capsule_half_height=0.0)
+    sizes = []
+    for k in range(7):
+        tgt = drive.targets((k + 1) * dt)
+        solver.set_attachment_targets(idx, torch.as_tensor(tgt, dtype=torch.float32, device=gpu))
+        solver.step(state, out, None, None, dt)
+        state, out = out, state
+        assert np.all(solver.last_stats()[:, 1] <= 1e-6) and solver.status() == 0
+        rec.capture(solver, state, (k + 1) * dt)
+        sizes.append((solver.active_particle_count, solver.active_tet_count))
+    info = rec.close()
+    print("frames", info, "sizes", sizes)
+    r = load_recording(tmp_path / "octo.npz")
+    assert r["frame_count"] == 7
+    nv, nt = sizes[-1]
+    np.testing.assert_array_equal(r["positions_at"](6), state.particle_q[:nv].cpu().numpy().astype(np.float64))
+    np.testing.assert_array_equal(r["tets_at"](6), solver.tet_indices[:nt].cpu().numpy())
+    if len(set(sizes)) > 1:
+        assert info["positions_ragged"] and info["tets"] == "ragged"
+    # the last frame scored with itself as the reference surface
+    pts, tets = r["positions_at"](6), r["tets_at"](6)
+    tris = surface_loss.surface_triangles_from_tets(tets)
+
+    class _Overlay:
+        active, valid = True, None
+        faces = tris
+        traj = pts[None]
+
+        def positions(self, frame):
+            return pts
+
+        def frame_for_time(self, t):
+            return 0
+
+    surf = np.unique(tris)
+    res = surface_loss.TrackedSurfaceLoss(_Overlay(), device=gpu).evaluate(pts[surf], frame=0)
+    assert res["mse"] == pytest.approx(0.0, abs=1e-18) and res["n_sim"] == len(surf)
