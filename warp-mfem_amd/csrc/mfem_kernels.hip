@@ -3,9 +3,10 @@
 //
 // Kernel map (one launch each unless noted; all use 256-thread workgroups):
 //   k_begin_step   per vertex   predictor, friction anchor, lagged normal force
-//   k_vertex       per vertex   kinetic/attachment gradient, barrier, friction
 //   k_element      per tet      F, SVD, c, G, energy, PSD Hessian, Schur blocks,
-//                               12x12 stiffness (10 unique 3x3 blocks) and force
+//                               12x12 stiffness (10 unique 3x3 blocks) and force;
+//                  per vertex   (its last workgroups) kinetic/attachment
+//                               gradient, barrier, friction, shell gradient
 //   k_assemble     per lane-slot of the SELL-64 matrix: deterministic gather of
 //                               the tet blocks that land on that block; the
 //                               right-hand side (gather of tet forces) and the
@@ -447,10 +448,9 @@ __global__ void __launch_bounds__(kBlock) k_begin_step(mfem_solver S, float dt) 
   S.lag_sid[i] = s;
 }
 
-__global__ void __launch_bounds__(kBlock) k_vertex(mfem_solver S, float dt) {
-  if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= S.n_verts) return;
+// per vertex: kinetic + attachment gradient, vertex and triangle barrier,
+// friction, shell gradient (run by the element kernel's vertex workgroups)
+__device__ __forceinline__ void vertex_update(const mfem_solver& S, int i, float dt) {
   const float idt2 = 1.f / (dt * dt);
   float x[3], xt[3];
   load3(S.x, i, x);
@@ -628,8 +628,14 @@ __device__ __forceinline__ void tet_element(const mfem_solver& S, int e, const f
 // G^T H_l g_l) and the energy, the local solve then running inside the line
 // search (k_linesearch, fused_local).
 template <bool FULL>
-__global__ void __launch_bounds__(kBlock, 2) k_element(mfem_solver S) {
+__global__ void __launch_bounds__(kBlock, 2) k_element(mfem_solver S, float dt, int n_tet_blocks) {
   if (S.status[MFEM_STATUS_ERRBITS] != 0) return;  // a grid barrier timed out: the step is void
+  if ((int)blockIdx.x >= n_tet_blocks) {
+    // the vertex workgroups (independent of the tets' work)
+    const int i = ((int)blockIdx.x - n_tet_blocks) * kBlock + threadIdx.x;
+    if (i < S.n_verts) vertex_update(S, i, dt);
+    return;
+  }
   // per-tet outputs leave through wave_store; lanes past the last tet compute
   // the last tet again and store nothing
   constexpr int kSlab = FULL ? kSlabFloats : 64 * 13;
@@ -3537,15 +3543,15 @@ int mfem_update_system(const mfem_solver* S, float dt, void* stream) {
     hipLaunchKernelGGL(k_tri_contact, dim3(blocks_for(S->n_tris)), dim3(kBlock), 0, st, *S);
     if (int rc = launched()) return rc;
   }
-  hipLaunchKernelGGL(k_vertex, dim3(blocks_for(S->n_verts)), dim3(kBlock), 0, st, *S, dt);
-  if (int rc = launched()) return rc;
   if (S->self_contact)
     if (int rc = mfem_self_contact_eval(S, stream))
       return fail(rc, std::string("self-contact: ") + mfem_self_contact_last_error());
+  // one launch: the tets' element work, then the vertices' terms
+  const int nt = (int)blocks_for(S->n_tets), nv = (int)blocks_for(S->n_verts);
   if (S->fused_local)
-    hipLaunchKernelGGL(k_element<false>, dim3(blocks_for(S->n_tets)), dim3(kBlock), 0, st, *S);
+    hipLaunchKernelGGL(k_element<false>, dim3(nt + nv), dim3(kBlock), 0, st, *S, dt, nt);
   else
-    hipLaunchKernelGGL(k_element<true>, dim3(blocks_for(S->n_tets)), dim3(kBlock), 0, st, *S);
+    hipLaunchKernelGGL(k_element<true>, dim3(nt + nv), dim3(kBlock), 0, st, *S, dt, nt);
   return launched();
 }
 
