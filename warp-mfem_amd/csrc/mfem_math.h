@@ -45,28 +45,32 @@ MFEM_DEV void inv3(const float* A, float* B) {
 // al. 2011, "Computing the singular value decomposition of 3x3 matrices with
 // minimal branching" (Jacobi + sort + QR), with plain Givens rotations.
 // ---------------------------------------------------------------------------
+// One Jacobi rotation of the symmetric A on (p, q), r the third index
+// (Golub & Van Loan, Algorithm 8.4.1): tau = (a_qq - a_pp) / (2 a_pq),
+// t = sign(tau) / (|tau| + sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2), s = t c;
+// then a_pp -= t a_pq, a_qq += t a_pq, a_pq = 0, (a_rp, a_rq) rotated, and
+// V's columns p, q rotated.  The reciprocals are the hardware's (1 ulp):
+// an inexact t still gives an orthogonal rotation (c, s come from the same
+// t), and what it leaves of a_pq is below FP32 rounding of the diagonal.
 MFEM_DEV void jacobi_rot(float* A, float* V, int p, int q) {
-  float apq = A[p * 3 + q];
-  float app = A[p * 3 + p], aqq = A[q * 3 + q];
-  float cs = 1.f, sn = 0.f;
-  if (apq != 0.f) {
-    float theta = (aqq - app) / (2.f * apq);
-    float t = copysignf(1.f, theta) / (fabsf(theta) + sqrtf(fmaf(theta, theta, 1.f)));
-    cs = rsqrtf(fmaf(t, t, 1.f));
-    sn = t * cs;
-  }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float akp = A[k * 3 + p], akq = A[k * 3 + q];
-    A[k * 3 + p] = cs * akp - sn * akq;
-    A[k * 3 + q] = sn * akp + cs * akq;
-  }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float apk = A[p * 3 + k], aqk = A[q * 3 + k];
-    A[p * 3 + k] = cs * apk - sn * aqk;
-    A[q * 3 + k] = sn * apk + cs * aqk;
-  }
+  const int r = 3 - p - q;
+  const float apq = A[p * 3 + q];
+  // (the hardware reciprocal flushes denormals: 2 a_pq below the normal
+  // range would give 0 * inf when a_pp == a_qq; such a rotation is a no-op
+  // at FP32 precision anyway)
+  if (!(fabsf(apq) > 1e-30f)) return;
+  const float app = A[p * 3 + p], aqq = A[q * 3 + q];
+  const float tau = (aqq - app) * __builtin_amdgcn_rcpf(2.f * apq);
+  const float t = copysignf(1.f, tau) * __builtin_amdgcn_rcpf(fabsf(tau) + sqrtf(fmaf(tau, tau, 1.f)));
+  const float cs = rsqrtf(fmaf(t, t, 1.f));
+  const float sn = t * cs;
+  A[p * 3 + p] = fmaf(-t, apq, app);
+  A[q * 3 + q] = fmaf(t, apq, aqq);
+  A[p * 3 + q] = A[q * 3 + p] = 0.f;
+  const float arp = A[r * 3 + p], arq = A[r * 3 + q];
+  const float nrp = cs * arp - sn * arq, nrq = sn * arp + cs * arq;
+  A[r * 3 + p] = A[p * 3 + r] = nrp;
+  A[r * 3 + q] = A[q * 3 + r] = nrq;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     float vkp = V[k * 3 + p], vkq = V[k * 3 + q];
