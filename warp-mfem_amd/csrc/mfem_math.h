@@ -38,8 +38,10 @@ MFEM_DEV void inv3(const float* A, float* B) {
 }
 
 // ---------------------------------------------------------------------------
-// 3x3 SVD, rotation variant.  Cyclic Jacobi on F^T F (5 fixed sweeps), sort
-// descending, then QR of B = F V by Gram-Schmidt: U and V are rotations and
+// 3x3 SVD, rotation variant.  Cyclic Jacobi on F^T F (4 fixed sweeps: the
+// off-diagonal is below FP32 resolution after the fourth on deformation
+// gradients from near-rest to 100 % strain, and a fifth changes nothing),
+// sort descending, then QR of B = F V by Gram-Schmidt: U and V are rotations and
 // the sign of det F lands on the smallest singular value, the convention of
 // rotational_svd (src/mfem/refinement/utils.py:46-67).  Method: McAdams et
 // al. 2011, "Computing the singular value decomposition of 3x3 matrices with
@@ -100,7 +102,7 @@ MFEM_DEV void svd3(const float* F, float* U, float* sig, float* V) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0) ? 1.f : 0.f;
 #pragma unroll
-  for (int sweep = 0; sweep < 5; ++sweep) {
+  for (int sweep = 0; sweep < 4; ++sweep) {
     jacobi_rot(A, V, 0, 1);
     jacobi_rot(A, V, 0, 2);
     jacobi_rot(A, V, 1, 2);
