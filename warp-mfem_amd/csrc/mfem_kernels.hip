@@ -2087,7 +2087,8 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       // with (C^T H C) a = C^T g.  It is never worse than the best single
       // candidate.  Gram entries from one SpMV per candidate (the histories
       // were written by the previous launch, so only the learned guess is
-      // published), seven 4-value grid reductions, a 6 x 6 Cholesky that
+      // published), seven 4-value grid reductions (five, overlapped with the
+      // SpMVs, without a learned guess), a 6 x 6 Cholesky that
       // drops candidates already in the span of the earlier ones (pivot <=
       // 1e-10 of the diagonal: equal or empty histories), then r0 = g - H x0.
       const double* h0 = S.dx_hist;
@@ -2128,17 +2129,6 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       for (int k = 0; k < CPW; ++k)
 #pragma unroll
         for (int a = 0; a < 3; ++a) rrp += r[k][a] * r[k][a];
-      {
-        double* gb = pxbuf(pubx);
-#pragma unroll
-        for (int k = 0; k < CPW; ++k) {
-          if (!own[k]) continue;
-          const int row = chunk[k] * 64 + lane;
-#pragma unroll
-          for (int a = 0; a < 3; ++a) pub_store(&gb[3 * row + a], tagx(cget(1, k, a), pubx));
-        }
-        ++pubx;
-      }
       // Gram entries (upper triangle) and right-hand sides, reduced in
       // groups of four between the SpMVs and kept in LDS (uniform)
       __shared__ double gal[28];
@@ -2153,52 +2143,115 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       // gal index of G_ij (i <= j) and of f_i
       auto gi = [](int i, int j) { return i * 6 - i * (i - 1) / 2 + (j - i); };
       constexpr int F0 = 21;
-      {
+      if (!S.guess) {
+        // No learned guess (c_1 = 0: its Gram row and right-hand side are
+        // zero and the Cholesky drops it).  Nothing is published (the
+        // histories were written by the previous launch), so the reductions
+        // carry values only and each candidate's SpMV runs inside the round
+        // trip of the reduction before it.
         double y[CPW][3];  // H c_0 = g - r
         reload_b();
 #pragma unroll
         for (int k = 0; k < CPW; ++k)
 #pragma unroll
           for (int a = 0; a < 3; ++a) y[k][a] = live[k] ? (double)b[k][a] - r[k][a] : 0.0;
-        double P[4] = {cdot(0, y), cdot(1, y), cdot(2, y), cdot(3, y)};
-        double Q[4] = {cdot(4, y), cdot(5, y), gdot(0), gdot(1)};
-        double Rr[4] = {gdot(2), gdot(3), gdot(4), gdot(5)};
-        if (!grid_reduce(gs, P)) return;  // (also publishes the learned guess)
-        keep4(P, gi(0, 0), gi(0, 1), gi(0, 2), gi(0, 3));
-        if (!grid_reduce(gs, Q)) return;
-        keep4(Q, gi(0, 4), gi(0, 5), F0 + 0, F0 + 1);
-        if (!grid_reduce(gs, Rr)) return;
-        keep4(Rr, F0 + 2, F0 + 3, F0 + 4, F0 + 5);
-      }
-      {
-        // (each grid reduction's workgroup barrier also frees the staged
-        // SpMV's line buffer for the next SpMV)
-        double y[CPW][3];
-        spmv_check64(pxbuf(pubx - 1), y);  // H c_1
-        double A4[4] = {cdot(1, y), cdot(2, y), cdot(3, y), cdot(4, y)};
-        double g15 = cdot(5, y);
-        if (!grid_reduce(gs, A4)) return;
-        keep4(A4, gi(1, 1), gi(1, 2), gi(1, 3), gi(1, 4));
+        const double g05 = cdot(5, y);
+        const double f0 = gdot(0), f2 = gdot(2), f3 = gdot(3), f4 = gdot(4), f5 = gdot(5);
+        double R1[4] = {cdot(0, y), cdot(2, y), cdot(3, y), cdot(4, y)};
+        grid_post(gs, R1, false);
         spmv_check64(S.dx, y);  // H c_2
-        double B4[4] = {g15, cdot(2, y), cdot(3, y), cdot(4, y)};
-        double g25 = cdot(5, y);
-        if (!grid_reduce(gs, B4)) return;
-        keep4(B4, gi(1, 5), gi(2, 2), gi(2, 3), gi(2, 4));
+        const double g22 = cdot(2, y), g23 = cdot(3, y), g24 = cdot(4, y), g25 = cdot(5, y);
+        if (!grid_wait(gs, R1)) return;
+        keep4(R1, gi(0, 0), gi(0, 2), gi(0, 3), gi(0, 4));
+        double R2[4] = {g05, f0, f2, f3};
+        grid_post(gs, R2, false);
         spmv_check64(h0, y);  // H c_3
-        double C4[4] = {g25, cdot(3, y), cdot(4, y), cdot(5, y)};
-        if (!grid_reduce(gs, C4)) return;
-        keep4(C4, gi(2, 5), gi(3, 3), gi(3, 4), gi(3, 5));
+        const double g33 = cdot(3, y), g34 = cdot(4, y), g35 = cdot(5, y);
+        if (!grid_wait(gs, R2)) return;
+        keep4(R2, gi(0, 5), F0 + 0, F0 + 2, F0 + 3);
+        double R3[4] = {f4, f5, g22, g23};
+        grid_post(gs, R3, false);
         spmv_check64(h1, y);  // H c_4
-        double g44 = cdot(4, y), g45 = cdot(5, y);
-        __syncthreads();
+        const double g44 = cdot(4, y), g45 = cdot(5, y);
+        if (!grid_wait(gs, R3)) return;
+        keep4(R3, F0 + 4, F0 + 5, gi(2, 2), gi(2, 3));
+        double R4[4] = {g24, g25, g33, g34};
+        grid_post(gs, R4, false);
         double g55 = 0.0;
         if (has5) {
           spmv_check64(h5, y);  // H c_5
           g55 = cdot(5, y);
         }
-        double D4[4] = {g44, g45, g55, 0.0};
-        if (!grid_reduce(gs, D4)) return;
-        keep4(D4, gi(4, 4), gi(4, 5), gi(5, 5), 27);
+        if (!grid_wait(gs, R4)) return;
+        keep4(R4, gi(2, 4), gi(2, 5), gi(3, 3), gi(3, 4));
+        double R5[4] = {g35, g44, g45, g55};
+        if (!grid_reduce(gs, R5)) return;
+        keep4(R5, gi(3, 5), gi(4, 4), gi(4, 5), gi(5, 5));
+        if (threadIdx.x == 0) {
+          gal[gi(0, 1)] = 0.0;
+#pragma unroll
+          for (int j = 1; j < 6; ++j) gal[gi(1, j)] = 0.0;
+          gal[F0 + 1] = 0.0;
+        }
+      } else {
+        {
+          double* gb = pxbuf(pubx);
+#pragma unroll
+          for (int k = 0; k < CPW; ++k) {
+            if (!own[k]) continue;
+            const int row = chunk[k] * 64 + lane;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) pub_store(&gb[3 * row + a], tagx(cget(1, k, a), pubx));
+          }
+          ++pubx;
+        }
+        {
+          double y[CPW][3];  // H c_0 = g - r
+          reload_b();
+#pragma unroll
+          for (int k = 0; k < CPW; ++k)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) y[k][a] = live[k] ? (double)b[k][a] - r[k][a] : 0.0;
+          double P[4] = {cdot(0, y), cdot(1, y), cdot(2, y), cdot(3, y)};
+          double Q[4] = {cdot(4, y), cdot(5, y), gdot(0), gdot(1)};
+          double Rr[4] = {gdot(2), gdot(3), gdot(4), gdot(5)};
+          if (!grid_reduce(gs, P)) return;  // (also publishes the learned guess)
+          keep4(P, gi(0, 0), gi(0, 1), gi(0, 2), gi(0, 3));
+          if (!grid_reduce(gs, Q)) return;
+          keep4(Q, gi(0, 4), gi(0, 5), F0 + 0, F0 + 1);
+          if (!grid_reduce(gs, Rr)) return;
+          keep4(Rr, F0 + 2, F0 + 3, F0 + 4, F0 + 5);
+        }
+        {
+          // (each grid reduction's workgroup barrier also frees the staged
+          // SpMV's line buffer for the next SpMV)
+          double y[CPW][3];
+          spmv_check64(pxbuf(pubx - 1), y);  // H c_1
+          double A4[4] = {cdot(1, y), cdot(2, y), cdot(3, y), cdot(4, y)};
+          double g15 = cdot(5, y);
+          if (!grid_reduce(gs, A4)) return;
+          keep4(A4, gi(1, 1), gi(1, 2), gi(1, 3), gi(1, 4));
+          spmv_check64(S.dx, y);  // H c_2
+          double B4[4] = {g15, cdot(2, y), cdot(3, y), cdot(4, y)};
+          double g25 = cdot(5, y);
+          if (!grid_reduce(gs, B4)) return;
+          keep4(B4, gi(1, 5), gi(2, 2), gi(2, 3), gi(2, 4));
+          spmv_check64(h0, y);  // H c_3
+          double C4[4] = {g25, cdot(3, y), cdot(4, y), cdot(5, y)};
+          if (!grid_reduce(gs, C4)) return;
+          keep4(C4, gi(2, 5), gi(3, 3), gi(3, 4), gi(3, 5));
+          spmv_check64(h1, y);  // H c_4
+          double g44 = cdot(4, y), g45 = cdot(5, y);
+          __syncthreads();
+          double g55 = 0.0;
+          if (has5) {
+            spmv_check64(h5, y);  // H c_5
+            g55 = cdot(5, y);
+          }
+          double D4[4] = {g44, g45, g55, 0.0};
+          if (!grid_reduce(gs, D4)) return;
+          keep4(D4, gi(4, 4), gi(4, 5), gi(5, 5), 27);
+        }
       }
       __syncthreads();
       double Gm[6][6], fv[6];
