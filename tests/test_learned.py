@@ -235,7 +235,7 @@ def _galerkin_host(H, g, cands, thr=1e-10):
     return C @ a, margin
 
 
-@pytest.mark.parametrize("jitter", [0.0, 0.05, 3.0])
+@pytest.mark.parametrize("jitter", [None, 0.0, 0.05, 3.0])
 def test_galerkin_start_matches_host(gpu, jitter):
     """The PCG starts from the lowest quadratic-model energy over the span of
     x~ - x, the learned guess, the last three Newton steps and the same Newton
@@ -244,10 +244,15 @@ def test_galerkin_start_matches_host(gpu, jitter):
     (stats[15]) equals the one of the same Galerkin start computed here in
     FP64 on the host from the assembled matrix and the solver's buffers, and
     that start's energy is at most every candidate's; nets from exact
-    previous-step (zero output layer) to far off."""
+    previous-step (zero output layer) to far off, and no network at all (the
+    kernel's schedule without a learned guess: that candidate is zero)."""
     m = scene.drop_box_scene()
-    net = _net(gpu, seed=5, jitter=jitter)
-    solver, state = _make(m, gpu, warm_start=net)
+    if jitter is None:
+        solver, state = _make(m, gpu)
+        solver.set_galerkin_start(True)
+    else:
+        net = _net(gpu, seed=5, jitter=jitter)
+        solver, state = _make(m, gpu, warm_start=net)
     out = state.clone()
     for k in range(2):
         solver.step(state, out, None, None, 0.01)
@@ -262,14 +267,16 @@ def test_galerkin_start_matches_host(gpu, jitter):
     for it in range(solver.iterations):
         solver.update_system(0.01)
         solver.assemble(0.01)
-        solver.warm_start(0.01, it)
+        if jitter is not None:
+            solver.warm_start(0.01, it)
         torch.cuda.synchronize()
         H = solver.system_csr().tocsr()
         g = b["g"][:n].double().cpu().numpy().reshape(-1)
         hist = b["dx_hist"].cpu().numpy()
         same = b["dx_same"].cpu().numpy()[3 * rp * it: 3 * rp * it + 3 * n]
         c0 = (b["x_tilde"][:n].double() - out.particle_q[:n].double()).cpu().numpy().reshape(-1)
-        cands = [c0, b["guess"][:n].double().cpu().numpy().reshape(-1), b["dx"][:n].cpu().numpy().reshape(-1),
+        guess = np.zeros_like(c0) if jitter is None else b["guess"][:n].double().cpu().numpy().reshape(-1)
+        cands = [c0, guess, b["dx"][:n].cpu().numpy().reshape(-1),
                  hist[: 3 * n], hist[3 * rp: 3 * rp + 3 * n], same]
         x0, margin = _galerkin_host(H, g, cands)
         phi = lambda c: 0.5 * c @ (H @ c) - g @ c
