@@ -1607,15 +1607,18 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
           A[i][j] = ac[sym21(i, j)];
           L[i][j] = 0.0;
         }
+      // (one division per pivot: rl = 1 / L_jj, products elsewhere)
+      double rl[6];
       for (int j = 0; j < 6; ++j) {
         double dj = A[j][j];
         for (int q = 0; q < j; ++q) dj -= L[j][q] * L[j][q];
         kept[j] = A[j][j] > 0.0 && dj > 1e-9 * A[j][j];
         L[j][j] = kept[j] ? sqrt(dj) : 1.0;
+        rl[j] = 1.0 / L[j][j];
         for (int i = j + 1; i < 6; ++i) {
           double v = A[j][i];
           for (int q = 0; q < j; ++q) v -= L[i][q] * L[j][q];
-          L[i][j] = kept[j] ? v / L[j][j] : 0.0;
+          L[i][j] = kept[j] ? v * rl[j] : 0.0;
         }
       }
       for (int i = 0; i < 6; ++i)  // a dropped row takes no part either
@@ -1625,7 +1628,7 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         for (int i = 0; i < 6; ++i) {
           double v = (i == c) ? 1.0 : 0.0;
           for (int q = 0; q < i; ++q) v -= L[i][q] * Li[q][c];
-          Li[i][c] = v / L[i][i];
+          Li[i][c] = v * rl[i];
         }
       for (int i = 0; i < 6; ++i)
         for (int j = i; j < 6; ++j) {
@@ -1988,7 +1991,11 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
   };
 
   // (re)start from the current r.  Standard: z = M r published, (r.z, r.r)
-  // reduced.  Pipelined: u = M r published (FP64), then w = H u.
+  // reduced.  Pipelined: u = M r published (FP64), then w = H u (inside the
+  // reduction's round trip when every workgroup stages its SpMV input).
+  // rs_extra: one more per-lane value that the restart's reduction sums (the
+  // Galerkin start's |g - H c_0|^2), 0 otherwise.
+  double rs_extra = 0.0;
   auto restart = [&](double& rz, double& rr) -> bool {
     rz = 0.0;
     rr = 0.0;
@@ -2006,9 +2013,10 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         }
       }
       ++pub;
-      double red2[1] = {rr};
+      double red2[2] = {rr, rs_extra};
       if (!grid_reduce(gs, red2)) return false;
       rr = red2[0];
+      rs_extra = red2[1];
       float y[CPW][3];
       spmv_rows<CPW, RW>(S, chunk, own, lane, rb, ub, y);
 #pragma unroll
@@ -2031,10 +2039,17 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         }
       }
       ++pubx;
-      double red2[1] = {rr};
-      if (!grid_reduce(gs, red2)) return false;
+      double red2[2] = {rr, rs_extra};
+      if (ovl) {
+        grid_post(gs, red2, false);
+        if (!spmv_tagged(ub, (pubx - 1) & 3, w)) return false;
+        if (!grid_wait(gs, red2)) return false;
+      } else {
+        if (!grid_reduce(gs, red2)) return false;
+        spmv_resident64(ub, w);
+      }
       rr = red2[0];
-      spmv_resident64(ub, w);
+      rs_extra = red2[1];
       if constexpr (COARSE) {
         coarse_coef(w[0], ccw);
 #pragma unroll
@@ -2055,10 +2070,11 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         }
       }
       ++pub;
-      double red2[2] = {rz, rr};
+      double red2[3] = {rz, rr, rs_extra};
       if (!grid_reduce(gs, red2)) return false;
       rz = red2[0];
       rr = red2[1];
+      rs_extra = red2[2];
     }
     return true;
   };
@@ -2143,6 +2159,8 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       // gal index of G_ij (i <= j) and of f_i
       auto gi = [](int i, int j) { return i * 6 - i * (i - 1) / 2 + (j - i); };
       constexpr int F0 = 21;
+      // without a learned guess: H c_2 .. H c_5, kept for the start's residual
+      double hc[4][CPW][3];
       if (!S.guess) {
         // No learned guess (c_1 = 0: its Gram row and right-hand side are
         // zero and the Cholesky drops it).  Nothing is published (the
@@ -2159,28 +2177,32 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         const double f0 = gdot(0), f2 = gdot(2), f3 = gdot(3), f4 = gdot(4), f5 = gdot(5);
         double R1[4] = {cdot(0, y), cdot(2, y), cdot(3, y), cdot(4, y)};
         grid_post(gs, R1, false);
-        spmv_check64(S.dx, y);  // H c_2
-        const double g22 = cdot(2, y), g23 = cdot(3, y), g24 = cdot(4, y), g25 = cdot(5, y);
+        spmv_check64(S.dx, hc[0]);  // H c_2
+        const double g22 = cdot(2, hc[0]), g23 = cdot(3, hc[0]), g24 = cdot(4, hc[0]), g25 = cdot(5, hc[0]);
         if (!grid_wait(gs, R1)) return;
         keep4(R1, gi(0, 0), gi(0, 2), gi(0, 3), gi(0, 4));
         double R2[4] = {g05, f0, f2, f3};
         grid_post(gs, R2, false);
-        spmv_check64(h0, y);  // H c_3
-        const double g33 = cdot(3, y), g34 = cdot(4, y), g35 = cdot(5, y);
+        spmv_check64(h0, hc[1]);  // H c_3
+        const double g33 = cdot(3, hc[1]), g34 = cdot(4, hc[1]), g35 = cdot(5, hc[1]);
         if (!grid_wait(gs, R2)) return;
         keep4(R2, gi(0, 5), F0 + 0, F0 + 2, F0 + 3);
         double R3[4] = {f4, f5, g22, g23};
         grid_post(gs, R3, false);
-        spmv_check64(h1, y);  // H c_4
-        const double g44 = cdot(4, y), g45 = cdot(5, y);
+        spmv_check64(h1, hc[2]);  // H c_4
+        const double g44 = cdot(4, hc[2]), g45 = cdot(5, hc[2]);
         if (!grid_wait(gs, R3)) return;
         keep4(R3, F0 + 4, F0 + 5, gi(2, 2), gi(2, 3));
         double R4[4] = {g24, g25, g33, g34};
         grid_post(gs, R4, false);
         double g55 = 0.0;
+#pragma unroll
+        for (int k = 0; k < CPW; ++k)
+#pragma unroll
+          for (int a = 0; a < 3; ++a) hc[3][k][a] = 0.0;
         if (has5) {
-          spmv_check64(h5, y);  // H c_5
-          g55 = cdot(5, y);
+          spmv_check64(h5, hc[3]);  // H c_5
+          g55 = cdot(5, hc[3]);
         }
         if (!grid_wait(gs, R4)) return;
         keep4(R4, gi(2, 4), gi(2, 5), gi(3, 3), gi(3, 4));
@@ -2296,40 +2318,62 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
         for (int i = j + 1; i < 6; ++i) v -= Gm[i][j] * cf[i];
         cf[j] = kept[j] ? v / Gm[j][j] : 0.0;
       }
-      // x0 = C a, published for its residual
-      double* xb = pxbuf(pubx);
-#pragma unroll
-      for (int k = 0; k < CPW; ++k) {
-        double xn[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-          xn[a] = cf[0] * cget(0, k, a) + cf[1] * cget(1, k, a) + cf[2] * cget(2, k, a) + cf[3] * cget(3, k, a) +
-                  cf[4] * cget(4, k, a) + cf[5] * cget(5, k, a);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) x[k][a] = xn[a];
-        if (own[k]) {
-          const int row = chunk[k] * 64 + lane;
-#pragma unroll
-          for (int a = 0; a < 3; ++a) pub_store(&xb[3 * row + a], tagx(x[k][a], pubx));
-        }
-      }
-      ++pubx;
-      double red0[1] = {rrp};
-      if (!grid_reduce(gs, red0)) return;
-      rrp = red0[0];
-      {
-        double y[CPW][3];
-        spmv_check64(xb, y);
+      if (!S.guess) {
+        // x0 = C a and its residual from the candidates' products:
+        // g - H x0 = g - a_0 (g - r) - sum_{i >= 2} a_i H c_i (no
+        // publication, no SpMV); |g - H c_0|^2 is summed by the restart's
+        // reduction
         reload_b();
 #pragma unroll
         for (int k = 0; k < CPW; ++k)
 #pragma unroll
-          for (int a = 0; a < 3; ++a) r[k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
+          for (int a = 0; a < 3; ++a) {
+            x[k][a] = cf[0] * cget(0, k, a) + cf[2] * cget(2, k, a) + cf[3] * cget(3, k, a) +
+                      cf[4] * cget(4, k, a) + cf[5] * cget(5, k, a);
+            const double hx = cf[0] * ((double)b[k][a] - r[k][a]) + cf[2] * hc[0][k][a] + cf[3] * hc[1][k][a] +
+                              cf[4] * hc[2][k][a] + cf[5] * hc[3][k][a];
+            r[k][a] = live[k] ? (double)b[k][a] - hx : 0.0;
+          }
         guess_used = 3;
-        guess_rrp = rrp;
+        rs_extra = rrp;
+      } else {
+        // x0 = C a, published for its residual
+        double* xb = pxbuf(pubx);
+#pragma unroll
+        for (int k = 0; k < CPW; ++k) {
+          double xn[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            xn[a] = cf[0] * cget(0, k, a) + cf[1] * cget(1, k, a) + cf[2] * cget(2, k, a) + cf[3] * cget(3, k, a) +
+                    cf[4] * cget(4, k, a) + cf[5] * cget(5, k, a);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) x[k][a] = xn[a];
+          if (own[k]) {
+            const int row = chunk[k] * 64 + lane;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) pub_store(&xb[3 * row + a], tagx(x[k][a], pubx));
+          }
+        }
+        ++pubx;
+        double red0[1] = {rrp};
+        if (!grid_reduce(gs, red0)) return;
+        rrp = red0[0];
+        {
+          double y[CPW][3];
+          spmv_check64(xb, y);
+          reload_b();
+#pragma unroll
+          for (int k = 0; k < CPW; ++k)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) r[k][a] = live[k] ? (double)b[k][a] - y[k][a] : 0.0;
+          guess_used = 3;
+          guess_rrp = rrp;
+        }
       }
     }
     if (!restart(rz, rr)) return;
+    if (guess_used == 3 && !S.guess) guess_rrp = rs_extra;
+    rs_extra = 0.0;
     // (rr: |g - H x0|^2 of the start, reduced by restart)
     if (guess_used == 3) guess_ratio = guess_rrp > 0.0 ? sqrt(rr / guess_rrp) : 0.0;
     bool first = true;
