@@ -537,9 +537,20 @@ def run_batch(args):
                       "validation": val}
         print(f"[bench] trained in {t2 - t1:.2f} s (kept epoch {hist['best_epoch']}); validation CG iterations "
               f"{val['cg_iters']}, kept {val['kept']}", flush=True)
+    # the learned legs' start: the network when validation kept it, else the
+    # Galerkin start alone (its output layer would be zero: the features,
+    # MLP and blend launches would add nothing)
+    value_net = None if not learned else (net if train_info["validation"]["kept"] == "trained" else "galerkin")
+
+    def attach(use_net):
+        if isinstance(use_net, str):
+            solver.set_galerkin_start(True)
+        else:
+            solver.set_warm_start(use_net)
+
     tune = None
     if not args.no_autotune:
-        solver.set_warm_start(net)
+        attach(value_net)
         solver.set_local_preconditioner(0, args.local_omega)
         tune = solver.autotune(state, dt, local_sweeps=_sweep_options(args))
         c = tune["chosen"]
@@ -555,7 +566,7 @@ def run_batch(args):
         steps stage by stage with events between the phases."""
         st, o = s0.clone(), s0.clone()
         solver.restore(snap0)
-        solver.set_warm_start(use_net)
+        attach(use_net)
         solver.set_local_preconditioner(sweeps, args.local_omega)
         timers: dict = {}
         hist = []
@@ -577,7 +588,7 @@ def run_batch(args):
         return wall, stats, timers, st
 
     def graph_leg(s0, use_net, sweeps=0):
-        solver.set_warm_start(use_net)
+        attach(use_net)
         solver.set_local_preconditioner(sweeps, args.local_omega)
         a, b = s0.clone(), s0.clone()
         solver.restore(snap0)
@@ -601,17 +612,17 @@ def run_batch(args):
     wall_p, st_p, _, end_p = timed_leg(s0, None)
     print(f"[bench] plain: {args.steps} steps in {wall_p:.3f} s", flush=True)
     if learned:
-        wall_e, st, timers, end_l = timed_leg(s0, net, args.local_sweeps)
+        wall_e, st, timers, end_l = timed_leg(s0, value_net, args.local_sweeps)
         print(f"[bench] learned + local preconditioner: {args.steps} steps in {wall_e:.3f} s", flush=True)
-        wall_j, st_j, _, _ = timed_leg(s0, net, 0)
+        wall_j, st_j, _, _ = timed_leg(s0, value_net, 0)
         print(f"[bench] learned, block Jacobi: {args.steps} steps in {wall_j:.3f} s", flush=True)
     else:
         wall_e, st, timers = wall_p, st_p, _
-    wall_g = graph_leg(s0, net, args.local_sweeps)
+    wall_g = graph_leg(s0, value_net, args.local_sweeps)
     print(f"[bench] learned, HIP graph replays: {args.steps} steps in {wall_g:.3f} s", flush=True)
     wall = min(wall_e, wall_g)
     # phase and PCG launch timings from an instrumented repeat of the value leg
-    _, _, timers, _ = timed_leg(s0, net, args.local_sweeps, instrumented=True)
+    _, _, timers, _ = timed_leg(s0, value_net, args.local_sweeps, instrumented=True)
 
     # per-body algorithmic bytes per CG iteration
     rp = solver.pattern.rowptr
@@ -642,6 +653,8 @@ def run_batch(args):
                            "value": f"block_jacobi + {args.local_sweeps} workgroup-local sweeps (omega "
                                     f"{args.local_omega})" if args.local_sweeps else "block_jacobi"},
         "learned_warm_start": learned,
+        "warm_start": (None if not learned else
+                       "network" if value_net is net else "galerkin_start (network not kept by validation)"),
         "reductions": "f64, per body",
         "launches_per_newton_iteration": {"k_pcg": 1, "k_linesearch": 1},
         "eager_ms_per_step": wall_e * 1e3 / args.steps,
