@@ -58,6 +58,13 @@ extern "C" {
 #define MFEM_PCG_LEARNED_GUESS 4 /* start from the lowest-energy combination of x~ - x, the learned guess, the last three Newton steps and the previous step's same iteration (needs dx_hist; with guess null the learned guess is left out and no network runs) */
 #define MFEM_PCG_COARSE 8 /* block Jacobi plus a workgroup-local rigid-mode correction (pipelined, resident rows;
                              see coarse_w): M^-1 = D^-1 + P (P^T H_aa P)^+ P^T per 256-row workgroup */
+#define MFEM_PCG_COOPERATIVE 16 /* launch the persistent kernels (PCG, line search) with
+                                   hipLaunchCooperativeKernel.  Off: ordinary launches of at most
+                                   the resident workgroup count (the kernels synchronise through
+                                   their own bounded grid barriers; a workgroup that never became
+                                   resident would end the launch with MFEM_ERR_BARRIER after the
+                                   barrier time limit).  A cooperative launch costs 11-20 us of
+                                   idle device before it starts on MI355X. */
 
 /* status[] words */
 #define MFEM_STATUS_ERRBITS 0 /* bit 0: grid barrier timed out; every kernel of a step (the persistent

@@ -297,6 +297,27 @@ def test_graph_free_repeatability(gpu):
     np.testing.assert_array_equal(res[0], res[1])
 
 
+@pytest.mark.parametrize("pcg", [{}, {"pcg_pipelined": True}, {"pcg_coarse": True, "pcg_coarse_omega": 0.75}])
+def test_cooperative_and_ordinary_launches_agree(gpu, pcg):
+    # the persistent kernels (PCG, line search) launched cooperatively or as
+    # ordinary launches within the resident workgroup count: the same kernels,
+    # so bit-identical states, CG iteration counts and line-search results
+    m = scene.drop_box_scene()
+    res = []
+    for coop in (True, False):
+        solver, state = _make(m, gpu, cooperative_launch=coop, **pcg)
+        out = state.clone()
+        stats = []
+        for k in range(3):
+            solver.step(state, out, None, None, 0.01)
+            state, out = out, state
+            stats.append(solver.last_stats()[:, :7].copy())
+        assert solver.status() == 0
+        res.append((state.particle_q.cpu().numpy().copy(), np.stack(stats)))
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    np.testing.assert_array_equal(res[0][1], res[1][1])
+
+
 def test_octopus_full_size_step(gpu):
     # full BASELINE size (~2e5 tets): size-independent properties
     m = scene.octopus_scene()

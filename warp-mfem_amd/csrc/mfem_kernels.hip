@@ -3721,7 +3721,10 @@ int mfem_global_solve(const mfem_solver* S, int newton_iter, void* stream) {
   g_last_pcg_variant = sel;
   mfem_solver Sc = *S;
   void* args[] = {(void*)&Sc, (void*)&newton_iter};
-  HIP_TRY(hipLaunchCooperativeKernel(kPcgVariants[sel].fn, dim3(nblocks), dim3(kBlock), args, 0, (hipStream_t)stream));
+  if (S->pcg_flags & MFEM_PCG_COOPERATIVE)
+    HIP_TRY(hipLaunchCooperativeKernel(kPcgVariants[sel].fn, dim3(nblocks), dim3(kBlock), args, 0, (hipStream_t)stream));
+  else
+    HIP_TRY(hipLaunchKernel(kPcgVariants[sel].fn, dim3(nblocks), dim3(kBlock), args, 0, (hipStream_t)stream));
   return launched();
 }
 
@@ -3762,8 +3765,12 @@ int mfem_line_search(const mfem_solver* S, float dt, int newton_iter, void* stre
   if (nb < 1) return fail(MFEM_EHIP, "line-search kernel cannot be resident");
   mfem_solver Sc = *S;
   void* args[] = {(void*)&Sc, (void*)&dt, (void*)&newton_iter};
-  HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_linesearch), dim3((unsigned)nb), dim3(kBlock),
-                                     args, 0, st));
+  if (S->pcg_flags & MFEM_PCG_COOPERATIVE)
+    HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_linesearch), dim3((unsigned)nb), dim3(kBlock),
+                                       args, 0, st));
+  else
+    HIP_TRY(hipLaunchKernel(reinterpret_cast<const void*>(&k_linesearch), dim3((unsigned)nb), dim3(kBlock), args, 0,
+                            st));
   return launched();
 }
 

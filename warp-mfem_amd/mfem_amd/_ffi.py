@@ -51,7 +51,7 @@ MFEM_STATS_PER_ITER = 16
 MFEM_MAX_STATS_ITERS = 64
 MFEM_RED_MAX_BLOCKS = 2048
 SHAPE_PLANE, SHAPE_CAPSULE = 0, 1
-PCG_PIPELINED, PCG_STREAM_ROWS, PCG_LEARNED_GUESS, PCG_COARSE = 1, 2, 4, 8
+PCG_PIPELINED, PCG_STREAM_ROWS, PCG_LEARNED_GUESS, PCG_COARSE, PCG_COOPERATIVE = 1, 2, 4, 8, 16
 MFEM_MLP_NPARAMS = 6288
 MFEM_SCAN_NOALLOC_MAX = 1 << 26
 MFEM_ERR_BARRIER, MFEM_ERR_CONTACT = 1, 2

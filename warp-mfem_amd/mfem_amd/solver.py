@@ -275,9 +275,12 @@ class RefinementSolver:
 
         S = _ffi.MfemSolverStruct()
         S.n_shapes = n_shapes
-        # solver-specific knobs (not reference kwargs): pipelined PCG, resident rows
+        # solver-specific knobs (not reference kwargs): pipelined PCG, resident
+        # rows, cooperative launches of the persistent kernels (off: ordinary
+        # launches within the resident workgroup count, no idle gap before them)
         S.pcg_flags = (_ffi.PCG_PIPELINED if kwargs.get("pcg_pipelined", False) else 0) | \
-                      (0 if kwargs.get("pcg_resident_rows", True) else _ffi.PCG_STREAM_ROWS)
+                      (0 if kwargs.get("pcg_resident_rows", True) else _ffi.PCG_STREAM_ROWS) | \
+                      (_ffi.PCG_COOPERATIVE if kwargs.get("cooperative_launch", False) else 0)
         S.pcg_variant = -1  # chosen from pcg_flags until autotune() picks one
         self.set_local_preconditioner(int(kwargs.get("pcg_local_sweeps", 0)),
                                       float(kwargs.get("pcg_local_omega", 0.5)), S)
