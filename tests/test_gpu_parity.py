@@ -297,6 +297,51 @@ def test_graph_free_repeatability(gpu):
     np.testing.assert_array_equal(res[0], res[1])
 
 
+@pytest.mark.parametrize("pcg,galerkin,tol,restart", [({}, False, 1e-10, True),
+                                                      ({"pcg_pipelined": True}, False, 1e-12, True),
+                                                      ({"pcg_pipelined": True}, True, 1e-12, False),
+                                                      ({"pcg_coarse": True, "pcg_coarse_omega": 0.75}, True, 1e-12,
+                                                       False)])
+def test_tight_tolerance_solves_and_restarts(gpu, pcg, galerkin, tol, restart):
+    # CG far below the default rel-res: the recurrences drift from the true
+    # residual there (FP32 directions at 1e-10, the pipelined forms' FP64
+    # recurrences near 1e-12), so the FP64 check restarts the Krylov
+    # sequence (the restart path, its SpMV inside the reduction's round trip
+    # in the pipelined forms); every solve must still end at the tolerance
+    # on the host's own FP64 residual of the assembled system
+    m = scene.drop_box_scene()
+    solver, state = _make(m, gpu, cg_tolerance=tol, cg_max_iterations=4000, **pcg)
+    if galerkin:
+        solver.set_galerkin_start(True)
+    out = state.clone()
+    solver.step(state, out, None, None, 0.01)  # (histories for the Galerkin start)
+    state, out = out, state
+    out.assign(state)
+    solver.bind(out)
+    solver.begin_step(0.01)
+    n = m.particle_count
+    restarts = 0
+    for it in range(4):
+        solver.update_system(0.01)
+        solver.assemble(0.01)
+        solver.global_solve(it)
+        st = solver.last_stats()[it]
+        dx = solver.buffers["dx"][:n].cpu().numpy().ravel()
+        g = solver.buffers["g"][:n].double().cpu().numpy().ravel()
+        H = solver.system_csr().tocsr().astype(np.float64)
+        rr = np.linalg.norm(g - H @ dx) / np.linalg.norm(g)
+        print(pcg, galerkin, "it", it, "cg", int(st[0]), "relres(dev)", st[1], "relres(host)", rr, "restarts",
+              int(st[7]), flush=True)
+        assert st[1] <= tol and rr <= 2 * tol
+        restarts += int(st[7])
+        solver.local_solve()
+        solver.line_search(0.01, it)
+    assert solver.status() == 0
+    print("restarts", restarts)
+    if restart:  # (these forms restart at this tolerance on this scene: the restart path ran)
+        assert restarts > 0
+
+
 @pytest.mark.parametrize("pcg", [{}, {"pcg_pipelined": True}, {"pcg_coarse": True, "pcg_coarse_omega": 0.75}])
 def test_cooperative_and_ordinary_launches_agree(gpu, pcg):
     # the persistent kernels (PCG, line search) launched cooperatively or as
