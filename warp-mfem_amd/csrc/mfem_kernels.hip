@@ -1506,22 +1506,26 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
     const int row = chunk[0] * 64 + lane;
     const bool lv = live[0];
     const int li = (int)threadIdx.x;  // this row's place in the aggregate (one chunk per wave)
+    // (the position and the smoothing weight loaded together)
     float xi[3] = {0.f, 0.f, 0.f};
     if (lv) {
 #pragma unroll
       for (int a = 0; a < 3; ++a) xi[a] = S.x[3 * row + a];
     }
-    double m4[4] = {xi[0], xi[1], xi[2], lv ? 1.0 : 0.0};
-    wg_sum<4>(m4, cred[cpar]);
+    const float om = lv ? (S.coarse_w ? S.coarse_w[row] : S.coarse_omega) : 0.f;
+    // centroid and RMS radius from one workgroup sum: sum |x - c|^2 =
+    // sum |x|^2 - |sum x|^2 / n (FP64 over FP32 positions; the radius only
+    // scales the rotation modes, which leaves M unchanged)
+    double m5[5] = {xi[0], xi[1], xi[2], (double)xi[0] * xi[0] + (double)xi[1] * xi[1] + (double)xi[2] * xi[2],
+                    lv ? 1.0 : 0.0};
+    wg_sum<5>(m5, cred[cpar]);
     cpar ^= 1;
-    const double cnt = m4[3];
+    const double cnt = m5[4];
     double d[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) d[a] = (lv && cnt > 0.0) ? (double)xi[a] - m4[a] / cnt : 0.0;
-    double r2[1] = {d[0] * d[0] + d[1] * d[1] + d[2] * d[2]};
-    wg_sum<1>(r2, cred[cpar]);
-    cpar ^= 1;
-    const double rad = (cnt > 0.0 && r2[0] > 0.0) ? sqrt(r2[0] / cnt) : 1.0;
+    for (int a = 0; a < 3; ++a) d[a] = (lv && cnt > 0.0) ? (double)xi[a] - m5[a] / cnt : 0.0;
+    const double r2 = cnt > 0.0 ? m5[3] - (m5[0] * m5[0] + m5[1] * m5[1] + m5[2] * m5[2]) / cnt : 0.0;
+    const double rad = (cnt > 0.0 && r2 > 0.0) ? sqrt(r2 / cnt) : 1.0;
     const float e0 = (float)(d[0] / rad), e1 = (float)(d[1] / rad), e2 = (float)(d[2] / rad);
     if (lv) rigid_basis(e0, e1, e2, Pc);
     // the aggregate's positions, then its smoothed basis, staged in hv
@@ -1539,7 +1543,6 @@ __device__ __forceinline__ void pcg_body(const mfem_solver& S, int newton_iter) 
       lpos[k] = (lv && k < cwd && (c >> 8) == (int)blockIdx.x) ? (c & 255) : -1;
     }
     __syncthreads();
-    const float om = lv ? (S.coarse_w ? S.coarse_w[row] : S.coarse_omega) : 0.f;
     if (om != 0.f) {
       float HP[3][6];
 #pragma unroll
