@@ -157,3 +157,36 @@ def test_batch_local_preconditioner(gpu):
         e = np.abs(xb - o.x).max() / np.abs(o.x).max()
         print("body", i, "rel err vs FP64 oracle", e)
         assert e <= 1e-4
+
+
+def test_batch_galerkin_start_matches_oracle(gpu):
+    # the Galerkin start without a network (what the batch bench runs when
+    # validation does not keep the warm-start network) in a batch: per-body
+    # histories and reductions, every body within the solver's bar of its
+    # own FP64 oracle run and every solve at rel-res <= 1e-6
+    from mfem_amd.batch import BatchSolver
+
+    models = _bodies()
+    params = {k: v for k, v in models[0].params.items() if k not in ("iterations", "energy")}
+    it = models[0].params["iterations"]
+    bs = BatchSolver(models, it, device=gpu, **params)
+    bs.solver.set_galerkin_start(True)
+    a, b = bs.make_state(), bs.make_state()
+    dt = 0.01
+    steps = 5
+    for k in range(steps):
+        bs.step(a, b, dt)
+        a, b = b, a
+        assert bs.status() == 0
+        for i in range(bs.n_bodies):
+            s_ = bs.body_stats(i)
+            assert np.all(s_[:, 1] <= 1e-6), s_
+            assert np.all(s_[:, 14] == 3)  # the Galerkin start ran
+    for i, m in enumerate(models):
+        xb = a.particle_q[bs.body_slice(i)].double().cpu().numpy()
+        o = Oracle(m, 64, params=dict(params, energy=m.params["energy"]))
+        for k in range(steps):
+            o.step(dt)
+        e_orc = np.abs(xb - o.x).max() / np.abs(o.x).max()
+        print("body", i, "rel err vs FP64 oracle", e_orc)
+        assert e_orc <= 1e-4
