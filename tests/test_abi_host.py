@@ -32,6 +32,21 @@ def test_struct_layout_matches_header():
     _ffi.check_layout()
 
 
+def test_constants_match_header():
+    # every #define the ctypes mirror restates has the header's value
+    text = open(os.path.join(ROOT, "include", "mfem_amd.h")).read()
+    defs = {k: int(v, 0) for k, v in re.findall(r"^#define\s+(MFEM_\w+)\s+\(?(-?(?:0x)?[0-9A-Fa-f]+)",
+                                                 text, flags=re.M)}
+    mirror = {"MFEM_STATS_PER_ITER": _ffi.MFEM_STATS_PER_ITER, "MFEM_MAX_STATS_ITERS": _ffi.MFEM_MAX_STATS_ITERS,
+              "MFEM_RED_MAX_BLOCKS": _ffi.MFEM_RED_MAX_BLOCKS, "MFEM_PCG_PIPELINED": _ffi.PCG_PIPELINED,
+              "MFEM_PCG_STREAM_ROWS": _ffi.PCG_STREAM_ROWS, "MFEM_PCG_LEARNED_GUESS": _ffi.PCG_LEARNED_GUESS,
+              "MFEM_PCG_COARSE": _ffi.PCG_COARSE, "MFEM_PCG_COOPERATIVE": _ffi.PCG_COOPERATIVE,
+              "MFEM_ERR_BARRIER": _ffi.MFEM_ERR_BARRIER, "MFEM_ERR_CONTACT": _ffi.MFEM_ERR_CONTACT}
+    for k, v in mirror.items():
+        assert k in defs, k
+        assert defs[k] == v, (k, defs[k], v)
+
+
 def test_errors_are_reported_not_raised_in_c():
     import ctypes
     lib = _ffi.load()
